@@ -1,0 +1,155 @@
+"""Pin the CPU oracle to the reference's own outputs (golden vectors).
+
+Every comparison is bit-exact (``np.array_equal``): the oracle reproduces the
+reference's float32 operation order, so any dropped term, swapped index or
+reordered product shows up as a mismatch.  The fixtures were written by
+``tests/golden/make_golden.py`` from the reference package.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from golden_cases import forward_case, load, migrate_case, tags
+from oracle import engines as oeng
+from oracle import gsls as ogsls
+from oracle import numerics as onum
+from oracle import rtm as ortm
+
+
+def test_derivatives_match_reference():
+    d = load("numerics")
+    f = d["deriv_field"]
+    for order in (2, 4, 6, 8):
+        for axis in (0, 1):
+            assert np.array_equal(onum.derivative(f, axis, order, 3.7, True),
+                                  d[f"dplus_o{order}_a{axis}"])
+            assert np.array_equal(onum.derivative(f, axis, order, 3.7, False),
+                                  d[f"dminus_o{order}_a{axis}"])
+
+
+def test_cpml_profiles_match_reference():
+    d = load("numerics")
+    for k in range(3):
+        n, h, npml, dt, vmax, f0, lo, hi = d[f"cpml{k}_params"]
+        a, b, _ = onum.cpml_profile(int(n), h, int(npml), dt, vmax, f0, bool(lo), bool(hi))
+        assert np.array_equal(a, d[f"cpml{k}_a"])
+        assert np.array_equal(b, d[f"cpml{k}_b"])
+
+
+def test_gsls_fit_matches_reference():
+    rows = load("gsls")["table"]
+    for row in rows:
+        q, f_ref, n, err = row[:4]
+        tau, delta, e = ogsls.fit(q, f_ref, int(n), 1.5)
+        k = int(n)
+        np.testing.assert_allclose(tau, row[4 : 4 + k], rtol=1e-12)
+        np.testing.assert_allclose(delta, row[7 : 7 + k], rtol=1e-12)
+        assert abs(e - err) <= 1e-12
+
+
+def _oracle_prop(c, q_override=None):
+    m = ortm.Model(c["nz"], c["nx"], c["dx"], c["dz"], c["vp"], c["vs"], c["rho"], c["qp"],
+                   c["x0"], c["z0"])
+    q = None
+    if c["engine"] == "viscoacoustic":
+        q = ortm.QSpec(**(q_override or c["q"]))
+    p = ortm.build(c["engine"], m, c["dt"], c["nt"], c["order"], c["npml"], c["boundary"], q=q)
+    p.set_source(c["src"]["x"], c["src"]["z"], c["src"]["mechanism"])
+    p.set_receivers(c["rx"], c["rz"])
+    return p
+
+
+@pytest.mark.parametrize("tag", tags("forward"))
+def test_forward_records_and_states_match_reference(tag):
+    c = forward_case(tag)
+    p = _oracle_prop(c)
+    s = c["src"]
+    w = ortm.wavelet("ricker", s["f0"], s["delay"], s["amplitude"], c["dt"], c["nt"])
+    rec = ortm.forward(p, w, c["components"])
+    for comp in c["components"]:
+        assert np.array_equal(rec[comp], c["rec"][comp]), comp
+    for name, ref in c["state"].items():
+        assert np.array_equal(getattr(p, name), ref), name
+    assert set(p.psi) == set(c["psi"])
+    for name, ref in c["psi"].items():
+        assert np.array_equal(p.psi[name], ref), name
+    if c["engine"] == "viscoacoustic":
+        assert np.array_equal(p.a, c["visco_a"])
+        assert np.array_equal(p.b, c["visco_b"])
+        assert np.array_equal(p.kr, c["visco_kr"])
+        assert [p.clipped, p.lowpass_steps] == list(c["visco_summary"])
+
+
+@pytest.mark.parametrize("tag", ["ac", "el", "vi"])
+def test_sample_and_inject_match_reference(tag):
+    d = load("inject")
+    case = {"ac": "ac_cpml", "el": "el_shear", "vi": "vi_fwd"}[tag]
+    c = forward_case(case)
+    p = _oracle_prop(c)
+    p.set_receivers(d[f"{tag}/rx"], d[f"{tag}/rz"])
+    for name in p.fields:
+        getattr(p, name)[...] = d[f"{tag}/before_{name}"]
+    comps = [str(v) for v in d[f"{tag}/components"]]
+    for comp in comps:
+        assert np.array_equal(p.sample((comp,))[comp], d[f"{tag}/sample_{comp}"]), comp
+    for comp in comps:
+        p.inject(comp, d[f"{tag}/values_{comp}"])
+    for name in p.fields:
+        assert np.array_equal(getattr(p, name), d[f"{tag}/after_{name}"]), name
+
+
+def test_replay_blocks_match_reference():
+    d = load("replay")
+    c = forward_case("ac_cpml")
+    p = _oracle_prop(c)
+    s = c["src"]
+    w = ortm.wavelet("ricker", s["f0"], s["delay"], s["amplitude"], c["dt"], c["nt"])
+    st = ortm.record(p, w, 1.0)
+    assert st.interval == int(d["interval"])
+    assert [x for x, _ in st.states] == list(d["starts"])
+    for b in (0, 3, len(st.states) - 1):
+        fields = ortm.replay(p, w, st, b, "stress")
+        assert np.array_equal(np.stack([f["P"] for f in fields]), d[f"block{b}"])
+
+
+def migrate_oracle(c, dtype="float32"):
+    m = ortm.Model(c["nz"], c["nx"], c["dx"], c["dz"], c["vp"], c["vs"], c["rho"], c["qp"],
+                   c["x0"], c["z0"])
+    cfg = ortm.Imaging(**c["imaging"])
+    q = None
+    q_enabled = False
+    if c["engine"] == "viscoacoustic":
+        qq = c["q"]
+        q = ortm.QSpec(f_ref=qq["f_ref"], n=qq["n"], band_ratio=qq["band_ratio"],
+                       alpha_c=qq["alpha_c"], amplitude_cap=qq["amplitude_cap"],
+                       lowpass_interval=qq["lowpass_interval"], f_max=qq["f_max"],
+                       gain_budget_db=qq["gain_budget_db"])
+        q_enabled = qq["enabled"]
+    return ortm.migrate(c["engine"], m, c["dt"], c["nt"], c["order"], c["boundary"], cfg,
+                        c["shot"], c["src"], c["rx"], c["rz"], q=q, q_enabled=q_enabled)
+
+
+@pytest.mark.parametrize("tag", tags("migrate"))
+def test_migrate_images_match_reference(tag):
+    c = migrate_case(tag)
+    img, info = migrate_oracle(c)
+    assert (info["interval"], info["n_states"]) == c["checkpoints"]
+    assert info["start_step"] == c["start_step"]
+    assert np.array_equal(img, c["image"])
+
+
+def test_forward_shot_records_feed_migrate_cases():
+    """The shot each migrate case used is the oracle's own forward record
+    (solver path: configured npml, ertm/solver.py:62-64)."""
+    c = migrate_case("el_pp_divcurl")
+    m = ortm.Model(c["nz"], c["nx"], c["dx"], c["dz"], c["vp"], c["vs"], c["rho"], c["qp"])
+    p = ortm.build("elastic", m, c["dt"], c["nt"], c["order"], c["npml"], c["boundary"])
+    p.set_source(c["src"]["x"], c["src"]["z"])
+    p.set_receivers(c["rx"], c["rz"])
+    s = c["src"]
+    w = ortm.wavelet("ricker", s["f0"], s["delay"], s["amplitude"], c["dt"], c["nt"])
+    rec = ortm.forward(p, w, c["components"])
+    for comp in c["components"]:
+        assert np.array_equal(rec[comp], c["shot"][comp])

@@ -1,0 +1,507 @@
+// C-ABI of libertm_hip.so (declared in include/ertm_hip.h): validation,
+// descriptor tables, launch chunking and the time loops.  No C++ exception
+// crosses this boundary; every entry point returns an ERTM_* status and
+// leaves a thread-local message for ertm_last_error().
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstddef>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "ertm_kernels.h"
+
+using namespace ertm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(ERTM_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+#define LAUNCHCHK(what)                                                               \
+    do {                                                                              \
+        hipError_t e_ = hipGetLastError();                                            \
+        if (e_ != hipSuccess)                                                         \
+            return fail(ERTM_EHIP, "%s launch failed: %s", what, hipGetErrorString(e_)); \
+    } while (0)
+
+struct FftCache {
+    hipfftHandle plan = 0;
+    void* buf = nullptr;
+};
+
+}  // namespace
+
+struct ertm_batch_s {
+    std::vector<ErtmProp> props;
+    std::vector<ErtmRecv> recv;
+    std::vector<ErtmImage> img;
+    ErtmProp* dprops = nullptr;
+    ErtmRecv* drecv = nullptr;
+    ErtmImage* dimg = nullptr;
+    int order = 8;
+    int max_tiles = 0, max_samp = 0, max_cells = 0, max_img = 0;
+    bool any_force = false;
+    bool all_visco = false;
+    float* scratch = nullptr;  // per propagator: peak, scale
+    std::map<std::pair<int, int>, FftCache> fft;
+    ~ertm_batch_s() {
+        if (dprops) (void)hipFree(dprops);
+        if (drecv) (void)hipFree(drecv);
+        if (dimg) (void)hipFree(dimg);
+        if (scratch) (void)hipFree(scratch);
+        for (auto& kv : fft) {
+            if (kv.second.plan) hipfftDestroy(kv.second.plan);
+            if (kv.second.buf) (void)hipFree(kv.second.buf);
+        }
+    }
+};
+
+static int check_prop(const ErtmProp& P, int k) {
+    if (P.engine < ERTM_ACOUSTIC || P.engine > ERTM_VISCO)
+        return fail(ERTM_EINVAL, "prop %d: unknown engine %d", k, P.engine);
+    if (P.order != 2 && P.order != 4 && P.order != 6 && P.order != 8)
+        return fail(ERTM_EINVAL, "prop %d: order %d not in {2,4,6,8}", k, P.order);
+    const int K = P.order / 2;
+    if (P.nzp <= 2 * K || P.nxp <= 2 * K)
+        return fail(ERTM_EINVAL, "prop %d: padded grid %dx%d too small for order %d", k, P.nzp,
+                    P.nxp, P.order);
+    if (P.ld < P.nxp) return fail(ERTM_EINVAL, "prop %d: ld %d < nxp %d", k, P.ld, P.nxp);
+    if (P.pad < K || P.nz + 2 * P.pad != P.nzp || P.nx + 2 * P.pad != P.nxp)
+        return fail(ERTM_EINVAL, "prop %d: inconsistent pad/nz/nx", k);
+    if (P.fstride < (int64_t)P.nzp * P.ld) return fail(ERTM_EINVAL, "prop %d: fstride too small", k);
+    if (P.wx < 0 || 2 * P.wx > P.nxp || P.wzt < 0 || P.wzb < 0 || P.wzt + P.wzb > P.nzp)
+        return fail(ERTM_EINVAL, "prop %d: bad CPML widths", k);
+    if (!P.state[0] || !P.state[1] || P.state[0] == P.state[1])
+        return fail(ERTM_EINVAL, "prop %d: need two distinct state buffers", k);
+    if (!P.irho || !P.ax || !P.bx || !P.az || !P.bz)
+        return fail(ERTM_EINVAL, "prop %d: missing medium or CPML arrays", k);
+    if (P.engine == ERTM_ELASTIC && (!P.lam || !P.mu))
+        return fail(ERTM_EINVAL, "prop %d: elastic needs lam and mu", k);
+    if (P.engine != ERTM_ELASTIC && !P.kappa)
+        return fail(ERTM_EINVAL, "prop %d: needs kappa", k);
+    if (P.engine == ERTM_VISCO &&
+        (P.nmech < 1 || P.nmech > ERTM_MAX_MECH || !P.qidx || !P.q_a || !P.q_delta || !P.q_tau))
+        return fail(ERTM_EINVAL, "prop %d: viscoacoustic tables missing or L out of range", k);
+    const int nf = P.engine == ERTM_ELASTIC ? 5 : (P.engine == ERTM_ACOUSTIC ? 3 : 4 + P.nmech);
+    if (P.nfields != nf) return fail(ERTM_EINVAL, "prop %d: nfields %d != %d", k, P.nfields, nf);
+    const int nterms = P.engine == ERTM_ELASTIC ? 4 : 2;
+    int64_t end = (int64_t)nf * P.fstride;
+    for (int t = 0; t < nterms; ++t) {
+        if (P.wx > 0) {
+            if (P.psix[t] < end) return fail(ERTM_EINVAL, "prop %d: psix[%d] overlaps", k, t);
+            end = P.psix[t] + (int64_t)P.nzp * 2 * P.wx;
+        }
+    }
+    for (int t = 0; t < nterms; ++t) {
+        if (P.wzt + P.wzb > 0) {
+            if (P.psiz[t] < end) return fail(ERTM_EINVAL, "prop %d: psiz[%d] overlaps", k, t);
+            end = P.psiz[t] + (int64_t)(P.wzt + P.wzb) * P.ld;
+        }
+    }
+    if (end > P.state_floats) return fail(ERTM_EINVAL, "prop %d: state buffer too small", k);
+    if (P.src_iz >= P.nzp || P.src_ix >= P.nxp || (P.src_iz < 0) != (P.src_ix < 0))
+        return fail(ERTM_EINVAL, "prop %d: source node out of range", k);
+    return ERTM_OK;
+}
+
+static int check_recv(const ErtmRecv& R, const ErtmProp& P, int k) {
+    if (R.nsamp < 0 || R.ncells < 0 || R.ncomp < 0 || R.ncomp > 8)
+        return fail(ERTM_EINVAL, "recv %d: bad counts", k);
+    if (R.nsamp > 0 && (!R.s_i0 || !R.s_j0 || !R.s_field || !R.s_rec || !R.s_col || !R.s_valid ||
+                        !R.s_tz || !R.s_tx))
+        return fail(ERTM_EINVAL, "recv %d: missing sampling tables", k);
+    if (R.ncells > 0 && (!R.cell_ptr || !R.cell_field || !R.cell_off || !R.e_comp || !R.e_rec ||
+                         !R.e_wa || (R.inject_mode == 0 && !R.e_wb)))
+        return fail(ERTM_EINVAL, "recv %d: missing injection tables", k);
+    for (int c = 0; c < R.ncomp; ++c)
+        if (!R.records[c]) return fail(ERTM_EINVAL, "recv %d: record %d missing", k, c);
+    (void)P;
+    return ERTM_OK;
+}
+
+extern "C" {
+
+int ertm_abi_version(void) { return ERTM_ABI_VERSION; }
+
+int ertm_struct_sizes(int64_t* out6) {
+    if (!out6) return fail(ERTM_EINVAL, "struct_sizes: NULL");
+    out6[0] = (int64_t)sizeof(ErtmProp);
+    out6[1] = (int64_t)sizeof(ErtmRecv);
+    out6[2] = (int64_t)sizeof(ErtmImage);
+    out6[3] = (int64_t)sizeof(ErtmRun);
+    out6[4] = (int64_t)offsetof(ErtmProp, state);
+    out6[5] = (int64_t)offsetof(ErtmRun, ctl);
+    return ERTM_OK;
+}
+
+const char* ertm_last_error(void) { return g_err.c_str(); }
+
+int ertm_device_info(int32_t* device, int32_t* n_cu, char* name, size_t name_len) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (device) *device = dev;
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (name && name_len) {
+        snprintf(name, name_len, "%s", prop.gcnArchName);
+    }
+    return ERTM_OK;
+}
+
+int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmImage* img,
+                      int32_t n, ertm_batch_t* out) {
+    if (!out || !props || n <= 0) return fail(ERTM_EINVAL, "batch_create: bad arguments");
+    *out = nullptr;
+    auto* b = new ertm_batch_s();
+    b->props.assign(props, props + n);
+    b->recv.resize(n);
+    b->img.resize(n);
+    std::memset(b->recv.data(), 0, sizeof(ErtmRecv) * n);
+    std::memset(b->img.data(), 0, sizeof(ErtmImage) * n);
+    if (recv) b->recv.assign(recv, recv + n);
+    if (img) b->img.assign(img, img + n);
+    b->order = props[0].order;
+    b->all_visco = true;
+    for (int k = 0; k < n; ++k) {
+        int rc = check_prop(b->props[k], k);
+        if (rc == ERTM_OK) rc = check_recv(b->recv[k], b->props[k], k);
+        if (rc == ERTM_OK && b->props[k].order != b->order)
+            rc = fail(ERTM_EINVAL, "batch mixes stencil orders");
+        if (rc != ERTM_OK) {
+            delete b;
+            return rc;
+        }
+        const ErtmProp& P = b->props[k];
+        b->max_tiles = std::max(b->max_tiles, tiles_of(P));
+        b->max_samp = std::max(b->max_samp, b->recv[k].nsamp);
+        b->max_cells = std::max(b->max_cells, b->recv[k].ncells);
+        b->max_img = std::max(b->max_img, P.nz * P.nx);
+        if (P.engine == ERTM_ELASTIC && P.mechanism == ERTM_SRC_FORCE_Z && P.src_iz >= 0)
+            b->any_force = true;
+        if (P.engine != ERTM_VISCO) b->all_visco = false;
+    }
+    hipError_t e1 = hipMalloc((void**)&b->dprops, sizeof(ErtmProp) * n);
+    hipError_t e2 = hipMalloc((void**)&b->drecv, sizeof(ErtmRecv) * n);
+    hipError_t e3 = hipMalloc((void**)&b->dimg, sizeof(ErtmImage) * n);
+    hipError_t e4 = hipMalloc((void**)&b->scratch, sizeof(float) * 2 * n + 64);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+        delete b;
+        return fail(ERTM_ENOMEM, "batch_create: device allocation failed");
+    }
+    hipError_t c1 = hipMemcpy(b->dprops, b->props.data(), sizeof(ErtmProp) * n, hipMemcpyHostToDevice);
+    hipError_t c2 = hipMemcpy(b->drecv, b->recv.data(), sizeof(ErtmRecv) * n, hipMemcpyHostToDevice);
+    hipError_t c3 = hipMemcpy(b->dimg, b->img.data(), sizeof(ErtmImage) * n, hipMemcpyHostToDevice);
+    if (c1 != hipSuccess || c2 != hipSuccess || c3 != hipSuccess) {
+        delete b;
+        return fail(ERTM_EHIP, "batch_create: descriptor upload failed");
+    }
+    *out = b;
+    return ERTM_OK;
+}
+
+int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img) {
+    if (!b || !img) return fail(ERTM_EINVAL, "batch_update_image: bad arguments");
+    const int n = (int)b->props.size();
+    HIPCHK(hipDeviceSynchronize());
+    b->img.assign(img, img + n);
+    HIPCHK(hipMemcpy(b->dimg, b->img.data(), sizeof(ErtmImage) * n, hipMemcpyHostToDevice));
+    return ERTM_OK;
+}
+
+int ertm_batch_destroy(ertm_batch_t b) {
+    if (!b) return ERTM_OK;
+    (void)hipDeviceSynchronize();
+    delete b;
+    return ERTM_OK;
+}
+
+int ertm_batch_set_variant(ertm_batch_t b, int32_t variant) {
+    if (!b || variant != 0) return fail(ERTM_EINVAL, "unknown kernel variant %d", variant);
+    return ERTM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// launch helpers (chunks of ERTM_MAX_LAUNCH propagators)
+// ---------------------------------------------------------------------------
+static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s) {
+    const int n = (int)b->props.size();
+    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+        LaunchArgs la{};
+        la.base = base;
+        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
+        la.cur = cur;
+        for (int k = 0; k < la.n; ++k) la.amp[k] = amp ? amp[base + k] : 0.0f;
+        launch_step(b->dprops, la, b->order, b->max_tiles, b->any_force, s);
+        LAUNCHCHK("step");
+    }
+    return ERTM_OK;
+}
+
+static int do_sample(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s) {
+    const int n = (int)b->props.size();
+    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+        LaunchArgs la{};
+        la.base = base;
+        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
+        la.cur = cur;
+        for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        launch_sample(b->dprops, b->drecv, la, b->max_samp, s);
+        LAUNCHCHK("sample");
+    }
+    return ERTM_OK;
+}
+
+static int do_inject(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s) {
+    const int n = (int)b->props.size();
+    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+        LaunchArgs la{};
+        la.base = base;
+        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
+        la.cur = cur;
+        for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        launch_inject(b->dprops, b->drecv, la, b->max_cells, s);
+        LAUNCHCHK("inject");
+    }
+    return ERTM_OK;
+}
+
+static int do_image(ertm_batch_t b, int cur, const int32_t* t, int op, hipStream_t s) {
+    const int n = (int)b->props.size();
+    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+        LaunchArgs la{};
+        la.base = base;
+        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
+        la.cur = cur;
+        la.op = op;
+        for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        launch_image(b->dprops, b->dimg, la, b->order, b->max_img, s);
+        LAUNCHCHK("image");
+    }
+    return ERTM_OK;
+}
+
+static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int interval,
+                        double* ctl, hipStream_t s) {
+    if (!b->all_visco) return fail(ERTM_EINVAL, "stabilize: batch is not viscoacoustic");
+    if (!ctl) return fail(ERTM_EINVAL, "stabilize: ctl is NULL");
+    const int n = (int)b->props.size();
+    if (n > ERTM_MAX_LAUNCH) return fail(ERTM_EINVAL, "stabilize: batch larger than %d", ERTM_MAX_LAUNCH);
+    float* peaks = b->scratch;
+    float* scales = b->scratch + n;
+    for (int k = 0; k < n; ++k) {
+        const ErtmProp& P = b->props[k];
+        launch_absmax(P.state[cur] + (int64_t)ERTM_VI_P * P.fstride, (int64_t)P.nzp * P.ld,
+                      peaks + k, s);
+        LAUNCHCHK("absmax");
+    }
+    LaunchArgs la{};
+    la.base = 0;
+    la.n = n;
+    la.cur = cur;
+    const bool lowpass_now = step_index >= 30 && interval > 1 && step_index % interval == 0;
+    launch_stab_control(b->dprops, la, peaks, ctl, step_index, cap, lowpass_now ? 1 : 0, scales, s);
+    LAUNCHCHK("stab_control");
+    if (step_index < 30) return ERTM_OK;
+    launch_stab_scale(b->dprops, la, scales, b->max_tiles, s);
+    LAUNCHCHK("stab_scale");
+    if (lowpass_now) {
+        for (int k = 0; k < n; ++k) {
+            const ErtmProp& P = b->props[k];
+            auto key = std::make_pair(P.nzp, P.nxp);
+            FftCache& fc = b->fft[key];
+            if (!fc.plan) {
+                if (hipfftPlan2d(&fc.plan, P.nzp, P.nxp, HIPFFT_Z2Z) != HIPFFT_SUCCESS)
+                    return fail(ERTM_EHIP, "hipfftPlan2d(%d,%d) failed", P.nzp, P.nxp);
+                HIPCHK(hipMalloc(&fc.buf, sizeof(double) * 2 * (size_t)P.nzp * P.nxp));
+            }
+            if (hipfftSetStream(fc.plan, s) != HIPFFT_SUCCESS)
+                return fail(ERTM_EHIP, "hipfftSetStream failed");
+            const double time = step_index * P.dt64;
+            double ratio = 0.9 - 0.4 * std::min(time / 1.0, 1.0);
+            ratio = std::max(ratio, 0.35);
+            const int fields[2] = {ERTM_VI_P, ERTM_VI_EPS};
+            for (int f : fields) {
+                float* x = P.state[cur] + (int64_t)f * P.fstride;
+                launch_lowpass_prep(x, P.nzp, P.nxp, P.ld, fc.buf, s);
+                LAUNCHCHK("lowpass_prep");
+                auto* c = (hipfftDoubleComplex*)fc.buf;
+                if (hipfftExecZ2Z(fc.plan, c, c, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
+                    return fail(ERTM_EHIP, "hipfftExecZ2Z forward failed");
+                launch_lowpass_response(fc.buf, P.nzp, P.nxp, (double)P.dx, (double)P.dz, ratio, s);
+                LAUNCHCHK("lowpass_response");
+                if (hipfftExecZ2Z(fc.plan, c, c, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
+                    return fail(ERTM_EHIP, "hipfftExecZ2Z backward failed");
+                launch_lowpass_finish(x, P.nzp, P.nxp, P.ld, fc.buf, s);
+                LAUNCHCHK("lowpass_finish");
+            }
+        }
+    }
+    return ERTM_OK;
+}
+
+extern "C" {
+
+int ertm_step(ertm_batch_t b, int32_t cur, const float* amp, void* stream) {
+    if (!b || (cur & ~1)) return fail(ERTM_EINVAL, "step: bad arguments");
+    return do_step(b, cur, amp, (hipStream_t)stream);
+}
+
+int ertm_stabilize(ertm_batch_t b, int32_t cur, int32_t step_index, double cap, int32_t interval,
+                   double* ctl, void* stream) {
+    if (!b || (cur & ~1)) return fail(ERTM_EINVAL, "stabilize: bad arguments");
+    return do_stabilize(b, cur, step_index, cap, interval, ctl, (hipStream_t)stream);
+}
+
+int ertm_sample(ertm_batch_t b, int32_t cur, const int32_t* t, void* stream) {
+    if (!b || !t || (cur & ~1)) return fail(ERTM_EINVAL, "sample: bad arguments");
+    return do_sample(b, cur, t, (hipStream_t)stream);
+}
+
+int ertm_inject(ertm_batch_t b, int32_t cur, const int32_t* t, void* stream) {
+    if (!b || !t || (cur & ~1)) return fail(ERTM_EINVAL, "inject: bad arguments");
+    return do_inject(b, cur, t, (hipStream_t)stream);
+}
+
+int ertm_image(ertm_batch_t b, int32_t cur, const int32_t* t, int32_t op, void* stream) {
+    if (!b || !t || (cur & ~1) || op < ERTM_IMG_EXTRACT || op > ERTM_IMG_EXC_APPLY)
+        return fail(ERTM_EINVAL, "image: bad arguments");
+    return do_image(b, cur, t, op, (hipStream_t)stream);
+}
+
+int ertm_copy_state(float* dst, const float* src, int64_t nfloats, void* stream) {
+    if (!dst || !src || nfloats < 0) return fail(ERTM_EINVAL, "copy_state: bad arguments");
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)nfloats, hipMemcpyDeviceToDevice,
+                          (hipStream_t)stream));
+    return ERTM_OK;
+}
+
+int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
+    if (!b || !r) return fail(ERTM_EINVAL, "run: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = (int)b->props.size();
+    if (r->t_begin < 0 || r->t_end < r->t_begin) return fail(ERTM_EINVAL, "run: bad time range");
+    if (r->phase < 0 || r->phase > 2) return fail(ERTM_EINVAL, "run: bad phase %d", r->phase);
+    if (r->phase != 2 && r->t_end > r->t_begin) {
+        if (!r->wavelets || r->nw < r->t_end)
+            return fail(ERTM_EINVAL, "run: wavelets missing or shorter than t_end");
+        for (int k = 0; k < n; ++k)
+            if (!r->wavelets[k]) return fail(ERTM_EINVAL, "run: wavelet %d is NULL", k);
+    }
+    if (r->phase == 1 && (!r->restore_arenas || r->restore_slot < 0))
+        return fail(ERTM_EINVAL, "run: replay needs restore arenas");
+    if (r->phase != 2 && r->ck_interval > 0 && !r->save_arenas)
+        return fail(ERTM_EINVAL, "run: checkpoint arenas missing");
+    if (r->stabilize && !b->all_visco) return fail(ERTM_EINVAL, "run: stabilize needs visco");
+    if (r->op_after > ERTM_IMG_EXC_APPLY) return fail(ERTM_EINVAL, "run: bad image op");
+    std::vector<float> amp(n, 0.0f);
+    std::vector<int32_t> tt(n, 0);
+    int cur = r->cur0 & 1;
+    int step_index = r->step_index0;
+    int rc = ERTM_OK;
+    if (r->phase == 1) {
+        for (int k = 0; k < n; ++k) {
+            const ErtmProp& P = b->props[k];
+            HIPCHK(hipMemcpyAsync(P.state[cur],
+                                  r->restore_arenas[k] + (int64_t)r->restore_slot * r->restore_stride,
+                                  sizeof(float) * (size_t)P.state_floats, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    const int nsteps = r->t_end - r->t_begin;
+    for (int q = 0; q < nsteps; ++q) {
+        const int t = r->phase == 2 ? r->t_end - 1 - q : r->t_begin + q;
+        for (int k = 0; k < n; ++k) {
+            tt[k] = t;
+            amp[k] = r->phase == 2 ? 0.0f : r->wavelets[k][t];
+        }
+        if (r->phase == 2) {
+            if ((rc = do_inject(b, cur, tt.data(), s)) != ERTM_OK) return rc;
+        }
+        if ((rc = do_step(b, cur, amp.data(), s)) != ERTM_OK) return rc;
+        cur ^= 1;
+        if (r->stabilize) {
+            if ((rc = do_stabilize(b, cur, step_index, r->cap, r->lp_interval, r->ctl, s)) != ERTM_OK)
+                return rc;
+        }
+        ++step_index;
+        if (r->phase != 2) {
+            if (r->do_sample && (rc = do_sample(b, cur, tt.data(), s)) != ERTM_OK) return rc;
+            if (r->ck_interval > 0 && t >= r->ck_origin && (t - r->ck_origin) % r->ck_interval == 0) {
+                const int64_t slot = (t - r->ck_origin) / r->ck_interval;
+                for (int k = 0; k < n; ++k) {
+                    const ErtmProp& P = b->props[k];
+                    HIPCHK(hipMemcpyAsync(r->save_arenas[k] + slot * r->save_stride, P.state[cur],
+                                          sizeof(float) * (size_t)P.state_floats,
+                                          hipMemcpyDeviceToDevice, s));
+                }
+            }
+        }
+        if (r->op_after >= 0) {
+            if ((rc = do_image(b, cur, tt.data(), r->op_after, s)) != ERTM_OK) return rc;
+        }
+    }
+    if (cur_out) *cur_out = cur;
+    return ERTM_OK;
+}
+
+int ertm_derivative(const void* f, void* out, int32_t n0, int32_t n1, int32_t axis, int32_t plus,
+                    int32_t order, double h, int32_t is_f64, void* stream) {
+    if (!f || !out || n0 <= 0 || n1 <= 0 || (axis != 0 && axis != 1))
+        return fail(ERTM_EINVAL, "derivative: bad arguments");
+    static const double C[4][4] = {{1.0, 0, 0, 0},
+                                   {9.0 / 8.0, -1.0 / 24.0, 0, 0},
+                                   {75.0 / 64.0, -25.0 / 384.0, 3.0 / 640.0, 0},
+                                   {1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0}};
+    if (order != 2 && order != 4 && order != 6 && order != 8)
+        return fail(ERTM_EINVAL, "derivative: order %d", order);
+    const int K = order / 2;
+    if ((axis == 0 ? n0 : n1) <= 2 * K) return fail(ERTM_EINVAL, "derivative: axis too short");
+    if (!(h > 0.0)) return fail(ERTM_EINVAL, "derivative: spacing must be positive");
+    DerivCoef c;
+    for (int k = 0; k < 4; ++k) {
+        c.c64[k] = C[K - 1][k];
+        c.c[k] = (float)C[K - 1][k];
+    }
+    const double hh = is_f64 ? h : (double)(float)h;
+    launch_derivative(f, out, n0, n1, axis, plus, order, c, hh, is_f64, (hipStream_t)stream);
+    LAUNCHCHK("derivative");
+    return ERTM_OK;
+}
+
+int ertm_absmax(const float* x, int64_t n, float* out, void* stream) {
+    if (!x || !out || n <= 0) return fail(ERTM_EINVAL, "absmax: bad arguments");
+    launch_absmax(x, n, out, (hipStream_t)stream);
+    LAUNCHCHK("absmax");
+    return ERTM_OK;
+}
+
+int ertm_sync(void* stream) {
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return ERTM_OK;
+}
+
+}  // extern "C"

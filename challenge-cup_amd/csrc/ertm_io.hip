@@ -1,0 +1,398 @@
+// Receiver sampling/injection, image-field extraction and imaging-condition
+// accumulation, reductions, the standalone derivative and the pieces of the
+// viscoacoustic stabiliser.  All deterministic: injection is an ordered
+// per-cell gather (never float atomics), reductions use order-independent max.
+#include "ertm_kernels.h"
+
+namespace ertm {
+
+// ---------------------------------------------------------------------------
+// sampling: ertm/numerics.py:478-511 via ertm/propagation.py:371-391
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sample_value(const ErtmProp& P, const float* S, int field,
+                                              int64_t off) {
+    if (field == -2) {  // elastic pressure 0.5*(sxx+szz), ertm/propagation.py:945-947
+        float s = S[(int64_t)ERTM_EL_SXX * P.fstride + off] + S[(int64_t)ERTM_EL_SZZ * P.fstride + off];
+        return 0.5f * s;
+    }
+    return S[(int64_t)field * P.fstride + off];
+}
+
+__global__ void k_sample(const ErtmProp* __restrict__ props, const ErtmRecv* __restrict__ recv,
+                         LaunchArgs la) {
+    const int slot = la.base + blockIdx.y;
+    const ErtmRecv& R = recv[slot];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= R.nsamp) return;
+    const ErtmProp& P = props[slot];
+    const float* S = P.state[la.cur];
+    const int t = la.t[blockIdx.y];
+    const int field = R.s_field[e];
+    double v = 0.0;
+    if (field != -1 && R.s_valid[e]) {
+        const int64_t o = (int64_t)R.s_i0[e] * P.ld + R.s_j0[e];
+        const double tz = R.s_tz[e], tx = R.s_tx[e];
+        const double wz0 = 1.0 - tz, wx0 = 1.0 - tx;
+        const double v00 = sample_value(P, S, field, o);
+        const double v01 = sample_value(P, S, field, o + 1);
+        const double v10 = sample_value(P, S, field, o + P.ld);
+        const double v11 = sample_value(P, S, field, o + P.ld + 1);
+        double a = v00 * wz0;
+        a = a * wx0;
+        double b = v01 * wz0;
+        b = b * tx;
+        double c = v10 * tz;
+        c = c * wx0;
+        double d = v11 * tz;
+        d = d * tx;
+        v = a + b;
+        v = v + c;
+        v = v + d;
+    }
+    R.records[R.s_rec[e]][(int64_t)t * R.nrec + R.s_col[e]] = (float)v;
+}
+
+void launch_sample(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,
+                   int max_samp, hipStream_t s) {
+    if (max_samp <= 0) return;
+    hipLaunchKernelGGL(k_sample, dim3((max_samp + 255) / 256, la.n), dim3(256), 0, s, dprops,
+                       drecv, la);
+}
+
+// ---------------------------------------------------------------------------
+// injection: ordered np.add.at semantics, ertm/propagation.py:331-369, :472-494
+// ---------------------------------------------------------------------------
+__global__ void k_inject(const ErtmProp* __restrict__ props, const ErtmRecv* __restrict__ recv,
+                         LaunchArgs la) {
+    const int slot = la.base + blockIdx.y;
+    const ErtmRecv& R = recv[slot];
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= R.ncells) return;
+    const ErtmProp& P = props[slot];
+    float* S = P.state[la.cur];
+    const int t = la.t[blockIdx.y];
+    float* f = S + (int64_t)R.cell_field[u] * P.fstride + R.cell_off[u];
+    float x = *f;
+    const int e1 = R.cell_ptr[u + 1];
+    for (int e = R.cell_ptr[u]; e < e1; ++e) {
+        const double v = (double)R.records[R.e_comp[e]][(int64_t)t * R.nrec + R.e_rec[e]];
+        double c = v * R.e_wa[e];
+        if (R.inject_mode == 0) c = c * R.e_wb[e];
+        x = (float)((double)x + c);
+    }
+    *f = x;
+}
+
+void launch_inject(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,
+                   int max_cells, hipStream_t s) {
+    if (max_cells <= 0) return;
+    hipLaunchKernelGGL(k_inject, dim3((max_cells + 255) / 256, la.n), dim3(256), 0, s, dprops,
+                       drecv, la);
+}
+
+// ---------------------------------------------------------------------------
+// image fields (ertm/propagation.py:496-497, :945-977) and imaging conditions
+// (ertm/imaging.py:279-324, :450-474)
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ float wave_value(const ErtmProp& P, const float* S, int sep, int wave,
+                                            int64_t o) {
+    if (P.engine != ERTM_ELASTIC) return S[(int64_t)ERTM_AC_P * P.fstride + o];
+    const float* vx = S + (int64_t)ERTM_EL_VX * P.fstride;
+    const float* vz = S + (int64_t)ERTM_EL_VZ * P.fstride;
+    if (sep == ERTM_SEP_STRESS) {
+        if (wave == ERTM_WAVE_S) return S[(int64_t)ERTM_EL_SXZ * P.fstride + o];
+        float s = S[(int64_t)ERTM_EL_SXX * P.fstride + o] + S[(int64_t)ERTM_EL_SZZ * P.fstride + o];
+        return 0.5f * s;
+    }
+    // divcurl: plain derivatives, no CPML memory (ertm/propagation.py:949-963)
+    if (wave == ERTM_WAVE_P) {
+        float a = dminus<K>(vx, o, 1, P.coef, P.dx);
+        float b = dminus<K>(vz, o, P.ld, P.coef, P.dz);
+        return a + b;
+    }
+    float a = dminus<K>(vx, o, P.ld, P.coef, P.dz);
+    float b = dminus<K>(vz, o, 1, P.coef, P.dx);
+    return a - b;
+}
+
+template <int K>
+__global__ void k_image(const ErtmProp* __restrict__ props, const ErtmImage* __restrict__ imgs,
+                        LaunchArgs la) {
+    const int slot = la.base + blockIdx.y;
+    const ErtmProp& P = props[slot];
+    const ErtmImage& I = imgs[slot];
+    const int64_t ncell = (int64_t)P.nz * P.nx;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncell) return;
+    const int i = (int)(q / P.nx), j = (int)(q - (int64_t)i * P.nx);
+    const int64_t o = (int64_t)(i + P.pad) * P.ld + (j + P.pad);
+    const float* S = P.state[la.cur];
+    const int t = la.t[blockIdx.y];
+    switch (la.op) {
+        case ERTM_IMG_EXTRACT: {
+            I.rbuf[(int64_t)(t - I.t0) * ncell + q] = wave_value<K>(P, S, I.separation, I.swave, o);
+            break;
+        }
+        case ERTM_IMG_CORRELATE: {
+            const float sv = I.rbuf[(int64_t)(t - I.t0) * ncell + q];
+            const bool cross = t >= I.start_step;
+            if (I.illum) {
+                float ss = sv * sv;
+                I.ill[q] = I.ill[q] + (double)ss;
+            }
+            for (int w = 0; w < I.nrw; ++w) {
+                const float rv = wave_value<K>(P, S, I.separation, I.rwave[w], o);
+                if (I.illum) {
+                    float rr = rv * rv;
+                    I.rill[w][q] = I.rill[w][q] + (double)rr;
+                }
+                if (cross) {
+                    float sr = sv * rv;
+                    I.num[w][q] = I.num[w][q] + (double)sr;
+                }
+            }
+            break;
+        }
+        case ERTM_IMG_EXC_PEAK: {
+            // excitation maps use the P field, ertm/imaging.py:465-473
+            const float a = fabsf(wave_value<K>(P, S, I.separation, ERTM_WAVE_P, o));
+            if ((double)a > I.running[q]) {
+                I.running[q] = (double)a;
+                I.peak_amp[q] = (double)a;
+                I.peak_step[q] = t;
+            }
+            break;
+        }
+        case ERTM_IMG_EXC_APPLY: {
+            if (t >= I.start_step && I.peak_step[q] == t) {
+                const float rv = wave_value<K>(P, S, I.separation, I.rwave[0], o);
+                I.num[0][q] = I.num[0][q] + I.peak_amp[q] * (double)rv;
+                float rr = rv * rv;
+                I.rill[0][q] = I.rill[0][q] + (double)rr;
+            }
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,
+                  int max_cells, hipStream_t s) {
+    if (max_cells <= 0) return;
+    dim3 g((max_cells + 255) / 256, la.n);
+    switch (order) {
+        case 2: hipLaunchKernelGGL(k_image<1>, g, dim3(256), 0, s, dprops, dimg, la); break;
+        case 4: hipLaunchKernelGGL(k_image<2>, g, dim3(256), 0, s, dprops, dimg, la); break;
+        case 6: hipLaunchKernelGGL(k_image<3>, g, dim3(256), 0, s, dprops, dimg, la); break;
+        default: hipLaunchKernelGGL(k_image<4>, g, dim3(256), 0, s, dprops, dimg, la); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// max |x| (order-independent, hence deterministic)
+// ---------------------------------------------------------------------------
+__global__ void k_zero_u32(unsigned int* p) { *p = 0u; }
+
+__global__ void k_absmax(const float* __restrict__ x, int64_t n, unsigned int* out) {
+    float m = 0.0f;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        float v = fabsf(x[k]);
+        m = (v > m || v != v) ? v : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        float o = __shfl_down(m, off, 64);
+        m = (o > m || o != o) ? o : m;
+    }
+    __shared__ float red[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = (red[w] > r || red[w] != red[w]) ? red[w] : r;
+        atomicMax(out, __float_as_uint(r));
+    }
+}
+
+void launch_absmax(const float* x, int64_t n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, (unsigned int*)out);
+    int64_t blocks = (n + 1023) / 1024;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_absmax, dim3((unsigned)blocks), dim3(1024), 0, s, x, n,
+                       (unsigned int*)out);
+}
+
+// ---------------------------------------------------------------------------
+// standalone staggered derivative of a dense array (ertm/numerics.py:68-137)
+// ---------------------------------------------------------------------------
+template <int K, typename T>
+__global__ void k_derivative(const T* __restrict__ f, T* __restrict__ out, int n0, int n1,
+                             int axis, int plus, DerivCoef c, double hd) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)n0 * n1) return;
+    const int i0 = (int)(q / n1), i1 = (int)(q - (int64_t)i0 * n1);
+    const int idx = axis == 0 ? i0 : i1;
+    const int n = axis == 0 ? n0 : n1;
+    const int64_t s = axis == 0 ? n1 : 1;
+    const T h = (T)hd;
+    T acc = (T)0;
+    if (idx >= K && idx < n - K) {
+#pragma unroll
+        for (int k = 1; k <= K; ++k) {
+            T d = plus ? (f[q + k * s] - f[q - (k - 1) * s]) : (f[q + (k - 1) * s] - f[q - k * s]);
+            T t = (T)c.c64[k - 1] * d;
+            if (sizeof(T) == 4) t = (T)c.c[k - 1] * d;
+            acc = acc + t;
+        }
+        acc = acc / h;
+    }
+    out[q] = acc;
+}
+
+template <int K>
+static void launch_derivative_k(const void* f, void* out, int n0, int n1, int axis, int plus,
+                                const DerivCoef& c, double h, int is_f64, hipStream_t s) {
+    const int64_t n = (int64_t)n0 * n1;
+    dim3 g((unsigned)((n + 255) / 256));
+    if (is_f64)
+        hipLaunchKernelGGL((k_derivative<K, double>), g, dim3(256), 0, s, (const double*)f,
+                           (double*)out, n0, n1, axis, plus, c, h);
+    else
+        hipLaunchKernelGGL((k_derivative<K, float>), g, dim3(256), 0, s, (const float*)f,
+                           (float*)out, n0, n1, axis, plus, c, h);
+}
+
+void launch_derivative(const void* f, void* out, int n0, int n1, int axis, int plus, int order,
+                       const DerivCoef& c, double h, int is_f64, hipStream_t s) {
+    switch (order) {
+        case 2: launch_derivative_k<1>(f, out, n0, n1, axis, plus, c, h, is_f64, s); break;
+        case 4: launch_derivative_k<2>(f, out, n0, n1, axis, plus, c, h, is_f64, s); break;
+        case 6: launch_derivative_k<3>(f, out, n0, n1, axis, plus, c, h, is_f64, s); break;
+        default: launch_derivative_k<4>(f, out, n0, n1, axis, plus, c, h, is_f64, s); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// viscoacoustic stabiliser, ertm/propagation.py:726-749
+// ---------------------------------------------------------------------------
+__global__ void k_stab_control(const float* __restrict__ peaks, double* __restrict__ ctl,
+                               int n, int step_index, double cap, int lowpass_now,
+                               float* __restrict__ scales) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double peak = (double)peaks[k];
+    double* c = ctl + 4 * k;
+    float sc = 1.0f;
+    if (step_index < 30) {
+        c[0] = c[0] > peak ? c[0] : peak;
+    } else {
+        const double ref = c[0] > 1e-30 ? c[0] : 1e-30;
+        const double limit = cap * ref;
+        if (peak > limit) {
+            sc = (float)(limit / peak);
+            c[1] += 1.0;
+        }
+        if (lowpass_now) c[2] += 1.0;
+    }
+    scales[k] = sc;
+}
+
+void launch_stab_control(const ErtmProp*, const LaunchArgs& la, const float* peaks, double* ctl,
+                         int step_index, double cap, int lowpass_now, float* scales,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_stab_control, dim3(1), dim3(64), 0, s, peaks, ctl, la.n, step_index, cap,
+                       lowpass_now, scales);
+}
+
+__global__ void k_stab_scale(const ErtmProp* __restrict__ props, LaunchArgs la,
+                             const float* __restrict__ scales) {
+    const int slot = la.base + blockIdx.y;
+    const float sc = scales[blockIdx.y];
+    if (sc == 1.0f) return;
+    const ErtmProp& P = props[slot];
+    float* S = P.state[la.cur];
+    const int64_t cells = (int64_t)P.nzp * P.ld;
+    const int nf = 4 + P.nmech;  // p, vx, vz, eps, r_0..r_{L-1}
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cells;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        for (int f = 0; f < nf; ++f) {
+            float* x = S + (int64_t)f * P.fstride + q;
+            *x = *x * sc;
+        }
+    }
+}
+
+void launch_stab_scale(const ErtmProp* dprops, const LaunchArgs& la, const float* scales,
+                       int max_tiles, hipStream_t s) {
+    (void)max_tiles;
+    hipLaunchKernelGGL(k_stab_scale, dim3(512, la.n), dim3(256), 0, s, dprops, la, scales);
+}
+
+// k-space low-pass of a padded field (ertm/numerics.py:424-442): the field is
+// widened to complex float64, transformed with hipFFT by the caller, weighted
+// by the radial ramp, transformed back and narrowed with the 1/N factor.
+__global__ void k_lp_prep(const float* __restrict__ f, int nzp, int nxp, int64_t ld,
+                          double2* __restrict__ c) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nzp * nxp) return;
+    const int i = (int)(q / nxp), j = (int)(q - (int64_t)i * nxp);
+    c[q] = make_double2((double)f[(int64_t)i * ld + j], 0.0);
+}
+
+__device__ __forceinline__ double fftfreq(int k, int n, double d) {
+    const int half = (n - 1) / 2 + 1;
+    const double v = 1.0 / (n * d);
+    return (k < half ? (double)k : (double)(k - n)) * v;
+}
+
+__global__ void k_lp_response(double2* __restrict__ c, int nzp, int nxp, double dx, double dz,
+                              double ratio) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nzp * nxp) return;
+    const int i = (int)(q / nxp), j = (int)(q - (int64_t)i * nxp);
+    double kzm = 0.5 / dz, kxm = 0.5 / dx;
+    kzm = kzm > 1e-12 ? kzm : 1e-12;
+    kxm = kxm > 1e-12 ? kxm : 1e-12;
+    const double kz = fftfreq(i, nzp, dz) / kzm;
+    const double kx = fftfreq(j, nxp, dx) / kxm;
+    const double r = sqrt(kz * kz + kx * kx);
+    double w = 1.0 - (r - ratio) / 0.15;
+    w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+    c[q].x *= w;
+    c[q].y *= w;
+}
+
+__global__ void k_lp_finish(float* __restrict__ f, int nzp, int nxp, int64_t ld,
+                            const double2* __restrict__ c) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nzp * nxp) return;
+    const int i = (int)(q / nxp), j = (int)(q - (int64_t)i * nxp);
+    f[(int64_t)i * ld + j] = (float)(c[q].x / ((double)nzp * (double)nxp));
+}
+
+void launch_lowpass_prep(const float* field, int nzp, int nxp, int64_t ld, void* cbuf,
+                         hipStream_t s) {
+    const int64_t n = (int64_t)nzp * nxp;
+    hipLaunchKernelGGL(k_lp_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, field, nzp,
+                       nxp, ld, (double2*)cbuf);
+}
+
+void launch_lowpass_response(void* cbuf, int nzp, int nxp, double dx, double dz, double ratio,
+                             hipStream_t s) {
+    const int64_t n = (int64_t)nzp * nxp;
+    hipLaunchKernelGGL(k_lp_response, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       (double2*)cbuf, nzp, nxp, dx, dz, ratio);
+}
+
+void launch_lowpass_finish(float* field, int nzp, int nxp, int64_t ld, const void* cbuf,
+                           hipStream_t s) {
+    const int64_t n = (int64_t)nzp * nxp;
+    hipLaunchKernelGGL(k_lp_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, field, nzp,
+                       nxp, ld, (const double2*)cbuf);
+}
+
+}  // namespace ertm

@@ -1,0 +1,50 @@
+// Internal interface between the C-ABI layer and the kernel translation units.
+#pragma once
+
+#include "ertm_device.h"
+
+namespace ertm {
+
+// Per-launch arguments: the launch covers propagators [base, base + n) of the
+// batch table; propagator base+k is at time index t[k] with source amplitude
+// amp[k]; states are read from state[cur] and written to state[cur ^ 1].
+struct LaunchArgs {
+    int32_t base;
+    int32_t n;
+    int32_t cur;
+    int32_t op;
+    int32_t t[ERTM_MAX_LAUNCH];
+    float amp[ERTM_MAX_LAUNCH];
+};
+
+struct DerivCoef {
+    float c[4];
+    double c64[4];
+};
+
+int tiles_of(const ErtmProp& P);
+void launch_step(const ErtmProp* dprops, const LaunchArgs& la, int order, int max_tiles,
+                 bool any_force, hipStream_t s);
+void launch_sample(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,
+                   int max_samp, hipStream_t s);
+void launch_inject(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,
+                   int max_cells, hipStream_t s);
+void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,
+                  int max_cells, hipStream_t s);
+void launch_absmax(const float* x, int64_t n, float* out, hipStream_t s);
+void launch_derivative(const void* f, void* out, int n0, int n1, int axis, int plus, int order,
+                       const DerivCoef& c, double h, int is_f64, hipStream_t s);
+// Viscoacoustic stabiliser pieces (ertm/propagation.py:726-749).
+void launch_stab_control(const ErtmProp* dprops, const LaunchArgs& la, const float* peaks,
+                         double* ctl, int step_index, double cap, int lowpass_now, float* scales,
+                         hipStream_t s);
+void launch_stab_scale(const ErtmProp* dprops, const LaunchArgs& la, const float* scales,
+                       int max_tiles, hipStream_t s);
+void launch_lowpass_prep(const float* field, int nzp, int nxp, int64_t ld, void* cbuf,
+                         hipStream_t s);
+void launch_lowpass_response(void* cbuf, int nzp, int nxp, double dx, double dz, double ratio,
+                             hipStream_t s);
+void launch_lowpass_finish(float* field, int nzp, int nxp, int64_t ld, const void* cbuf,
+                           hipStream_t s);
+
+}  // namespace ertm

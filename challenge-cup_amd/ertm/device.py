@@ -1,0 +1,537 @@
+"""GPU-resident state of the engine: one ``Core`` per propagator, receiver and
+imaging tables, and ``Batch`` handles over the C-ABI.
+
+Memory layout (all allocated as torch tensors, which are used only as HBM
+allocations; every computation is a HIP kernel of libertm_hip.so):
+
+* two ping-pong state buffers per propagator, each holding the field slabs
+  (padded grid, rows padded to 64 floats = 256 B) followed by the CPML memory
+  strips;
+* medium arrays on the padded grid: (kappa, 1/rho) acoustic, (lambda, mu,
+  1/rho) elastic, (K_R, 1/rho, Q class) + per-Q tables viscoacoustic -- the
+  face averages are recomputed in the kernels;
+* CPML profiles per padded row / column.
+
+Set-up arithmetic (edge padding, moduli, CPML profiles, fractional receiver
+indices and weights) is done on the host in the reference's own precision
+and operation order (ertm/propagation.py:165-187, :251-262, :331-369), so
+the kernels see exactly the reference's float32 inputs.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from . import numerics as num
+from .exceptions import SolverError
+
+FIELDS = {
+    "acoustic": ("p", "vx", "vz"),
+    "elastic": ("vx", "vz", "sxx", "szz", "sxz"),
+    "viscoacoustic": ("p", "vx", "vz", "eps"),
+}
+# CPML memory names per engine, in psix / psiz order (the reference's names)
+PSI_X = {
+    "acoustic": ("vx_dp_dx", "p_dvx_dx"),
+    "viscoacoustic": ("vx_dp_dx", "p_dvx_dx"),
+    "elastic": ("vx_dsxx", "vz_dsxz", "sxx_dvx", "sxz_dvz"),
+}
+PSI_Z = {
+    "acoustic": ("vz_dp_dz", "p_dvz_dz"),
+    "viscoacoustic": ("vz_dp_dz", "p_dvz_dz"),
+    "elastic": ("vx_dsxz", "vz_dszz", "szz_dvz", "sxz_dvx"),
+}
+OFFSETS = {"vx": (-0.5, 0.0), "vz": (0.0, -0.5), "sxz": (-0.5, -0.5)}
+ROW_ALIGN = 64
+
+
+def torch_mod():
+    import torch
+
+    return torch
+
+
+def _rup(x: int, m: int) -> int:
+    return (int(x) + m - 1) // m * m
+
+
+def edge_pad(a: np.ndarray, pad: int, dtype=np.float32) -> np.ndarray:
+    """ertm/propagation.py:165-166."""
+    return np.pad(np.asarray(a).astype(dtype, copy=False), pad, mode="edge")
+
+
+def to_device(a: np.ndarray, dtype=None):
+    torch = torch_mod()
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda", non_blocking=False)
+
+
+def padded_rows(a: np.ndarray, ld: int):
+    """Upload an (nzp, nxp) host array into a (nzp, ld) device array."""
+    torch = torch_mod()
+    nzp, nxp = a.shape
+    out = torch.zeros((nzp, ld), dtype=torch.from_numpy(np.zeros(1, a.dtype)).dtype, device="cuda")
+    out[:, :nxp] = torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    return out
+
+
+class Core:
+    """One propagator's device state and its C descriptor."""
+
+    def __init__(self, engine: str, nz: int, nx: int, dx: float, dz: float, x0: float, z0: float,
+                 order: int, dt: float, npml: int, boundary: str, vmax: float, media: dict,
+                 nmech: int = 0):
+        torch = torch_mod()
+        L.lib()
+        self.engine = engine
+        self.nz, self.nx = int(nz), int(nx)
+        self.dx, self.dz, self.x0, self.z0 = float(dx), float(dz), float(x0), float(z0)
+        self.order = int(order)
+        self.pad = num.ghost_width(order)
+        self.nzp, self.nxp = self.nz + 2 * self.pad, self.nx + 2 * self.pad
+        self.ld = _rup(self.nxp, ROW_ALIGN)
+        self.dt = float(dt)
+        self.boundary = boundary
+        self.nmech = int(nmech)
+        enabled = boundary != "none"
+        free = boundary == "free"
+        ax, bx, wx = num.cpml_profile(self.nxp, self.dx, npml, self.dt, vmax, 25.0, True, True,
+                                      enabled)
+        az, bz, wz = num.cpml_profile(self.nzp, self.dz, npml, self.dt, vmax, 25.0, not free, True,
+                                      enabled)
+        self.wx = wx if enabled else 0
+        self.wzt = wz if (enabled and not free) else 0
+        self.wzb = wz if enabled else 0
+        self.prof = {"ax": ax, "bx": bx, "az": az, "bz": bz}
+        self.d_prof = {k: to_device(v) for k, v in self.prof.items()}
+        self.field_names = FIELDS[engine]
+        self.nfields = len(self.field_names) + (self.nmech if engine == "viscoacoustic" else 0)
+        self.fstride = _rup(self.nzp * self.ld, 64)
+        off = self.nfields * self.fstride
+        self.psix, self.psiz = [-1] * 4, [-1] * 4
+        nterms = len(PSI_X[engine])
+        self.xsize = self.nzp * 2 * self.wx
+        self.zsize = (self.wzt + self.wzb) * self.ld
+        for k in range(nterms):
+            if self.wx > 0:
+                self.psix[k] = off
+                off += _rup(self.xsize, 64)
+        for k in range(nterms):
+            if self.zsize > 0:
+                self.psiz[k] = off
+                off += _rup(self.zsize, 64)
+        self.state_floats = off
+        self.state = torch.zeros((2, self.state_floats), dtype=torch.float32, device="cuda")
+        self.cur = 0
+        self.media = {k: (padded_rows(v, self.ld) if isinstance(v, np.ndarray) and v.ndim == 2
+                          else (to_device(v) if isinstance(v, np.ndarray) else v))
+                      for k, v in media.items()}
+        self.src = (-1, -1)
+        self.mechanism = L.SRC_EXPLOSIVE
+        self._desc = None
+
+    # ----------------------------------------------------------- descriptor
+    def desc(self) -> L.ErtmProp:
+        d = L.ErtmProp()
+        d.engine = L.ENGINE_CODE[self.engine]
+        d.nzp, d.nxp, d.ld = self.nzp, self.nxp, self.ld
+        d.pad, d.nz, d.nx, d.order = self.pad, self.nz, self.nx, self.order
+        d.free_surface = int(self.boundary == "free" and self.engine != "viscoacoustic")
+        d.nmech = self.nmech
+        d.wx, d.wzt, d.wzb = self.wx, self.wzt, self.wzb
+        d.nfields = self.nfields
+        d.dx, d.dz, d.dt = self.dx, self.dz, self.dt  # rounded to float32 by ctypes
+        for k, c in enumerate(num.coef_f32(self.order)):
+            d.coef[k] = c
+        d.dt64 = self.dt
+        d.fstride = self.fstride
+        for k in range(4):
+            d.psix[k] = self.psix[k]
+            d.psiz[k] = self.psiz[k]
+        d.state_floats = self.state_floats
+        m = self.media
+        d.kappa = L.ptr(m.get("kappa"))
+        d.irho = L.ptr(m.get("irho"))
+        d.lam = L.ptr(m.get("lam"))
+        d.mu = L.ptr(m.get("mu"))
+        d.qidx = L.ptr(m.get("qidx"))
+        d.q_a = L.ptr(m.get("q_a"))
+        d.q_delta = L.ptr(m.get("q_delta"))
+        d.q_tau = L.ptr(m.get("q_tau"))
+        p = self.d_prof
+        d.ax, d.bx, d.az, d.bz = L.ptr(p["ax"]), L.ptr(p["bx"]), L.ptr(p["az"]), L.ptr(p["bz"])
+        d.state[0] = L.ptr(self.state[0])
+        d.state[1] = L.ptr(self.state[1])
+        d.src_iz, d.src_ix = self.src
+        d.mechanism = self.mechanism
+        return d
+
+    # ------------------------------------------------------------- fields
+    def slab(self, k: int, cur: int | None = None):
+        """(nzp, nxp) device view of field slab k of state[cur]."""
+        c = self.cur if cur is None else cur
+        s = self.state[c, k * self.fstride : k * self.fstride + self.nzp * self.ld]
+        return s.view(self.nzp, self.ld)[:, : self.nxp]
+
+    def field_index(self, name: str) -> int:
+        return self.field_names.index(name)
+
+    def psi_views(self, cur: int | None = None) -> dict:
+        c = self.cur if cur is None else cur
+        out = {}
+        for k, name in enumerate(PSI_X[self.engine]):
+            if self.psix[k] >= 0:
+                o = self.psix[k]
+                out[name] = self.state[c, o : o + self.xsize].view(self.nzp, 2 * self.wx)
+        for k, name in enumerate(PSI_Z[self.engine]):
+            if self.psiz[k] >= 0:
+                o = self.psiz[k]
+                out[name] = self.state[c, o : o + self.zsize].view(self.wzt + self.wzb,
+                                                                    self.ld)[:, : self.nxp]
+        return out
+
+    def psi_bytes(self) -> int:
+        return 4 * (len(PSI_X[self.engine]) * self.xsize
+                    + len(PSI_Z[self.engine]) * (self.wzt + self.wzb) * self.nxp)
+
+    def expand_psi(self, name: str, compact: np.ndarray) -> np.ndarray:
+        """Full padded-grid array of one CPML memory (zeros outside the strips)."""
+        full = np.zeros((self.nzp, self.nxp), np.float32)
+        if name in PSI_X[self.engine]:
+            w = self.wx
+            full[:, :w] = compact[:, :w]
+            full[:, self.nxp - w :] = compact[:, w:]
+        else:
+            full[: self.wzt, :] = compact[: self.wzt]
+            if self.wzb:
+                full[self.nzp - self.wzb :, :] = compact[self.wzt :]
+        return full
+
+    # ----------------------------------------------------------- geometry
+    def node_index(self, x: float, z: float) -> tuple[int, int]:
+        """Nearest padded node (round half to even, clipped), ertm/propagation.py:246-249."""
+        ix = int(round((float(x) - self.x0) / self.dx))
+        iz = int(round((float(z) - self.z0) / self.dz))
+        ix = min(max(ix, 0), self.nx - 1)
+        iz = min(max(iz, 0), self.nz - 1)
+        return iz + self.pad, ix + self.pad
+
+    def fractional_index(self, x, z, comp: str):
+        """ertm/propagation.py:251-262 with the per-component half-cell offsets."""
+        ox, oz = OFFSETS.get(comp, (0.0, 0.0))
+        xf = (np.asarray(x, dtype=np.float64) - self.x0) / self.dx + self.pad + ox
+        zf = (np.asarray(z, dtype=np.float64) - self.z0) / self.dz + self.pad + oz
+        return zf, xf
+
+    def corners(self, x, z, comp: str):
+        zf, xf = self.fractional_index(x, z, comp)
+        z0 = np.floor(zf).astype(np.int64)
+        x0 = np.floor(xf).astype(np.int64)
+        tz, tx = zf - z0, xf - x0
+        ok = (z0 >= 0) & (z0 < self.nzp - 1) & (x0 >= 0) & (x0 < self.nxp - 1)
+        return np.clip(z0, 0, self.nzp - 2), np.clip(x0, 0, self.nxp - 2), tz, tx, ok
+
+
+# ---------------------------------------------------------------------------
+# receiver tables
+# ---------------------------------------------------------------------------
+def sample_field_code(engine: str, comp: str) -> tuple[int, str]:
+    """(slab or special code, offset name) for sampling ``comp``."""
+    if engine == "elastic":
+        if comp == "p":
+            return -2, "sxx"
+        if comp in FIELDS["elastic"]:
+            return FIELDS["elastic"].index(comp), comp
+    elif engine == "acoustic":
+        if comp == "sxz":
+            return -1, comp
+        if comp in FIELDS["acoustic"]:
+            return FIELDS["acoustic"].index(comp), comp
+    else:
+        if comp in ("p", "vx", "vz"):
+            return ("p", "vx", "vz").index(comp), comp
+    names = {"elastic": "vx, vz, sxx, szz, sxz, p", "acoustic": "p, vx, vz",
+             "viscoacoustic": "p, vx, vz"}[engine]
+    raise SolverError(f"{engine} propagator cannot sample component {comp!r} (available: {names})")
+
+
+def inject_targets(engine: str, comp: str) -> list[tuple[int, str, bool]]:
+    """[(slab, offset name, scaled_by_1/K_R)] that ``inject(comp)`` writes."""
+    if engine == "elastic":
+        if comp == "p":  # full value into both normal stresses, ertm/propagation.py:907-910
+            return [(2, "sxx", False), (3, "sxx", False)]
+        if comp in FIELDS["elastic"]:
+            return [(FIELDS["elastic"].index(comp), comp, False)]
+    elif engine == "acoustic":
+        if comp in FIELDS["acoustic"]:
+            return [(FIELDS["acoustic"].index(comp), comp, False)]
+    else:
+        if comp == "p":  # pressure also enters the strain, ertm/propagation.py:770-774
+            return [(0, "p", False), (3, "p", True)]
+        if comp in ("vx", "vz"):
+            return [(("p", "vx", "vz").index(comp), comp, False)]
+    raise SolverError(f"{engine} propagator cannot inject component {comp!r}")
+
+
+class RecvTables:
+    """Device tables behind one ErtmRecv (sampling and/or injection)."""
+
+    def __init__(self):
+        self.s = L.ErtmRecv()
+        self.keep: list = []
+
+    def _dev(self, a, dtype):
+        t = to_device(np.asarray(a, dtype=dtype))
+        self.keep.append(t)
+        return L.ptr(t)
+
+    def set_records(self, records: list):
+        """records[k]: (nt, nrec) float32 device tensors, one per component."""
+        self.s.ncomp = len(records)
+        for k, r in enumerate(records):
+            self.s.records[k] = L.ptr(r)
+        self.keep.extend(records)
+
+    def build_sampling(self, core: Core, rx, rz, comps):
+        i0, j0, fld, rec, col, val, tz, tx = [], [], [], [], [], [], [], []
+        nrec = len(rx)
+        for ci, comp in enumerate(comps):
+            code, offname = sample_field_code(core.engine, comp)
+            zc, xc, tzz, txx, ok = core.corners(rx, rz, offname)
+            i0.append(zc)
+            j0.append(xc)
+            fld.append(np.full(nrec, code))
+            rec.append(np.full(nrec, ci))
+            col.append(np.arange(nrec))
+            val.append(ok.astype(np.int32))
+            tz.append(tzz)
+            tx.append(txx)
+        s = self.s
+        s.nrec = nrec
+        s.nsamp = nrec * len(comps)
+        cat = np.concatenate
+        s.s_i0 = self._dev(cat(i0), np.int32)
+        s.s_j0 = self._dev(cat(j0), np.int32)
+        s.s_field = self._dev(cat(fld), np.int32)
+        s.s_rec = self._dev(cat(rec), np.int32)
+        s.s_col = self._dev(cat(col), np.int32)
+        s.s_valid = self._dev(cat(val), np.int32)
+        s.s_tz = self._dev(cat(tz), np.float64)
+        s.s_tx = self._dev(cat(tx), np.float64)
+
+    def build_injection(self, core: Core, rx, rz, comps, k_relaxed: np.ndarray | None = None):
+        """Ordered scatter tables reproducing np.add.at (ertm/propagation.py:331-369,
+        :472-494, :916-942): entries in (component, target, corner pass,
+        receiver) order, grouped per target cell with that order kept."""
+        nrec = len(rx)
+        mode = 1 if core.engine == "viscoacoustic" else 0
+        keys, comp_i, rec_i, wa_l, wb_l = [], [], [], [], []
+        plane = core.fstride
+        for ci, comp in enumerate(comps):
+            for slab, offname, scaled in inject_targets(core.engine, comp):
+                zc, xc, tz, tx, ok = core.corners(rx, rz, offname)
+                zc, xc, tz, tx = zc[ok], xc[ok], tz[ok], tx[ok]
+                ridx = np.arange(nrec)[ok]
+                if mode == 1:
+                    if scaled:
+                        sc = 1.0 / np.maximum(np.abs(k_relaxed), 1e-20)  # float32 ops
+                        fac = np.asarray(sc, np.float32)[zc, xc].astype(np.float64)
+                    else:
+                        fac = np.float64(np.float32(1.0))
+                for dz_, dx_ in ((0, 0), (0, 1), (1, 0), (1, 1)):
+                    wz = tz if dz_ else 1 - tz
+                    wx = tx if dx_ else 1 - tx
+                    off = (zc + dz_) * core.ld + (xc + dx_)
+                    keys.append(slab * plane + off)
+                    comp_i.append(np.full(ridx.size, ci))
+                    rec_i.append(ridx)
+                    if mode == 0:
+                        wa_l.append(wz)
+                        wb_l.append(wx)
+                    else:
+                        wa_l.append(wz * wx * fac)
+                        wb_l.append(np.zeros(ridx.size))
+        s = self.s
+        s.inject_mode = mode
+        s.nrec = nrec
+        if not keys:
+            s.ncells = 0
+            return
+        key = np.concatenate(keys)
+        order = np.argsort(key, kind="stable")
+        key = key[order]
+        cells, start = np.unique(key, return_index=True)
+        ptrs = np.concatenate([start, [key.size]]).astype(np.int32)
+        s.ncells = int(cells.size)
+        s.nentries = int(key.size)
+        s.cell_ptr = self._dev(ptrs, np.int32)
+        s.cell_field = self._dev(cells // plane, np.int32)
+        s.cell_off = self._dev(cells % plane, np.int64)
+        s.e_comp = self._dev(np.concatenate(comp_i)[order], np.int32)
+        s.e_rec = self._dev(np.concatenate(rec_i)[order], np.int32)
+        s.e_wa = self._dev(np.concatenate(wa_l)[order], np.float64)
+        s.e_wb = self._dev(np.concatenate(wb_l)[order], np.float64)
+
+
+class ImageBuffers:
+    """Accumulators and replay slots behind one ErtmImage."""
+
+    def __init__(self, nz: int, nx: int, separation: str, swave: str, rwaves, start_step: int,
+                 illum: bool, excitation: bool = False, rbuf=None, nslots: int = 0):
+        torch = torch_mod()
+        self.s = L.ErtmImage()
+        s = self.s
+        s.separation = L.SEP_DIVCURL if separation == "divcurl" else L.SEP_STRESS
+        s.swave = L.WAVE_S if swave == "S" else L.WAVE_P
+        s.nrw = len(rwaves)
+        for k, w in enumerate(rwaves):
+            s.rwave[k] = L.WAVE_S if w == "S" else L.WAVE_P
+        s.start_step = int(start_step)
+        s.illum = int(bool(illum))
+        z = lambda dt: torch.zeros((nz, nx), dtype=dt, device="cuda")  # noqa: E731
+        self.num = [z(torch.float64) for _ in rwaves]
+        self.rill = [z(torch.float64) for _ in rwaves]
+        self.ill = z(torch.float64)
+        for k in range(len(rwaves)):
+            s.num[k] = L.ptr(self.num[k])
+            s.rill[k] = L.ptr(self.rill[k])
+        s.ill = L.ptr(self.ill)
+        self.rbuf = rbuf
+        s.rbuf = L.ptr(rbuf)
+        s.nslots = int(nslots)
+        if excitation:
+            self.peak_step = torch.full((nz, nx), -1, dtype=torch.int32, device="cuda")
+            self.peak_amp = z(torch.float64)
+            self.running = z(torch.float64)
+            s.peak_step = L.ptr(self.peak_step)
+            s.peak_amp = L.ptr(self.peak_amp)
+            s.running = L.ptr(self.running)
+
+
+# ---------------------------------------------------------------------------
+# batches
+# ---------------------------------------------------------------------------
+class Batch:
+    """A set of propagators advanced together (one ertm_batch_t)."""
+
+    def __init__(self, cores: list, recvs: list | None = None, imgs: list | None = None):
+        lib = L.lib()
+        n = len(cores)
+        self.cores = list(cores)
+        self.recvs = recvs
+        self.imgs = imgs
+        self._props = (L.ErtmProp * n)(*[c.desc() for c in cores])
+        self._recv = (L.ErtmRecv * n)(*[r.s for r in recvs]) if recvs else None
+        self._img = (L.ErtmImage * n)(*[i.s for i in imgs]) if imgs else None
+        h = C.c_void_p()
+        L.check(lib.ertm_batch_create(self._props, self._recv, self._img, n, C.byref(h)),
+                "ertm_batch_create")
+        self.h = h
+        self.n = n
+
+    def update_images(self, imgs: list):
+        self.imgs = imgs
+        self._img = (L.ErtmImage * self.n)(*[i.s for i in imgs])
+        L.check(L.lib().ertm_batch_update_image(self.h, self._img), "ertm_batch_update_image")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            L.lib().ertm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    # single-call wrappers ------------------------------------------------
+    def step(self, cur: int, amps) -> None:
+        a = (C.c_float * self.n)(*[float(v) for v in amps])
+        L.check(L.lib().ertm_step(self.h, cur, a, L.stream_handle()), "ertm_step")
+
+    def sample(self, cur: int, t) -> None:
+        tt = (C.c_int32 * self.n)(*[int(v) for v in t])
+        L.check(L.lib().ertm_sample(self.h, cur, tt, L.stream_handle()), "ertm_sample")
+
+    def inject(self, cur: int, t) -> None:
+        tt = (C.c_int32 * self.n)(*[int(v) for v in t])
+        L.check(L.lib().ertm_inject(self.h, cur, tt, L.stream_handle()), "ertm_inject")
+
+    def image(self, cur: int, t, op: int) -> None:
+        tt = (C.c_int32 * self.n)(*[int(v) for v in t])
+        L.check(L.lib().ertm_image(self.h, cur, tt, op, L.stream_handle()), "ertm_image")
+
+    def stabilize(self, cur: int, step_index: int, cap: float, interval: int, ctl) -> None:
+        L.check(L.lib().ertm_stabilize(self.h, cur, int(step_index), float(cap), int(interval),
+                                       C.c_void_p(L.ptr(ctl)), L.stream_handle()),
+                "ertm_stabilize")
+
+    def _ptr_array(self, tensors, keep):
+        arr = (C.c_void_p * self.n)(*[L.ptr(a) for a in tensors])
+        keep.append(arr)
+        return C.cast(arr, C.POINTER(C.c_void_p))
+
+    def run(self, phase: int, t_begin: int, t_end: int, cur0: int, *, wavelets=None,
+            do_sample=False, op_after=-1, ck_interval=0, ck_origin=0, save=None, save_stride=0,
+            restore=None, restore_stride=0, restore_slot=0, stabilize=False, step_index0=0,
+            lp_interval=0, cap=0.0, ctl=None) -> int:
+        """One ertm_run call; returns the parity of the final state."""
+        r = L.ErtmRun()
+        r.phase, r.t_begin, r.t_end, r.cur0 = phase, t_begin, t_end, cur0
+        r.do_sample, r.op_after = int(do_sample), int(op_after)
+        r.ck_interval, r.ck_origin = int(ck_interval), int(ck_origin)
+        keep = []
+        if save is not None:
+            r.save_arenas = self._ptr_array(save, keep)
+            r.save_stride = int(save_stride)
+        if restore is not None:
+            r.restore_arenas = self._ptr_array(restore, keep)
+            r.restore_stride = int(restore_stride)
+            r.restore_slot = int(restore_slot)
+        if wavelets is not None:
+            ws = [np.ascontiguousarray(w, dtype=np.float32) for w in wavelets]
+            keep.extend(ws)
+            arr = (C.c_void_p * self.n)(*[w.ctypes.data for w in ws])
+            keep.append(arr)
+            r.wavelets = C.cast(arr, C.POINTER(C.c_void_p))
+            r.nw = min(len(w) for w in ws)
+        r.stabilize = int(stabilize)
+        r.step_index0 = int(step_index0)
+        r.lp_interval = int(lp_interval)
+        r.cap = float(cap)
+        r.ctl = L.ptr(ctl)
+        cur = C.c_int32(0)
+        L.check(L.lib().ertm_run(self.h, C.byref(r), C.byref(cur), L.stream_handle()), "ertm_run")
+        return int(cur.value)
+
+
+# ---------------------------------------------------------------------------
+# utilities
+# ---------------------------------------------------------------------------
+def derivative(array: np.ndarray, axis: int, order: int, h: float, plus: bool) -> np.ndarray:
+    torch = torch_mod()
+    lib = L.lib()
+    is64 = np.result_type(array.dtype, np.float32) == np.float64
+    dt = np.float64 if is64 else np.float32
+    x = to_device(np.asarray(array, dtype=dt))
+    out = torch.empty_like(x)
+    n0, n1 = array.shape
+    L.check(lib.ertm_derivative(C.c_void_p(L.ptr(x)), C.c_void_p(L.ptr(out)), n0, n1, axis,
+                                int(plus), order, float(h), int(is64), L.stream_handle()),
+            "ertm_derivative")
+    return out.cpu().numpy()
+
+
+def absmax(t) -> float:
+    torch = torch_mod()
+    lib = L.lib()
+    out = torch.zeros(1, dtype=torch.float32, device="cuda")
+    L.check(lib.ertm_absmax(C.c_void_p(L.ptr(t)), t.numel(), C.c_void_p(L.ptr(out)),
+                            L.stream_handle()), "ertm_absmax")
+    return float(out.item())

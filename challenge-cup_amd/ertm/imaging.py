@@ -1,0 +1,543 @@
+"""Reverse-time migration of one shot on the GPU (ertm/imaging.py).
+
+``RTMImager.migrate`` keeps the reference's algorithm and quirks
+(ertm/imaging.py:205-349):
+
+1. forward propagation of the source wavelet with checkpoints every
+   ``interval`` steps, the interval coming from the reference's budget rule
+   on the fields-only state size (:113-137);
+2. blocks in reverse: each block is replayed from its checkpoint, with the
+   reference's replay convention (the state saved after step s is stepped
+   with w[s] again, :140-162), writing the source-side image field of every
+   step into HBM replay slots;
+3. the backward propagator, fed the time-reversed record at the receivers,
+   steps through the block in reverse while the imaging condition
+   (zero_lag / normalized / excitation) is accumulated in float64 on the GPU;
+4. normalisation, Laplacian, wavenumber low-pass and the PML mask (once per
+   shot, float64 on the host, as the reference does).
+
+Everything between the upload of the record and the download of the
+accumulators stays in HBM: checkpoints are device-to-device copies into an
+arena, replayed image fields live in replay slots, accumulators never leave
+the device until the end.  When one block's replay slots would not fit the
+replay budget, the block is replayed in segments from sub-checkpoints
+(nested replay); the segments reproduce the same fields bit for bit.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Any, Callable
+
+import numpy as np
+
+from . import _lib as L
+from . import device as dev
+from . import numerics as num
+from .config import GpuConfig, ImagingConfig, QConfig, SolverConfig
+from .exceptions import ImagingError
+from .propagation import (
+    PropagationSpec,
+    Propagator2D,
+    QOptions,
+    build_propagator,
+    check_hard_stability,
+    check_stability,
+    source_wavelet,
+)
+from .types import ElasticModel, ImageResult, ShotRecord
+
+#: imaging mode -> (source wave, receiver wave), ertm/imaging.py:47-52
+WAVE_MODES: dict[str, tuple[str, str]] = {
+    "pp": ("P", "P"),
+    "ps": ("P", "S"),
+    "sp": ("S", "P"),
+    "ss": ("S", "S"),
+}
+_ACOUSTIC_MODES = {"pp"}
+
+
+def zero_lag_condition(source_wavefield: np.ndarray, receiver_wavefield: np.ndarray) -> np.ndarray:
+    """Integrand W_s * W_r of the zero-lag cross-correlation (ertm/imaging.py:61-66)."""
+    return np.asarray(source_wavefield) * np.asarray(receiver_wavefield)
+
+
+def normalized_terms(source_wavefield, receiver_wavefield):
+    """(W_s W_r, W_s^2, W_r^2) (ertm/imaging.py:69-79)."""
+    s = np.asarray(source_wavefield)
+    r = np.asarray(receiver_wavefield)
+    return s * r, s * s, r * r
+
+
+def excitation_terms(source_amplitude, receiver_wavefield) -> np.ndarray:
+    """A_s(x) W_r(x, t_exc) (ertm/imaging.py:82-87)."""
+    return np.asarray(source_amplitude) * np.asarray(receiver_wavefield)
+
+
+# ---------------------------------------------------------------------------
+# checkpoints
+# ---------------------------------------------------------------------------
+@dataclass
+class CheckpointStore:
+    """Source-wavefield checkpoints held in an HBM arena (ertm/imaging.py:93-110).
+
+    ``states`` lists (step, arena slot); ``bytes_per_state`` is the reference's
+    budgeting size (fields only, see record_source_wavefield); ``arena_bytes``
+    is what the arena really holds (fields + CPML strips per state).
+    """
+
+    states: list[tuple[int, int]] = field(default_factory=list)
+    interval: int = 1
+    bytes_per_state: int = 0
+    arena: Any = None
+    state_floats: int = 0
+
+    @property
+    def memory_mb(self) -> float:
+        return len(self.states) * self.bytes_per_state / (1024.0 ** 2)
+
+    @property
+    def arena_bytes(self) -> int:
+        return 0 if self.arena is None else int(self.arena.numel() * 4)
+
+    def summary(self) -> dict[str, Any]:
+        return {"interval": int(self.interval), "n_states": int(len(self.states)),
+                "memory_mb": round(self.memory_mb, 3),
+                "device_arena_mb": round(self.arena_bytes / 1024.0 ** 2, 3)}
+
+    def bounds(self, b: int, nt: int) -> tuple[int, int]:
+        start = self.states[b][0]
+        end = self.states[b + 1][0] if b + 1 < len(self.states) else nt
+        return start, end
+
+
+def checkpoint_interval(nt: int, per_state: int, budget_mb: float) -> int:
+    """ertm/imaging.py:121-124."""
+    budget = max(float(budget_mb), 1.0) * 1024.0 ** 2
+    max_states = max(1, int(budget // max(int(per_state), 1)))
+    return max(1, int(np.ceil(nt / max_states)))
+
+
+def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
+                            budget_mb: float) -> CheckpointStore:
+    """Forward-propagate and checkpoint into an HBM arena (ertm/imaging.py:113-137).
+
+    The interval is computed from ``state_bytes()`` taken before the first
+    step (fields only: the CPML memories do not exist yet), exactly as the
+    reference does; the arena itself stores fields and CPML strips.
+    """
+    import torch
+
+    nt = int(len(wavelet))
+    per_state = max(int(prop.state_bytes()), 1)
+    interval = checkpoint_interval(nt, per_state, budget_mb)
+    starts = list(range(0, nt, interval))
+    core = prop._core
+    arena = torch.empty((len(starts), core.state_floats), dtype=torch.float32, device="cuda")
+    b = dev.Batch([core])
+    w = np.asarray(wavelet, dtype=np.float32)
+    core.cur = b.run(0, 0, nt, core.cur, wavelets=[w], ck_interval=interval, ck_origin=0,
+                     save=[arena], save_stride=core.state_floats, step_index0=prop.step_index,
+                     **prop._run_kwargs())
+    b.close()
+    prop.pml.touched = True
+    prop.step_index += nt
+    return CheckpointStore(states=[(s, k) for k, s in enumerate(starts)], interval=interval,
+                           bytes_per_state=per_state, arena=arena, state_floats=core.state_floats)
+
+
+def replay_block(prop: Propagator2D, wavelet: np.ndarray, store: CheckpointStore,
+                 block_index: int, separation: str) -> list[dict[str, np.ndarray]]:
+    """Host copies of the image fields of one replayed block (ertm/imaging.py:140-162).
+
+    Drop-in API for callers that want the fields on the host; migrate keeps
+    them in HBM replay slots instead.
+    """
+    import torch
+
+    core = prop._core
+    nt = int(len(wavelet))
+    start, end = store.bounds(block_index, nt)
+    k = end - start
+    waves = ("P", "S") if prop._engine == "elastic" else ("P",)
+    bufs = {wv: torch.zeros((max(k, 1), core.nz, core.nx), dtype=torch.float32, device="cuda")
+            for wv in waves}
+    w = np.asarray(wavelet, dtype=np.float32)
+    # first pass: the P (or only) field; second pass (elastic): S
+    for n, wv in enumerate(waves):
+        img = dev.ImageBuffers(core.nz, core.nx, separation, wv, (), 0, False, rbuf=bufs[wv],
+                               nslots=k)
+        img.s.t0 = start
+        b = dev.Batch([core], imgs=[img])
+        core.cur = b.run(1, start, end, core.cur, wavelets=[w], op_after=L.IMG_EXTRACT,
+                         restore=[store.arena], restore_stride=store.state_floats,
+                         restore_slot=store.states[block_index][1])
+        b.close()
+    prop.step_index = store.states[block_index][0] + 1 + k
+    host = {wv: bufs[wv].cpu().numpy() for wv in waves}
+    return [{wv: host[wv][i] for wv in waves} for i in range(k)]
+
+
+# ---------------------------------------------------------------------------
+# imager
+# ---------------------------------------------------------------------------
+class RTMImager:
+    """Single-shot two-way-wave RTM on the GPU (ertm/imaging.py:168-563)."""
+
+    name = "rtm-imager"
+
+    def __init__(self, imaging_config: ImagingConfig, solver_config: SolverConfig | None = None,
+                 q_config: QConfig | None = None, gpu_config: GpuConfig | None = None) -> None:
+        self.config = imaging_config
+        self.config.validate()
+        self.solver_config = solver_config or SolverConfig()
+        self.solver_config.validate()
+        self.q_config = q_config or QConfig()
+        self.q_config.validate()
+        self.gpu_config = gpu_config or GpuConfig()
+        self.gpu_config.validate()
+        self._validate_engine_mode()
+        self.last_timing: dict[str, float] = {}
+
+    def q_options(self, *, compensation: bool = False) -> QOptions | None:
+        """GSLS options for the viscoacoustic engine (ertm/imaging.py:187-202)."""
+        if self.solver_config.engine != "viscoacoustic":
+            return None
+        q = self.q_config
+        return QOptions(enabled=True, f_ref=float(q.f_ref), n_mechanisms=int(q.n_mechanisms),
+                        band_ratio=float(q.band_ratio),
+                        compensation=bool(compensation and q.enabled), alpha_c=float(q.alpha_c),
+                        amplitude_cap=float(q.amplitude_cap),
+                        lowpass_interval=int(q.lowpass_interval),
+                        f_max=None if q.f_max is None else float(q.f_max),
+                        gain_budget_db=float(q.gain_budget_db))
+
+    # ------------------------------------------------------------ public API
+    def migrate(self, shot: ShotRecord, model: ElasticModel, *, wavelet: np.ndarray | None = None,
+                preprocess: Callable[[np.ndarray, float], np.ndarray] | None = None) -> ImageResult:
+        """Migrate one shot record into a depth image (ertm/imaging.py:205-349)."""
+        return self.migrate_modes(shot, model, (self.config.mode,), wavelet=wavelet,
+                                  preprocess=preprocess)[self.config.mode]
+
+    def migrate_modes(self, shot: ShotRecord, model: ElasticModel, modes: tuple[str, ...], *,
+                      wavelet: np.ndarray | None = None,
+                      preprocess: Callable[[np.ndarray, float], np.ndarray] | None = None
+                      ) -> dict[str, ImageResult]:
+        """Several wave modes sharing the source wave (e.g. pp + ps) in one pass.
+
+        Each image equals what ``migrate`` returns for that mode alone; the
+        forward, replay and backward propagations are shared.
+        """
+        t_start = time.perf_counter()
+        try:
+            shot.validate()
+            model.validate()
+        except ValueError as exc:
+            raise ImagingError(f"invalid imaging input: {exc}") from exc
+        spec = PropagationSpec.from_config(self.solver_config, model)
+        try:
+            if self.gpu_config.strict_cfl:
+                check_stability(model, spec)
+            else:
+                check_hard_stability(model, spec)
+        except Exception as exc:  # SolverError -> ImagingError
+            raise ImagingError(str(exc)) from exc
+        engine = self.solver_config.engine
+        modes = tuple(str(m).lower() for m in modes)
+        for m in modes:
+            if m not in WAVE_MODES:
+                raise ImagingError(f"unsupported imaging mode {m!r}")
+            if engine in ("acoustic", "viscoacoustic") and m not in _ACOUSTIC_MODES:
+                raise ImagingError(f"acoustic/viscoacoustic engines image PP only, got {m!r}")
+        swaves = {WAVE_MODES[m][0] for m in modes}
+        if len(swaves) != 1 or len(modes) > 2:
+            raise ImagingError("fused modes must share the source wave (at most two modes)")
+        swave = swaves.pop()
+        rwaves = tuple(WAVE_MODES[m][1] for m in modes)
+        condition = str(self.config.condition).lower()
+        separation = self.config.separation
+
+        record = self._prepare_record(shot, spec, preprocess)
+        if not record:
+            raise ImagingError("empty shot record: nothing to back-propagate")
+        wave = self._prepare_wavelet(shot, spec, wavelet)
+        img, info = self._run_shot(model, spec, engine, shot, record, wave, swave, rwaves,
+                                   condition, separation)
+        results = {}
+        for k, m in enumerate(modes):
+            image = self._postprocess(img[k], model, spec)
+            res = ImageResult(image=np.asarray(image, dtype=np.float32), grid=model.grid, mode=m,
+                              meta=self._meta(spec, engine, shot, info, condition, m))
+            res.validate()
+            results[m] = res
+        self.last_timing["total_s"] = time.perf_counter() - t_start
+        for r in results.values():
+            r.meta["perf"] = dict(self.last_timing)
+        return results
+
+    def image_condition(self, source_wavefield: np.ndarray, receiver_wavefield: np.ndarray):
+        """Per-time-step integrand (ertm/imaging.py:351-368)."""
+        s = np.asarray(source_wavefield)
+        r = np.asarray(receiver_wavefield)
+        if s.shape != r.shape:
+            raise ImagingError(f"imaging-condition inputs differ in shape: {s.shape} vs {r.shape}")
+        return zero_lag_condition(s, r).astype(np.float32)
+
+    # ------------------------------------------------------------ engine
+    def _routing(self, name: str) -> str:
+        """Record name -> injected component (ertm/imaging.py:440-445)."""
+        if name in ("primary", "p") and self.solver_config.engine == "acoustic":
+            return "p"
+        if name == "primary":
+            return "vz"
+        return name
+
+    def _run_shot(self, model, spec, engine, shot, record, wave, swave, rwaves, condition,
+                  separation):
+        import torch
+
+        timing = {}
+        t0 = time.perf_counter()
+        nt = int(spec.nt)
+        nz, nx = model.shape
+        fwd = build_propagator(model, spec, engine=engine, q_options=self.q_options(compensation=False))
+        fwd.set_source(shot.source)
+        fwd.set_receivers(shot.receivers)
+        store = record_source_wavefield(fwd, wave, budget_mb=float(self.config.checkpoint_mb))
+        back = build_propagator(model, spec, engine=engine, q_options=self.q_options(compensation=True))
+        back.set_source(shot.source)
+        back.set_receivers(shot.receivers)
+        names = list(record)
+        recs = [dev.to_device(np.asarray(record[n], dtype=np.float32)) for n in names]
+        tab = dev.RecvTables()
+        tab.build_injection(back._core, shot.receivers.x, shot.receivers.z,
+                            [self._routing(n) for n in names], back._k_relaxed_host())
+        tab.set_records(recs)
+        start_step = int(np.floor((float(shot.source.delay) + float(self.config.start_periods)
+                                   / max(float(shot.source.f0), 1e-6)) / spec.dt))
+        start_step = int(np.clip(start_step, 0, nt))
+        illum = condition == "normalized" or bool(self.config.receiver_normalization)
+        fcore, bcore = fwd._core, back._core
+        bkw = back._run_kwargs()
+        back_steps = 0
+        nblocks = len(store.states)
+        w = np.asarray(wave, dtype=np.float32)
+        replayed = 0
+        torch.cuda.synchronize()
+        timing["forward_s"] = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        if condition == "excitation":
+            ib = dev.ImageBuffers(nz, nx, separation, "P", rwaves[:1], start_step, False,
+                                  excitation=True)
+            fb = dev.Batch([fcore], imgs=[ib])
+            for blk in range(nblocks):
+                s0, s1 = store.bounds(blk, nt)
+                fcore.cur = fb.run(1, s0, s1, fcore.cur, wavelets=[w], op_after=L.IMG_EXC_PEAK,
+                                   restore=[store.arena], restore_stride=store.state_floats,
+                                   restore_slot=store.states[blk][1])
+                replayed += s1 - s0
+            fb.close()
+            bb = dev.Batch([bcore], recvs=[tab], imgs=[ib])
+            for blk in range(nblocks - 1, -1, -1):
+                s0, s1 = store.bounds(blk, nt)
+                bcore.cur = bb.run(2, s0, s1, bcore.cur, op_after=L.IMG_EXC_APPLY,
+                                   step_index0=back_steps, **bkw)
+                back_steps += s1 - s0
+            bb.close()
+            num_t = ib.num
+            ill = ib.peak_amp ** 2
+            rill = ib.rill
+        else:
+            slots = self._replay_slots(store.interval, nz * nx)
+            rbuf = torch.empty((slots, nz, nx), dtype=torch.float32, device="cuda")
+            ib = dev.ImageBuffers(nz, nx, separation, swave, rwaves, start_step, illum, rbuf=rbuf,
+                                  nslots=slots)
+            fimg = dev.ImageBuffers(nz, nx, separation, swave, (), 0, False, rbuf=rbuf,
+                                    nslots=slots)
+            fb = dev.Batch([fcore], imgs=[fimg])
+            bb = dev.Batch([bcore], recvs=[tab], imgs=[ib])
+            sub = None
+            if slots < store.interval:
+                nsub = int(math.ceil(store.interval / slots))
+                sub = torch.empty((nsub, fcore.state_floats), dtype=torch.float32, device="cuda")
+            for blk in range(nblocks - 1, -1, -1):
+                s0, s1 = store.bounds(blk, nt)
+                segs = [(a, min(a + slots, s1)) for a in range(s0, s1, slots)]
+                if len(segs) > 1:
+                    # sub-checkpoints: replay state just before w[a] of each later segment
+                    fcore.cur = fb.run(1, s0, segs[-1][0], fcore.cur, wavelets=[w],
+                                       ck_interval=slots, ck_origin=s0 + slots - 1, save=[sub],
+                                       save_stride=fcore.state_floats, restore=[store.arena],
+                                       restore_stride=store.state_floats,
+                                       restore_slot=store.states[blk][1])
+                    replayed += segs[-1][0] - s0
+                for k in range(len(segs) - 1, -1, -1):
+                    a, e = segs[k]
+                    fimg.s.t0 = a
+                    ib.s.t0 = a
+                    fb.update_images([fimg])
+                    bb.update_images([ib])
+                    if k == 0:
+                        src = dict(restore=[store.arena], restore_stride=store.state_floats,
+                                   restore_slot=store.states[blk][1])
+                    else:
+                        src = dict(restore=[sub], restore_stride=fcore.state_floats,
+                                   restore_slot=k - 1)
+                    fcore.cur = fb.run(1, a, e, fcore.cur, wavelets=[w], op_after=L.IMG_EXTRACT,
+                                       **src)
+                    replayed += e - a
+                    bcore.cur = bb.run(2, a, e, bcore.cur, op_after=L.IMG_CORRELATE,
+                                       step_index0=back_steps, **bkw)
+                    back_steps += e - a
+            fb.close()
+            bb.close()
+            num_t, ill, rill = ib.num, ib.ill, ib.rill
+        torch.cuda.synchronize()
+        timing["replay_backward_s"] = time.perf_counter() - t1
+        back.step_index = back_steps
+        back.pml.touched = True
+        eps = float(self.config.eps)
+        outs = []
+        ill_h = ill.cpu().numpy()
+        for k in range(len(rwaves)):
+            image = num_t[k].cpu().numpy()
+            if condition == "normalized" and self.config.illumination:
+                image = image / (ill_h + eps)
+            if self.config.receiver_normalization:
+                image = image / (rill[k].cpu().numpy() + eps)
+            outs.append(image)
+        info = dict(store=store, start_step=start_step, prop=fwd, back=back,
+                    replayed_steps=replayed, nt=nt, cell_updates=(nt + replayed + back_steps) * nz * nx)
+        timing["cell_updates"] = float(info["cell_updates"])
+        self.last_timing = timing
+        return outs, info
+
+    def _replay_slots(self, interval: int, cells: int) -> int:
+        """Replay slots per segment: a whole block unless over the replay budget."""
+        import torch
+
+        if self.gpu_config.replay_slots > 0:
+            return max(1, min(interval, int(self.gpu_config.replay_slots)))
+        free, _total = torch.cuda.mem_get_info()
+        budget = min(16 * 1024 ** 3, 0.25 * free)
+        return max(1, min(interval, int(budget // (4 * cells))))
+
+    # ------------------------------------------------------------ helpers
+    def _validate_engine_mode(self) -> None:
+        if (self.solver_config.engine in ("acoustic", "viscoacoustic")
+                and self.config.mode not in _ACOUSTIC_MODES):
+            raise ImagingError(
+                f"acoustic/viscoacoustic engines image PP only, got imaging.mode="
+                f"{self.config.mode!r}; use solver.engine='elastic' for converted waves")
+
+    def _resolve_mode(self) -> tuple[str, str]:
+        mode = str(self.config.mode).lower()
+        if mode not in WAVE_MODES:
+            raise ImagingError(f"unsupported imaging mode {mode!r}")
+        return WAVE_MODES[mode]
+
+    def _prepare_wavelet(self, shot: ShotRecord, spec: PropagationSpec,
+                         wavelet: np.ndarray | None) -> np.ndarray:
+        if wavelet is not None:
+            return np.asarray(wavelet, dtype=np.float32)
+        return source_wavelet(shot.source, spec.dt, spec.nt)
+
+    def _prepare_record(self, shot: ShotRecord, spec: PropagationSpec,
+                        preprocess: Callable[[np.ndarray, float], np.ndarray] | None
+                        ) -> dict[str, np.ndarray]:
+        """Align the record to (nt, dt): sinc resampling, optional preprocess,
+        truncate / zero-pad (ertm/imaging.py:396-423)."""
+        comps = dict(shot.components) or {"primary": shot.data}
+        out: dict[str, np.ndarray] = {}
+        for name, arr in comps.items():
+            data = np.asarray(arr, dtype=np.float64)
+            if abs(float(shot.dt) - spec.dt) > 1e-12 and data.shape[0] > 1:
+                data = num.resample_sinc(data, float(shot.dt), spec.dt)
+            if preprocess is not None:
+                data = np.asarray(preprocess(data, spec.dt), dtype=np.float64)
+            out[name] = self._fit_length(data, int(spec.nt))
+        return out
+
+    @staticmethod
+    def _fit_length(data: np.ndarray, nt: int) -> np.ndarray:
+        if data.shape[0] >= nt:
+            return data[:nt]
+        padded = np.zeros((nt, data.shape[1]), dtype=data.dtype)
+        padded[: data.shape[0]] = data
+        return padded
+
+    def _postprocess(self, image: np.ndarray, model: ElasticModel, spec: PropagationSpec):
+        """Laplacian, wavenumber low-pass, PML mask (ertm/imaging.py:476-490)."""
+        data = np.asarray(image, dtype=np.float64)
+        if self.config.laplacian:
+            data = num.laplacian(data, model.grid.dx, model.grid.dz)
+        if self.config.lowpass_fraction < 1.0:
+            data = num.lowpass_wavenumber(data, model.grid.dx, model.grid.dz,
+                                          self.config.lowpass_fraction)
+        return data * self._pml_mask(model, spec)
+
+    @staticmethod
+    def _pml_mask(model: ElasticModel, spec: PropagationSpec) -> np.ndarray:
+        """Zero npml physical cells at the absorbing sides (ertm/imaging.py:492-502)."""
+        nz, nx = model.shape
+        n = int(spec.npml)
+        mask = np.ones((nz, nx), dtype=np.float64)
+        mask[:, :n] = 0.0
+        mask[:, -n:] = 0.0
+        mask[-n:, :] = 0.0
+        if spec.boundary == "cpml":
+            mask[:n, :] = 0.0
+        return mask
+
+    def _meta(self, spec, engine, shot, info, condition, mode) -> dict[str, Any]:
+        q = self.q_config
+        q_meta: dict[str, Any] = {"model": str(q.model), "enabled": bool(q.enabled),
+                                  "alpha_c": float(q.alpha_c), "f_ref": float(q.f_ref)}
+        if engine == "viscoacoustic":
+            fs = getattr(info["prop"], "quality_summary", None)
+            if isinstance(fs, dict):
+                q_meta.update(fs)
+            bs = getattr(info["back"], "quality_summary", None)
+            if isinstance(bs, dict):
+                for key in ("compensation", "alpha_c", "alpha_mechanisms", "f_peak_mechanisms_hz",
+                            "f_max", "gain_budget_db"):
+                    if key in bs:
+                        q_meta[key] = bs[key]
+                q_meta["clipped_steps"] = int(bs.get("clipped", 0))
+                q_meta["lowpass_steps"] = int(bs.get("lowpass_steps", 0))
+        store: CheckpointStore = info["store"]
+        return {
+            "placeholder": False,
+            "method": "reverse-time migration (two-way wave equation)",
+            "imager": self.name,
+            "engine": engine,
+            "device": "hip-gfx950",
+            "condition": condition,
+            "separation": self.config.separation,
+            "modes": [mode],
+            "illumination": bool(self.config.illumination),
+            "receiver_normalization": bool(self.config.receiver_normalization),
+            "laplacian": bool(self.config.laplacian),
+            "lowpass_fraction": float(self.config.lowpass_fraction),
+            "checkpoints": store.summary(),
+            "start_step": int(info["start_step"]),
+            "start_periods": float(self.config.start_periods),
+            "q": q_meta,
+            "nt": int(spec.nt),
+            "dt": float(spec.dt),
+            "order": int(spec.order),
+            "npml": int(spec.npml),
+            "source_x": float(shot.source.x),
+            "source_z": float(shot.source.z),
+            "n_receivers": int(shot.receivers.nrec),
+            "replayed_steps": int(info["replayed_steps"]),
+            "replay_overhead": float(info["replayed_steps"]) / max(int(spec.nt), 1),
+        }
+
+
+def build_imager(config: ImagingConfig, solver_config: SolverConfig | None = None,
+                 q_config: QConfig | None = None, gpu_config: GpuConfig | None = None) -> RTMImager:
+    """ertm/imaging.py:566-572."""
+    return RTMImager(config, solver_config, q_config, gpu_config)

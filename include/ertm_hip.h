@@ -1,0 +1,309 @@
+/*
+ * ertm_hip.h -- C-ABI of the MI355X (gfx950) RTM engine, libertm_hip.so.
+ *
+ * The reference package is pure Python/NumPy, so it has no FFI of its own;
+ * each entry point below replaces one group of NumPy operations in the
+ * reference's hot path (citations are to the reference tree):
+ *
+ *   ertm_step            AcousticPropagator.step      ertm/propagation.py:430-454
+ *                        ElasticPropagator.step       ertm/propagation.py:840-883
+ *                        ViscoacousticPropagator.step ertm/propagation.py:686-723
+ *                        (with CPML2D.derivative      ertm/numerics.py:283-308)
+ *   ertm_stabilize       ViscoacousticPropagator._stabilize ertm/propagation.py:726-749
+ *   ertm_sample          Propagator2D._sample_field + sample()  ertm/propagation.py:371-391,
+ *                        :456-470, :751-762, :885-898 (bilinear_sample ertm/numerics.py:478-511)
+ *   ertm_inject          inject()/_add_at_nodes/_spread_into  ertm/propagation.py:331-369,
+ *                        :472-494, :764-774, :900-942 (np.add.at order kept)
+ *   ertm_image           image_fields() + the imaging-condition accumulation of
+ *                        RTMImager.migrate            ertm/propagation.py:496-497, :945-977;
+ *                        ertm/imaging.py:263-331, :450-474
+ *   ertm_copy_state      Propagator2D.state/load_state ertm/propagation.py:283-299
+ *                        (checkpoint save/restore, ertm/imaging.py:113-162)
+ *   ertm_run             the time loops of record_source_wavefield / replay_block /
+ *                        the backward loop of migrate  ertm/imaging.py:113-162, :279-324
+ *   ertm_derivative      dplus/dminus                  ertm/numerics.py:68-137
+ *   ertm_absmax          max_amplitude                 ertm/propagation.py:503-505
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a non-zero ERTM_E* code on error;
+ *     ertm_last_error() returns a thread-local message.  No C++ exception
+ *     crosses the ABI.
+ *   - All device memory named in the descriptors is allocated by the caller
+ *     (PyTorch tensors on the Python side); the library only keeps small
+ *     descriptor tables of its own.
+ *   - Work is enqueued asynchronously on the given HIP stream (NULL = the
+ *     default stream).  Nothing synchronises unless stated.
+ *   - Arrays on the padded grid are row-major with row stride `ld` floats
+ *     (x fastest); physical-grid arrays (images, accumulators) are dense
+ *     (nz, nx).
+ */
+#ifndef ERTM_HIP_H
+#define ERTM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ERTM_ABI_VERSION 1
+
+enum { ERTM_OK = 0, ERTM_EINVAL = 1, ERTM_EHIP = 2, ERTM_ENOMEM = 3, ERTM_ESTATE = 4 };
+
+enum { ERTM_ACOUSTIC = 0, ERTM_ELASTIC = 1, ERTM_VISCO = 2 };
+enum { ERTM_SRC_EXPLOSIVE = 0, ERTM_SRC_SHEAR = 1, ERTM_SRC_FORCE_Z = 2 };
+
+/* Field slab indices inside a state buffer. */
+enum { ERTM_AC_P = 0, ERTM_AC_VX = 1, ERTM_AC_VZ = 2 };
+enum { ERTM_EL_VX = 0, ERTM_EL_VZ = 1, ERTM_EL_SXX = 2, ERTM_EL_SZZ = 3, ERTM_EL_SXZ = 4 };
+enum { ERTM_VI_P = 0, ERTM_VI_VX = 1, ERTM_VI_VZ = 2, ERTM_VI_EPS = 3, ERTM_VI_R0 = 4 };
+
+#define ERTM_MAX_MECH 8
+
+/*
+ * One propagator: geometry, medium, CPML profiles, the two ping-pong state
+ * buffers and the point source.  A step reads state[cur] and writes
+ * state[cur ^ 1].
+ *
+ * State buffer layout (floats): `nfields` slabs of `fstride` floats, each an
+ * (nzp, ld) array; then the CPML memory arrays at the offsets psix/psiz.
+ * x-direction memories are (nzp, 2*wx) arrays covering columns [0, wx) and
+ * [nxp-wx, nxp); z-direction memories are (wzt+wzb, ld) arrays covering rows
+ * [0, wzt) and [nzp-wzb, nzp).  Outside those strips the reference's
+ * recursion is an exact no-op (a = 0, b = 1), so nothing is stored there.
+ *
+ * Medium (padded grid, stride ld):
+ *   acoustic  kappa = rho*vp*vp (f32 ops), irho = 1/rho
+ *   elastic   lam, mu, irho
+ *   visco     kappa = K_R (relaxed modulus), irho, qidx (Q class per cell),
+ *             q_a[nq*L] = f32 exp(-dt/tau), q_delta/q_tau[nq*L] (f64)
+ * Face averages (1/rho on x/z faces, mu on cell corners, lam+2mu) are
+ * recomputed in the kernels with the reference's float32 operation order.
+ */
+typedef struct ErtmProp {
+    int32_t engine;        /* ERTM_ACOUSTIC / ERTM_ELASTIC / ERTM_VISCO */
+    int32_t nzp, nxp, ld;  /* padded shape and row stride (floats) */
+    int32_t pad, nz, nx;   /* ghost width and physical shape */
+    int32_t order;         /* 2, 4, 6 or 8 */
+    int32_t free_surface;  /* 1: zero p (acoustic) or szz,sxz (elastic) on row `pad` */
+    int32_t nmech;         /* viscoacoustic mechanisms L (<= ERTM_MAX_MECH) */
+    int32_t wx, wzt, wzb;  /* CPML strip widths */
+    int32_t nfields;
+    float dx, dz, dt;      /* float32 values of the spacing and time step */
+    float coef[4];         /* float32 stencil weights, zero beyond order/2 */
+    double dt64;           /* the time step as given (float64), for the GSLS b_l */
+    int64_t fstride;       /* floats between field slabs */
+    int64_t psix[4];       /* offsets of the x-memories (see ertm_step) */
+    int64_t psiz[4];       /* offsets of the z-memories */
+    int64_t state_floats;  /* floats in one state buffer */
+    const float* kappa;
+    const float* irho;
+    const float* lam;
+    const float* mu;
+    const int32_t* qidx;
+    const float* q_a;
+    const double* q_delta;
+    const double* q_tau;
+    const float* ax;       /* CPML a, b per padded column (nxp) */
+    const float* bx;
+    const float* az;       /* CPML a, b per padded row (nzp) */
+    const float* bz;
+    float* state[2];
+    int32_t src_iz, src_ix; /* padded source node, -1 = none */
+    int32_t mechanism;      /* ERTM_SRC_* (elastic) */
+    int32_t _pad0;
+} ErtmProp;
+
+/*
+ * Receiver tables of one propagator (built on the host in float64 exactly as
+ * the reference computes fractional indices).
+ *
+ * Sampling: nsamp entries; entry e reads padded corner (i0[e], j0[e]) of
+ * field `sfield[e]` (-1: acoustic "sxz", always 0; -2: elastic pressure
+ * 0.5*(sxx+szz)) with weights tz[e], tx[e] (valid[e]==0 -> 0) and writes
+ * record `srec[e]` at column `scol[e]`.
+ *
+ * Injection: for each target cell u (ncells), entries [cell_ptr[u],
+ * cell_ptr[u+1]) are applied in order to field slab `cell_field[u]` at flat
+ * padded offset `cell_off[u]`:  value = record[ecomp[e]][t][erec[e]] (f32),
+ *   mode 0: c = (value * wa[e]) * wb[e]   (acoustic / elastic scatter)
+ *   mode 1: c = value * wa[e]             (viscoacoustic _add_at_nodes)
+ *   field = (float)((double)field + c)
+ * which is np.add.at's order and rounding.
+ */
+typedef struct ErtmRecv {
+    int32_t nsamp;
+    int32_t ncells;
+    int32_t nentries;
+    int32_t inject_mode;
+    int32_t nrec;
+    int32_t ncomp;
+    const int32_t* s_i0;
+    const int32_t* s_j0;
+    const int32_t* s_field;
+    const int32_t* s_rec;
+    const int32_t* s_col;
+    const int32_t* s_valid;
+    const double* s_tz;
+    const double* s_tx;
+    const int32_t* cell_ptr;
+    const int32_t* cell_field;
+    const int64_t* cell_off;
+    const int32_t* e_comp;
+    const int32_t* e_rec;
+    const double* e_wa;
+    const double* e_wb;
+    float* records[8];       /* per component: (nt, nrec) float32 */
+} ErtmRecv;
+
+/* Image-field selectors. */
+enum { ERTM_WAVE_P = 0, ERTM_WAVE_S = 1 };
+enum { ERTM_SEP_STRESS = 0, ERTM_SEP_DIVCURL = 1 };
+enum {
+    ERTM_IMG_EXTRACT = 0,      /* write the source field into rbuf[t - t0] */
+    ERTM_IMG_CORRELATE = 1,    /* accumulate zero_lag / normalized terms */
+    ERTM_IMG_EXC_PEAK = 2,     /* excitation: update running/peak maps */
+    ERTM_IMG_EXC_APPLY = 3     /* excitation: masked accumulation */
+};
+
+/*
+ * Imaging buffers of one propagator (physical grid, dense nz*nx).
+ *   rbuf      replay slots (K, nz, nx) float32, source-side field per step
+ *   num[2]    cross-correlation numerators (float64), one per receiver wave
+ *   ill       source illumination (float64); rill[2] receiver illumination
+ *   peak_step int32, peak_amp / running float64 (excitation)
+ */
+typedef struct ErtmImage {
+    int32_t separation;    /* ERTM_SEP_* (elastic) */
+    int32_t swave;         /* source-side wave (ERTM_WAVE_*) */
+    int32_t nrw;           /* receiver waves accumulated (1 or 2) */
+    int32_t rwave[2];
+    int32_t start_step;    /* cross term only for t >= start_step */
+    int32_t illum;         /* 1: accumulate ill and rill every step */
+    int32_t t0;            /* time index of replay slot 0 */
+    int32_t nslots;        /* K */
+    int32_t _pad0;
+    float* rbuf;
+    double* num[2];
+    double* ill;
+    double* rill[2];
+    int32_t* peak_step;
+    double* peak_amp;
+    double* running;
+} ErtmImage;
+
+typedef struct ertm_batch_s* ertm_batch_t;
+
+/* ---- library ---------------------------------------------------------- */
+int ertm_abi_version(void);
+/* sizeof(ErtmProp), sizeof(ErtmRecv), sizeof(ErtmImage), sizeof(ErtmRun),
+ * offsetof(ErtmProp, state), offsetof(ErtmRun, ctl): lets a binding check
+ * its structure layout without a GPU. */
+int ertm_struct_sizes(int64_t* out6);
+const char* ertm_last_error(void);
+int ertm_device_info(int32_t* device, int32_t* n_cu, char* name, size_t name_len);
+
+/* ---- batches: n propagators advanced together --------------------------- */
+/* Copies the descriptors to device memory.  recv/img may be NULL (or an
+ * entry may have nsamp=ncells=0 / all-NULL buffers). */
+int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmImage* img,
+                      int32_t n, ertm_batch_t* out);
+int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img);
+int ertm_batch_destroy(ertm_batch_t b);
+/* Kernel variant / tile shape of the stencil kernels (0 = library default);
+ * see ertm_kernel_variants for the list. */
+int ertm_batch_set_variant(ertm_batch_t b, int32_t variant);
+
+/*
+ * Per-launch time table: propagator i of the batch is at time index t[i] and
+ * its point source adds amp[i] (0 = no source).  At most ERTM_MAX_LAUNCH
+ * propagators are advanced per launch; larger batches are split internally.
+ */
+#define ERTM_MAX_LAUNCH 64
+typedef struct ErtmTimes {
+    int32_t t[ERTM_MAX_LAUNCH];
+    float amp[ERTM_MAX_LAUNCH];
+} ErtmTimes;
+
+/*
+ * One time step for every propagator of the batch: state[cur] -> state[cur^1].
+ * amp (host, n entries, NULL = all 0) is added exactly as the reference adds
+ * float32(amp) at the source node.
+ */
+int ertm_step(ertm_batch_t b, int32_t cur, const float* amp, void* stream);
+
+/* Viscoacoustic compensation stabiliser after a step (state[cur] is the
+ * post-step state).  step_index = the propagator's step counter before
+ * increment; ctl (device, 4 doubles per propagator): [ref_amp, clipped,
+ * lowpass_steps, unused]; cap = amplitude_cap; interval = lowpass_interval. */
+int ertm_stabilize(ertm_batch_t b, int32_t cur, int32_t step_index, double cap,
+                   int32_t interval, double* ctl, void* stream);
+
+/* Sample every receiver of every propagator from state[cur] into record row
+ * t[i] (host array, n entries). */
+int ertm_sample(ertm_batch_t b, int32_t cur, const int32_t* t, void* stream);
+/* Inject record row t[i] into state[cur] (before the backward step). */
+int ertm_inject(ertm_batch_t b, int32_t cur, const int32_t* t, void* stream);
+/* Imaging operation `op` (ERTM_IMG_*) at time index t[i] on state[cur]. */
+int ertm_image(ertm_batch_t b, int32_t cur, const int32_t* t, int32_t op, void* stream);
+/* Copy a whole state buffer (checkpoint save: dst = arena slot, src =
+ * state[cur]; restore: the reverse).  Plain device-to-device copy. */
+int ertm_copy_state(float* dst, const float* src, int64_t nfloats, void* stream);
+
+/*
+ * Time loops (one HIP launch sequence per step, no host synchronisation):
+ *   phase 0 (forward):   for t in [t_begin, t_end): step(amp = w[t]);
+ *                        sample(t) if do_sample; image(op_after, t) if >= 0;
+ *                        checkpoint (below).
+ *   phase 1 (replay):    first copy restore_arenas[i] + restore_slot *
+ *                        restore_stride into state[cur0], then as phase 0.
+ *   phase 2 (backward):  for t = t_end-1 .. t_begin: inject(t); step(0);
+ *                        image(op_after, t).
+ * Checkpoints (phases 0, 1): with ck_interval > 0, after step t with
+ * t >= ck_origin and (t - ck_origin) % ck_interval == 0 the new state is
+ * copied to save_arenas[i] + ((t - ck_origin) / ck_interval) * save_stride.
+ * With stabilize = 1 (viscoacoustic compensation) ertm_stabilize runs after
+ * every step with the step counter starting at step_index0.
+ * `wavelets` is a host array of n pointers to host float32 wavelets of length
+ * nw (unused in phase 2).  The parity of the final state goes to *cur_out.
+ */
+typedef struct ErtmRun {
+    int32_t phase;
+    int32_t t_begin, t_end;
+    int32_t cur0;
+    int32_t do_sample;
+    int32_t op_after;          /* -1 none, else ERTM_IMG_* */
+    int32_t ck_interval;       /* 0 = no checkpoints */
+    int32_t ck_origin;
+    int64_t save_stride;       /* floats between saved slots */
+    float* const* save_arenas;
+    int32_t restore_slot;
+    int32_t nw;                /* wavelet length */
+    int64_t restore_stride;
+    float* const* restore_arenas;
+    const float* const* wavelets;
+    int32_t stabilize;         /* viscoacoustic compensation stabiliser */
+    int32_t step_index0;
+    int32_t lp_interval;
+    int32_t _pad1;
+    double cap;
+    double* ctl;               /* device, 4 doubles per propagator */
+} ErtmRun;
+int ertm_run(ertm_batch_t b, const ErtmRun* run, int32_t* cur_out, void* stream);
+
+/* ---- utilities ----------------------------------------------------------- */
+/* Staggered derivative of a dense (n0, n1) array (dplus/dminus): axis 0 or
+ * 1, plus = 1 for the value at i+1/2; is_f64 selects float64 data (else
+ * float32, where h is rounded to float32 first, as NumPy does). */
+int ertm_derivative(const void* f, void* out, int32_t n0, int32_t n1, int32_t axis,
+                    int32_t plus, int32_t order, double h, int32_t is_f64, void* stream);
+/* max |x| over n floats into *out (device float). */
+int ertm_absmax(const float* x, int64_t n, float* out, void* stream);
+/* Block until all work on the stream is done. */
+int ertm_sync(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ERTM_HIP_H */

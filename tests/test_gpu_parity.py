@@ -1,0 +1,229 @@
+"""GPU parity: the HIP engine against the reference's golden vectors and the
+oracle.  Bar: bit-exact (``np.array_equal``) for every float32 propagation,
+sampling, injection, replay and imaging result; the viscoacoustic
+compensation path (hipFFT low-pass vs NumPy's pocketfft) is held to a
+relative L2 tolerance of 1e-5 on records and 1e-4 on images.
+"""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from golden_cases import forward_case, load, migrate_case, tags
+from product_cases import migrate_objects, propagator_of, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+VISCO_COMP_TOL_RECORD = 1e-5
+VISCO_COMP_TOL_IMAGE = 1e-4
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _engine():
+    from ertm import _lib
+
+    _lib.lib()
+
+
+@pytest.mark.parametrize("tag", tags("forward"))
+def test_forward_matches_reference(tag):
+    _say("forward golden", tag)
+    from ertm.propagation import source_wavelet
+
+    c = forward_case(tag)
+    p, spec = propagator_of(c)
+    from product_cases import source_of
+
+    w = source_wavelet(source_of(c), spec.dt, spec.nt)
+    rec = p.run_steps(w, c["components"])
+    comp = c.get("q", {}).get("compensation", False)
+    for name in c["components"]:
+        if comp:
+            assert rel_l2(rec[name], c["rec"][name]) < VISCO_COMP_TOL_RECORD, name
+        else:
+            assert np.array_equal(rec[name], c["rec"][name]), name
+    if comp:
+        s = p.quality_summary
+        assert [s["clipped"], s["lowpass_steps"]] == list(c["visco_summary"])
+        return
+    for name, ref in c["state"].items():
+        assert np.array_equal(getattr(p, name), ref), name
+    psi = {k: p._core.expand_psi(k, v.cpu().numpy()) for k, v in p._core.psi_views().items()}
+    for name, ref in c["psi"].items():
+        got = psi.get(name, np.zeros_like(ref))
+        assert np.array_equal(got, ref), name
+
+
+def test_stepwise_api_matches_run_loop():
+    """prop.step()+prop.sample() (the reference's loop) == the C-side loop."""
+    _say("stepwise vs loop")
+    from ertm.propagation import source_wavelet
+    from product_cases import source_of
+
+    c = forward_case("el_shear")
+    p, spec = propagator_of(c)
+    w = source_wavelet(source_of(c), spec.dt, spec.nt)
+    rec = {n: np.zeros((spec.nt, len(c["rx"])), np.float32) for n in c["components"]}
+    for s in range(spec.nt):
+        p.step(float(w[s]))
+        smp = p.sample(c["components"])
+        for n in c["components"]:
+            rec[n][s] = smp[n]
+    for n in c["components"]:
+        assert np.array_equal(rec[n], c["rec"][n]), n
+
+
+@pytest.mark.parametrize("tag", ["ac", "el", "vi"])
+def test_sample_and_inject_match_reference(tag):
+    _say("inject golden", tag)
+    from ertm.types import ReceiverSpec
+
+    d = load("inject")
+    case = {"ac": "ac_cpml", "el": "el_shear", "vi": "vi_fwd"}[tag]
+    c = forward_case(case)
+    p, _ = propagator_of(c)
+    p.set_receivers(ReceiverSpec(x=d[f"{tag}/rx"], z=d[f"{tag}/rz"]))
+    for name in p._field_names():
+        p.set_field(name, d[f"{tag}/before_{name}"])
+    comps = [str(v) for v in d[f"{tag}/components"]]
+    for comp in comps:
+        assert np.array_equal(p.sample((comp,))[comp], d[f"{tag}/sample_{comp}"]), comp
+    for comp in comps:
+        p.inject(comp, d[f"{tag}/values_{comp}"])
+    for name in p._field_names():
+        assert np.array_equal(getattr(p, name), d[f"{tag}/after_{name}"]), name
+
+
+def test_replay_blocks_match_reference():
+    _say("replay golden")
+    from ertm.imaging import record_source_wavefield, replay_block
+    from ertm.propagation import source_wavelet
+    from product_cases import source_of
+
+    d = load("replay")
+    c = forward_case("ac_cpml")
+    p, spec = propagator_of(c)
+    w = source_wavelet(source_of(c), spec.dt, spec.nt)
+    store = record_source_wavefield(p, w, budget_mb=1.0)
+    assert store.interval == int(d["interval"])
+    assert [s for s, _ in store.states] == list(d["starts"])
+    for b in (0, 3, len(store.states) - 1):
+        fields = replay_block(p, w, store, b, "stress")
+        assert np.array_equal(np.stack([f["P"] for f in fields]), d[f"block{b}"]), b
+
+
+@pytest.mark.parametrize("tag", tags("migrate"))
+def test_migrate_matches_reference(tag):
+    _say("migrate golden", tag)
+    from ertm.imaging import RTMImager
+
+    c = migrate_case(tag)
+    shot, model, imaging, solver, q = migrate_objects(c)
+    res = RTMImager(imaging, solver, q).migrate(shot, model)
+    assert (res.meta["checkpoints"]["interval"], res.meta["checkpoints"]["n_states"]) == \
+        c["checkpoints"]
+    assert res.meta["start_step"] == c["start_step"]
+    if c["engine"] == "viscoacoustic" and c["q"]["enabled"]:
+        assert rel_l2(res.image, c["image"]) < VISCO_COMP_TOL_IMAGE
+    else:
+        assert np.array_equal(res.image, c["image"])
+
+
+@pytest.mark.parametrize("tag", ["ac_zero_lag", "el_ps_divcurl"])
+def test_nested_replay_is_bit_identical(tag):
+    """Segmented replay from sub-checkpoints reproduces the same image."""
+    _say("nested replay", tag)
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case(tag)
+    shot, model, imaging, solver, q = migrate_objects(c)
+    res = RTMImager(imaging, solver, q, GpuConfig(replay_slots=3)).migrate(shot, model)
+    assert res.meta["replay_overhead"] > 1.0
+    assert np.array_equal(res.image, c["image"])
+
+
+def test_fused_pp_ps_equals_separate_runs():
+    _say("fused pp+ps")
+    from ertm.imaging import RTMImager
+
+    c = migrate_case("el_pp_divcurl")
+    shot, model, imaging, solver, q = migrate_objects(c)
+    both = RTMImager(imaging, solver, q).migrate_modes(shot, model, ("pp", "ps"))
+    assert np.array_equal(both["pp"].image, c["image"])
+    assert np.array_equal(both["ps"].image, migrate_case("el_ps_divcurl")["image"])
+
+
+def test_checkpoint_restore_is_bit_exact():
+    """state()/load_state() then continue == uninterrupted run (tests/test_rtm.py:188-214)."""
+    _say("checkpoint restore")
+    from ertm.propagation import source_wavelet
+    from product_cases import source_of
+
+    c = forward_case("ac_cpml")
+    a, spec = propagator_of(c)
+    b, _ = propagator_of(c)
+    w = source_wavelet(source_of(c), spec.dt, spec.nt)
+    for s in range(60):
+        a.step(float(w[s]))
+    b.step(float(w[0]))
+    st = b.state()
+    b.load_state(st)
+    for s in range(1, 60):
+        b.step(float(w[s]))
+    for name in ("p", "vx", "vz"):
+        assert np.array_equal(getattr(a, name), getattr(b, name))
+    expected = sum(getattr(b, n).nbytes for n in ("p", "vx", "vz")) + b.pml.memory_bytes()
+    assert b.state_bytes() == expected
+
+
+def test_derivatives_match_reference():
+    _say("derivatives")
+    from ertm import numerics as num
+
+    d = load("numerics")
+    f = d["deriv_field"]
+    for order in (2, 4, 6, 8):
+        for axis in (0, 1):
+            cs = num.staggered_coefficients(order)
+            assert np.array_equal(num.dplus(f, axis, cs, 3.7), d[f"dplus_o{order}_a{axis}"])
+            assert np.array_equal(num.dminus(f, axis, cs, 3.7), d[f"dminus_o{order}_a{axis}"])
+
+
+def test_acoustic_c1_shape_forward_matches_oracle():
+    """Config-1-shaped acoustic run (200 x 300, CPML 20) against the oracle, 300 steps."""
+    _say("C1-shaped forward vs oracle")
+    from ertm.propagation import PropagationSpec, build_propagator, source_wavelet
+    from ertm.types import ElasticModel, GridSpec, ReceiverSpec, SourceSpec
+    from oracle import rtm as ortm
+
+    nz, nx = 200, 300
+    z = np.arange(nz)[:, None] * 5.0
+    vp = np.select([z < 250, z < 500, z < 750], [2000.0, 2500.0, 3000.0], 3500.0) * np.ones((1, nx))
+    vp = vp.astype(np.float32)
+    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    m = ElasticModel(GridSpec(nx=nx, nz=nz, dx=5.0, dz=5.0), vp=vp, vs=(vp / 1.8).astype(np.float32),
+                     rho=rho, qp=np.full_like(vp, 100.0), qs=np.full_like(vp, 50.0))
+    spec = PropagationSpec(dt=5e-4, nt=300, order=8, npml=20, boundary="cpml")
+    src = SourceSpec(x=750.0, z=10.0, f0=25.0, delay=0.048)
+    rec = ReceiverSpec(x=np.arange(300) * 5.0, z=np.full(300, 10.0))
+    p = build_propagator(m, spec, engine="acoustic")
+    p.set_source(src)
+    p.set_receivers(rec)
+    w = source_wavelet(src, spec.dt, spec.nt)
+    got = p.run_steps(w, ("p",))["p"]
+    om = ortm.Model(nz, nx, 5.0, 5.0, vp, m.vs, rho, m.qp)
+    op = ortm.build("acoustic", om, spec.dt, spec.nt, 8, 20, "cpml")
+    op.set_source(src.x, src.z)
+    op.set_receivers(rec.x, rec.z)
+    ref = ortm.forward(op, w, ("p",))["p"]
+    assert np.array_equal(got, ref)
+    assert np.array_equal(p.p, op.p)

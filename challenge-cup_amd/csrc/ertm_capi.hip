@@ -64,6 +64,9 @@ struct ertm_batch_s {
     int max_tiles = 0, max_samp = 0, max_cells = 0, max_img = 0;
     bool any_force = false;
     bool all_visco = false;
+    bool all_elastic = false;
+    int variant = 0;
+    int zc = 64;
     float* scratch = nullptr;  // per propagator: peak, scale
     std::map<std::pair<int, int>, FftCache> fft;
     ~ertm_batch_s() {
@@ -185,6 +188,7 @@ int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmIma
     if (img) b->img.assign(img, img + n);
     b->order = props[0].order;
     b->all_visco = true;
+    b->all_elastic = true;
     for (int k = 0; k < n; ++k) {
         int rc = check_prop(b->props[k], k);
         if (rc == ERTM_OK) rc = check_recv(b->recv[k], b->props[k], k);
@@ -202,6 +206,7 @@ int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmIma
         if (P.engine == ERTM_ELASTIC && P.mechanism == ERTM_SRC_FORCE_Z && P.src_iz >= 0)
             b->any_force = true;
         if (P.engine != ERTM_VISCO) b->all_visco = false;
+        if (P.engine != ERTM_ELASTIC) b->all_elastic = false;
     }
     hipError_t e1 = hipMalloc((void**)&b->dprops, sizeof(ErtmProp) * n);
     hipError_t e2 = hipMalloc((void**)&b->drecv, sizeof(ErtmRecv) * n);
@@ -238,8 +243,16 @@ int ertm_batch_destroy(ertm_batch_t b) {
     return ERTM_OK;
 }
 
-int ertm_batch_set_variant(ertm_batch_t b, int32_t variant) {
-    if (!b || variant != 0) return fail(ERTM_EINVAL, "unknown kernel variant %d", variant);
+int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc) {
+    if (!b) return fail(ERTM_EINVAL, "set_variant: NULL batch");
+    if (variant < 0 || variant > 3) return fail(ERTM_EINVAL, "unknown kernel variant %d", variant);
+    if (variant != 0 && (!b->all_elastic || b->order != 8))
+        return fail(ERTM_EINVAL, "fused variants need an all-elastic, order-8 batch");
+    if (zc < 8 || zc > 4096) return fail(ERTM_EINVAL, "z chunk %d out of [8, 4096]", zc);
+    if (variant != 0 && zc % 9 != 1)
+        return fail(ERTM_EINVAL, "fused variants need a z chunk with zc %% 9 == 1, got %d", zc);
+    b->variant = variant;
+    b->zc = zc;
     return ERTM_OK;
 }
 
@@ -256,7 +269,9 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s) {
         la.n = std::min(ERTM_MAX_LAUNCH, n - base);
         la.cur = cur;
         for (int k = 0; k < la.n; ++k) la.amp[k] = amp ? amp[base + k] : 0.0f;
-        launch_step(b->dprops, la, b->order, b->max_tiles, b->any_force, s);
+        if (b->variant == 0 ||
+            !launch_step_fused(b->dprops, b->props.data(), la, b->order, b->variant, b->zc, s))
+            launch_step(b->dprops, la, b->order, b->max_tiles, b->any_force, s);
         LAUNCHCHK("step");
     }
     return ERTM_OK;

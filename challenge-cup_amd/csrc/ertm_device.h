@@ -15,34 +15,64 @@
 
 namespace ertm {
 
+// Global-address-space views of device pointers.  Pointers read from the
+// descriptor tables are generic (flat) to the compiler; flat memory
+// operations also count against the LDS counter, so every LDS barrier would
+// wait for all in-flight global loads.  Casting restores global_load/store.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const float gcfloat;
+__device__ __forceinline__ gfloat* gp(float* p) { return (gfloat*)p; }
+__device__ __forceinline__ gcfloat* gp(const float* p) { return (gcfloat*)p; }
+
+// Correctly rounded float32 quotient s/h without a division instruction.
+// q0 = s * RN64(1/h) is within 2^-52 (relative) of s/h; one float64 Newton
+// step with the exact FMA residual r = s - q0*h brings q1 within ~2^-104.
+//   * normal results: with 24-bit significands s/h is never a float32
+//     rounding midpoint and stays >= ~2^-48 (relative) away from every
+//     midpoint, so rounding q1 (or even q0) to float32 gives RN32(s/h);
+//   * subnormal results: s/h can sit exactly on a midpoint; that midpoint has
+//     25 significant bits, so it is a float64 number and q1 equals it
+//     exactly, and the final rounding applies ties-to-even like the IEEE
+//     division.
+// Non-finite inputs keep the plain product (the residual would be NaN).
+__device__ __forceinline__ float div_h(float s, float h, double rh) {
+    const double sd = (double)s;
+    const double q0 = sd * rh;
+    const double r = fma(-q0, (double)h, sd);
+    const double q1 = fma(r, rh, q0);
+    return __builtin_isfinite(q1) ? (float)q1 : (float)q0;
+}
+
 // Half-stencil K = order/2.  Staggered first derivative, ertm/numerics.py:68-117:
 //   plus : sum_k c_k (f[i+k]   - f[i-k+1]) / h   (value at i+1/2)
 //   minus: sum_k c_k (f[i+k-1] - f[i-k])   / h   (value at i-1/2)
-// accumulated from 0 in k order, then divided; zero outside [K, n-K).
-template <int K>
-__device__ __forceinline__ float dplus(const float* __restrict__ f, int64_t idx, int64_t s,
-                                       const float* c, float h) {
-    float acc = 0.0f;
+// accumulated in k order, then divided; zero outside [K, n-K).  The
+// reference starts the sum from 0; 0 + t == t (up to the sign of a zero
+// result), so the sum starts from the first term.
+template <int K, typename F>
+__device__ __forceinline__ float dplus(const F& f, int64_t idx, int64_t s, const float* c, float h,
+                                       double rh) {
+    float acc = c[0] * (f[idx + s] - f[idx]);
 #pragma unroll
-    for (int k = 1; k <= K; ++k) {
+    for (int k = 2; k <= K; ++k) {
         float d = f[idx + k * s] - f[idx - (k - 1) * s];
         float t = c[k - 1] * d;
         acc = acc + t;
     }
-    return acc / h;
+    return div_h(acc, h, rh);
 }
 
-template <int K>
-__device__ __forceinline__ float dminus(const float* __restrict__ f, int64_t idx, int64_t s,
-                                        const float* c, float h) {
-    float acc = 0.0f;
+template <int K, typename F>
+__device__ __forceinline__ float dminus(const F& f, int64_t idx, int64_t s, const float* c,
+                                        float h, double rh) {
+    float acc = c[0] * (f[idx] - f[idx - s]);
 #pragma unroll
-    for (int k = 1; k <= K; ++k) {
+    for (int k = 2; k <= K; ++k) {
         float d = f[idx + (k - 1) * s] - f[idx - k * s];
         float t = c[k - 1] * d;
         acc = acc + t;
     }
-    return acc / h;
+    return div_h(acc, h, rh);
 }
 
 // CPML memory update and corrected derivative, ertm/numerics.py:221-227:
@@ -53,6 +83,15 @@ __device__ __forceinline__ float cpml(float d, float psi_in, float a, float b, f
     m = m + ad;
     *psi_out = m;
     return d + m;
+}
+
+// Same recursion, returning the new memory instead of storing it (used where
+// halo cells are recomputed and only the owner stores).
+__device__ __forceinline__ float cpml_r(float d, float psi_in, float a, float b, float& psi) {
+    float m = psi_in * b;
+    float ad = a * d;
+    psi = m + ad;
+    return d + psi;
 }
 
 // Offsets of the compact CPML memories (see ErtmProp): x-memory of column j

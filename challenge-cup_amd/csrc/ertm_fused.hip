@@ -1,0 +1,470 @@
+// Fused single-launch time step for the elastic P-SV engine.
+//
+// One launch advances velocities AND stresses by one step (52 B of HBM
+// traffic per cell: 5 state reads + 5 writes + lambda, mu, 1/rho), instead of
+// the two launches of ertm_stencil.hip (72 B).
+//
+// Work decomposition: a block of NT threads owns a band of TX = NT - 4K
+// columns and a chunk of ZC rows.  Thread t handles column x0 - 2K + t and
+// walks the chunk row by row (z-streaming):
+//   * old stresses are loaded one row ahead of use (row k+2K at iteration k)
+//     into per-thread register queues that hold the 2K+1 rows a z-derivative
+//     needs; new velocities go to register queues too;
+//   * x-neighbours are exchanged through one LDS row per field;
+//   * the velocity of row k+K is computed for columns [x0-K, x0+TX+K) (the
+//     halo columns and the K rows above/below the chunk are recomputed, not
+//     stored), then the stresses of row k for the owned columns.
+// The per-cell arithmetic is the same inline code as the two-launch kernels
+// (ertm_device.h), so results are bit-identical to them and to the
+// reference.  Ping-pong state buffers make the recomputed halos race-free.
+//
+// Block order is XCD-aware: the hardware deals consecutive workgroups to the
+// 8 XCDs round-robin; the remap gives each XCD a contiguous run of logical
+// blocks, i.e. vertically adjacent chunks of one band, whose halo rows then
+// hit in that XCD's L2.
+#include "ertm_kernels.h"
+
+namespace ertm {
+
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = bid & 7, idx = bid >> 3;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// Buffer resources: a 128-bit descriptor per array with the byte size as
+// range.  Buffer loads outside [0, size) return 0 and such stores are
+// dropped, which covers rows above/below the grid without guards; columns
+// beyond the grid only ever feed lanes whose results are discarded.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+// The b32 builtins move raw 32-bit patterns: bit-cast, never convert.
+__device__ __forceinline__ float bload(rsrc_t r, int off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void bstore(rsrc_t r, int off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0, 0);
+}
+
+// Per-block constants of the fused elastic step.
+struct ElBlock {
+    rsrc_t rS, rT, rIR, rLAM, rMU;
+    gcfloat* AZ;
+    gcfloat* BZ;
+    int nzp, nxp, z0, z1, c, c4, tid, ld4;
+    int fVX, fVZ, fXX, fZZ, fXZ;
+    int px0, px1, px2, px3, pz0, pz1, pz2, pz3;
+    int wx, wzt, wzb, xrow4, jx4;
+    int src_iz, mech, fs_row;
+    bool colv, velc, own, inx, src_col;
+    float dt, hx, hz, amp, axc, bxc;
+    double rhx, rhz;
+    float cf[4];
+};
+
+// Byte offset that every buffer range check rejects: stores to it are dropped.
+#define ERTM_NOWRITE (-4)
+
+// The row loop.  STRIP = false is the interior version, used by blocks that
+// touch no CPML strip: its body is straight-line (every lane issues the same
+// loads and stores; lanes that must not write store to ERTM_NOWRITE), so the
+// number of memory operations per row is fixed and the compiler's wait for
+// the prefetched loads leaves the previous row's stores in flight (vmcnt
+// also counts stores on this architecture).  STRIP = true adds the CPML
+// memory reads/writes, inside per-lane branches.
+//
+// One descriptor covers a whole state buffer (fields + CPML strips); the field
+// slab offset is folded into the per-lane byte offset, so the hardware range
+// check applies to the full offset: reads outside the buffer return 0, reads
+// between slabs return finite values that only feed lanes whose results are
+// discarded.  The launcher only selects this kernel when a state buffer is
+// smaller than 2 GiB.
+// Running per-lane byte offsets of one row iteration k (each advances by one
+// row per iteration): stress loads of row k+2K, velocity row m = k+K (old
+// loads of the next row are prefetched from here + one row), 1/rho row
+// m+1, mu row k+1, lambda / stress stores row k.
+struct ElOffs {
+    int sxx, szz, sxz, vx, vz, ir, mu, lam, txx, tzz, txz;
+    __device__ __forceinline__ void advance(int d) {
+        sxx += d; szz += d; sxz += d; vx += d; vz += d; ir += d; mu += d; lam += d;
+        txx += d; tzz += d; txz += d;
+    }
+};
+
+template <int K, int NT>
+struct ElState {
+    float q_sxx[2 * K + 1], q_szz[2 * K + 1], q_sxz[2 * K + 1];  // rows k .. k+2K
+    float q_vx[2 * K];                                          // rows k-K+1 .. k+K
+    float q_vz[2 * K + 1];                                      // rows k-K .. k+K
+    float ir_m, mu_k;                                           // own column, rows m, k
+    float n_sxx, n_szz, n_sxz, n_vx, n_vz, n_ir, n_mu, n_lam;   // prefetched inputs
+};
+
+// One row iteration.  STRESS = false: velocity only (the 2K prologue rows
+// of a chunk, whose stress rows lie above the chunk).  Every lane issues the
+// same memory operations (lanes that must not write store to ERTM_NOWRITE),
+// so the wait for the prefetched loads leaves earlier stores in flight
+// (vmcnt also counts stores on this architecture).  STRIP adds the CPML
+// memory reads/writes in per-lane branches.
+template <int K, int NT, bool STRIP, bool STRESS>
+__device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, NT>& X, ElOffs& O, int k,
+                                        float* l_sxx, float* l_sxz, float* l_ir, float* l_vx,
+                                        float* l_vz, float* l_mu0, float* l_mu1) {
+    constexpr int QS = 2 * K + 1, QX = 2 * K, QZ = 2 * K + 1;
+    const int tid = B.tid, ld4 = B.ld4, nzp = B.nzp;
+    const int m = k + K;
+    const int om = O.vx, okk = O.lam;  // current-row offsets before advancing
+    const int om_z = O.vz, otxx = O.txx, otzz = O.tzz, otxz = O.txz;
+    const float vx_old = X.n_vx, vz_old = X.n_vz, ir_m1 = X.n_ir, mu_k1 = X.n_mu;
+    const float lam_k = X.n_lam;
+#pragma unroll
+    for (int q = 0; q < QS - 1; ++q) {
+        X.q_sxx[q] = X.q_sxx[q + 1];
+        X.q_szz[q] = X.q_szz[q + 1];
+        X.q_sxz[q] = X.q_sxz[q + 1];
+    }
+    X.q_sxx[QS - 1] = X.n_sxx;
+    X.q_szz[QS - 1] = X.n_szz;
+    X.q_sxz[QS - 1] = X.n_sxz;
+    O.advance(ld4);
+    // prefetch the next iteration's inputs; in flight during this row's compute
+    X.n_sxx = bload(B.rS, O.sxx);
+    X.n_szz = bload(B.rS, O.szz);
+    X.n_sxz = bload(B.rS, O.sxz);
+    X.n_vx = bload(B.rS, O.vx);
+    X.n_vz = bload(B.rS, O.vz);
+    X.n_ir = bload(B.rIR, O.ir);
+    X.n_mu = bload(B.rMU, O.mu);
+    X.n_lam = bload(B.rLAM, O.lam);
+
+    // ---- phase A: x-neighbour rows for the velocity update of row m
+    l_sxx[tid] = X.q_sxx[K];
+    l_sxz[tid] = X.q_sxz[K];
+    l_ir[tid] = X.ir_m;
+    __syncthreads();
+
+    // velocity of row m on every lane (lanes outside the band and rows
+    // outside the grid compute values nobody uses)
+    const bool inz_m = m >= K && m < nzp - K;
+    const bool own_m = B.own && m >= B.z0 && m < B.z1;
+    float a = dplus<K>(l_sxx, tid, 1, B.cf, B.hx, B.rhx);
+    float b = dminus<K>(X.q_sxz, K, 1, B.cf, B.hz, B.rhz);
+    float cc = dminus<K>(l_sxz, tid, 1, B.cf, B.hx, B.rhx);
+    float d = dplus<K>(X.q_szz, K, 1, B.cf, B.hz, B.rhz);
+    a = B.inx ? a : 0.0f;
+    cc = B.inx ? cc : 0.0f;
+    b = inz_m ? b : 0.0f;
+    d = inz_m ? d : 0.0f;
+    if (STRIP) {
+        if (B.velc && m >= 0 && m < nzp) {
+            int pzo = -1;  // byte offset of (m, c) inside a z-memory
+            if (m < B.wzt) pzo = m * ld4 + B.c4;
+            else if (B.wzb > 0 && m >= nzp - B.wzb) pzo = (m - nzp + B.wzt + B.wzb) * ld4 + B.c4;
+            const int pxo = B.jx4 >= 0 ? m * B.xrow4 + B.jx4 : -1;
+            float pn;
+            if (pxo >= 0) {
+                a = cpml_r(a, bload(B.rS, B.px0 + pxo), B.axc, B.bxc, pn);
+                bstore(B.rT, own_m ? B.px0 + pxo : ERTM_NOWRITE, pn);
+                cc = cpml_r(cc, bload(B.rS, B.px1 + pxo), B.axc, B.bxc, pn);
+                bstore(B.rT, own_m ? B.px1 + pxo : ERTM_NOWRITE, pn);
+            }
+            if (pzo >= 0) {
+                const float az = B.AZ[m], bz = B.BZ[m];
+                b = cpml_r(b, bload(B.rS, B.pz0 + pzo), az, bz, pn);
+                bstore(B.rT, own_m ? B.pz0 + pzo : ERTM_NOWRITE, pn);
+                d = cpml_r(d, bload(B.rS, B.pz1 + pzo), az, bz, pn);
+                bstore(B.rT, own_m ? B.pz1 + pzo : ERTM_NOWRITE, pn);
+            }
+        }
+    }
+    float irx, irz;
+    {
+        float s = X.ir_m + l_ir[tid + 1];
+        irx = (B.c < B.nxp - 1) ? 0.5f * s : X.ir_m;
+        float t = X.ir_m + ir_m1;
+        irz = (m < nzp - 1) ? 0.5f * t : X.ir_m;
+    }
+    const float bxv = B.dt * irx;
+    const float bzv = B.dt * irz;
+    const float sx = a + b;
+    const float sz = cc + d;
+    const float ux = bxv * sx;
+    const float uz = bzv * sz;
+    const float vx_new = vx_old + ux;
+    const float vz_new = vz_old + uz;
+    // force_z source: added to the stored vz after the stresses of this
+    // step have used the pre-source value (ertm/propagation.py:870-871)
+    float vzs = vz_new;
+    if (B.src_col && m == B.src_iz && B.mech == ERTM_SRC_FORCE_Z) vzs = vzs + B.amp;
+    bstore(B.rT, own_m ? om : ERTM_NOWRITE, vx_new);
+    bstore(B.rT, own_m ? om_z : ERTM_NOWRITE, vzs);
+#pragma unroll
+    for (int q = 0; q < QX - 1; ++q) X.q_vx[q] = X.q_vx[q + 1];
+    X.q_vx[QX - 1] = vx_new;
+#pragma unroll
+    for (int q = 0; q < QZ - 1; ++q) X.q_vz[q] = X.q_vz[q + 1];
+    X.q_vz[QZ - 1] = vz_new;
+
+    // ---- phase B: x-neighbour rows for the stress update of row k
+    if (STRESS) {
+        l_vx[tid] = X.q_vx[K - 1];
+        l_vz[tid] = X.q_vz[K];
+        l_mu0[tid] = X.mu_k;
+        l_mu1[tid] = mu_k1;
+    }
+    __syncthreads();
+    if (STRESS) {
+        const bool inz_k = k >= K && k < nzp - K;
+        const bool own_k = B.own;
+        float ex = dminus<K>(l_vx, tid, 1, B.cf, B.hx, B.rhx);
+        float ez = dminus<K>(X.q_vz, K, 1, B.cf, B.hz, B.rhz);
+        float e = dplus<K>(X.q_vx, K - 1, 1, B.cf, B.hz, B.rhz);
+        float f = dplus<K>(l_vz, tid, 1, B.cf, B.hx, B.rhx);
+        ex = B.inx ? ex : 0.0f;
+        f = B.inx ? f : 0.0f;
+        ez = inz_k ? ez : 0.0f;
+        e = inz_k ? e : 0.0f;
+        if (STRIP) {
+            if (own_k) {
+                int pzo = -1;
+                if (k < B.wzt) pzo = k * ld4 + B.c4;
+                else if (B.wzb > 0 && k >= nzp - B.wzb) pzo = (k - nzp + B.wzt + B.wzb) * ld4 + B.c4;
+                const int pxo = B.jx4 >= 0 ? k * B.xrow4 + B.jx4 : -1;
+                float pn;
+                if (pxo >= 0) {
+                    ex = cpml_r(ex, bload(B.rS, B.px2 + pxo), B.axc, B.bxc, pn);
+                    bstore(B.rT, B.px2 + pxo, pn);
+                    f = cpml_r(f, bload(B.rS, B.px3 + pxo), B.axc, B.bxc, pn);
+                    bstore(B.rT, B.px3 + pxo, pn);
+                }
+                if (pzo >= 0) {
+                    const float az = B.AZ[k], bz = B.BZ[k];
+                    ez = cpml_r(ez, bload(B.rS, B.pz2 + pzo), az, bz, pn);
+                    bstore(B.rT, B.pz2 + pzo, pn);
+                    e = cpml_r(e, bload(B.rS, B.pz3 + pzo), az, bz, pn);
+                    bstore(B.rT, B.pz3 + pzo, pn);
+                }
+            }
+        }
+        const float two_mu = 2.0f * X.mu_k;
+        const float l2m = lam_k + two_mu;
+        float mx0, mx1;  // mu on the cell corner: mean_z(mean_x(mu))
+        {
+            float s0 = X.mu_k + l_mu0[tid + 1];
+            float s1 = mu_k1 + l_mu1[tid + 1];
+            const bool xin = B.c < B.nxp - 1;
+            mx0 = xin ? 0.5f * s0 : X.mu_k;
+            mx1 = xin ? 0.5f * s1 : mu_k1;
+        }
+        float muxz;
+        {
+            float s = mx0 + mx1;
+            muxz = (k < nzp - 1) ? 0.5f * s : mx0;
+        }
+        float t1 = l2m * ex, t2 = lam_k * ez;
+        const float sxx_rate = t1 + t2;
+        float t3 = lam_k * ex, t4 = l2m * ez;
+        const float szz_rate = t3 + t4;
+        const float sxx_inc = B.dt * sxx_rate;
+        const float szz_inc = B.dt * szz_rate;
+        float sxx = X.q_sxx[0] + sxx_inc;
+        float szz = X.q_szz[0] + szz_inc;
+        const float dtmu = B.dt * muxz;
+        const float ef = e + f;
+        const float sxz_inc = dtmu * ef;
+        float sxz = X.q_sxz[0] + sxz_inc;
+        if (B.src_col && k == B.src_iz) {
+            if (B.mech == ERTM_SRC_SHEAR) {
+                sxz = sxz + B.amp;
+            } else if (B.mech == ERTM_SRC_EXPLOSIVE) {
+                sxx = sxx + B.amp;
+                szz = szz + B.amp;
+            }
+        }
+        if (k == B.fs_row) {
+            szz = 0.0f;
+            sxz = 0.0f;
+        }
+        bstore(B.rT, own_k ? otxx : ERTM_NOWRITE, sxx);
+        bstore(B.rT, own_k ? otzz : ERTM_NOWRITE, szz);
+        bstore(B.rT, own_k ? otxz : ERTM_NOWRITE, sxz);
+    }
+    (void)okk;
+    X.ir_m = ir_m1;
+    X.mu_k = mu_k1;
+}
+
+// One chunk: 2K prologue rows (velocity only), then zc rows (both updates).
+// One descriptor covers a whole state buffer (fields + CPML strips); the field
+// slab offset is part of the per-lane byte offset, so the hardware range
+// check applies to the full offset: reads outside the buffer return 0, reads
+// between slabs return finite values that only feed lanes whose results are
+// discarded.  The launcher only selects this kernel when a state buffer is
+// smaller than 2 GiB.
+template <int K, int NT, bool STRIP>
+__device__ __forceinline__ void el_loop(const ElBlock& B, float* l_sxx, float* l_sxz, float* l_ir,
+                                        float* l_vx, float* l_vz, float* l_mu0, float* l_mu1) {
+    constexpr int QX = 2 * K, QZ = 2 * K + 1;
+    const int ld4 = B.ld4;
+    ElState<K, NT> X;
+#pragma unroll
+    for (int q = 0; q < QX; ++q) X.q_vx[q] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < QZ; ++q) X.q_vz[q] = 0.0f;
+    const int kbeg = B.z0 - 2 * K;
+    const int o0 = kbeg * ld4 + B.c4;  // byte offset of (kbeg, c) in a slab
+#pragma unroll
+    for (int q = 0; q < 2 * K; ++q) {  // stress rows kbeg .. kbeg+2K-1 -> slots 1..2K
+        const int o = o0 + q * ld4;
+        X.q_sxx[q + 1] = bload(B.rS, o + B.fXX);
+        X.q_szz[q + 1] = bload(B.rS, o + B.fZZ);
+        X.q_sxz[q + 1] = bload(B.rS, o + B.fXZ);
+    }
+    X.ir_m = bload(B.rIR, o0 + K * ld4);
+    X.mu_k = bload(B.rMU, o0);
+    ElOffs O;  // offsets of iteration kbeg
+    O.sxx = o0 + 2 * K * ld4 + B.fXX;
+    O.szz = o0 + 2 * K * ld4 + B.fZZ;
+    O.sxz = o0 + 2 * K * ld4 + B.fXZ;
+    O.vx = o0 + K * ld4 + B.fVX;
+    O.vz = o0 + K * ld4 + B.fVZ;
+    O.ir = o0 + (K + 1) * ld4;
+    O.mu = o0 + ld4;
+    O.lam = o0;
+    O.txx = o0 + B.fXX;
+    O.tzz = o0 + B.fZZ;
+    O.txz = o0 + B.fXZ;
+    X.n_sxx = bload(B.rS, O.sxx);
+    X.n_szz = bload(B.rS, O.szz);
+    X.n_sxz = bload(B.rS, O.sxz);
+    X.n_vx = bload(B.rS, O.vx);
+    X.n_vz = bload(B.rS, O.vz);
+    X.n_ir = bload(B.rIR, O.ir);
+    X.n_mu = bload(B.rMU, O.mu);
+    X.n_lam = bload(B.rLAM, O.lam);
+    int k = kbeg;
+    for (; k < B.z0; ++k)
+        el_iter<K, NT, STRIP, false>(B, X, O, k, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+    for (; k < B.z1; ++k)
+        el_iter<K, NT, STRIP, true>(B, X, O, k, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+}
+
+template <int K, int NT>
+__global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict__ props,
+                                                      LaunchArgs la, int zc) {
+    constexpr int TX = NT - 4 * K;
+    const int slot = la.base + blockIdx.y;
+    const ErtmProp& P = props[slot];
+    const int nzp = P.nzp, nxp = P.nxp;
+    const int nbands = (nxp + TX - 1) / TX;
+    const int nchunks = (nzp + zc - 1) / zc;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    if (bid >= nbands * nchunks) return;
+    const int band = bid / nchunks, chunk = bid - band * nchunks;
+    ElBlock B;
+    const int x0 = band * TX;
+    B.nzp = nzp;
+    B.nxp = nxp;
+    B.z0 = chunk * zc;
+    B.z1 = min(B.z0 + zc, nzp);
+    B.tid = threadIdx.x;
+    B.c = x0 - 2 * K + B.tid;
+    B.c4 = 4 * B.c;
+    B.colv = B.c >= 0 && B.c < nxp;
+    B.velc = B.colv && B.tid >= K && B.tid < NT - K;
+    B.own = B.colv && B.tid >= 2 * K && B.tid < NT - 2 * K;
+    B.ld4 = 4 * P.ld;
+    const int fs4 = (int)(4 * P.fstride);  // bytes between field slabs
+    const int64_t sbytes = 4 * P.state_floats;
+    B.rS = make_rsrc(P.state[la.cur], sbytes);
+    B.rT = make_rsrc(P.state[la.cur ^ 1], sbytes);
+    const int64_t mbytes = (int64_t)nzp * P.ld * 4;
+    B.rIR = make_rsrc(P.irho, mbytes);
+    B.rLAM = make_rsrc(P.lam, mbytes);
+    B.rMU = make_rsrc(P.mu, mbytes);
+    B.fVX = ERTM_EL_VX * fs4;
+    B.fVZ = ERTM_EL_VZ * fs4;
+    B.fXX = ERTM_EL_SXX * fs4;
+    B.fZZ = ERTM_EL_SZZ * fs4;
+    B.fXZ = ERTM_EL_SXZ * fs4;
+    B.AZ = gp(P.az);
+    B.BZ = gp(P.bz);
+    B.px0 = (int)(4 * P.psix[0]);
+    B.px1 = (int)(4 * P.psix[1]);
+    B.px2 = (int)(4 * P.psix[2]);
+    B.px3 = (int)(4 * P.psix[3]);
+    B.pz0 = (int)(4 * P.psiz[0]);
+    B.pz1 = (int)(4 * P.psiz[1]);
+    B.pz2 = (int)(4 * P.psiz[2]);
+    B.pz3 = (int)(4 * P.psiz[3]);
+    B.wx = P.wx;
+    B.wzt = P.wzt;
+    B.wzb = P.wzb;
+    B.xrow4 = 8 * P.wx;  // bytes per x-memory row
+    B.src_iz = P.src_iz;
+    B.mech = P.mechanism;
+    B.fs_row = P.free_surface ? P.pad : -1;
+    B.dt = P.dt;
+    B.hx = P.dx;
+    B.hz = P.dz;
+    B.rhx = P.rdx;
+    B.rhz = P.rdz;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) B.cf[q] = P.coef[q];
+    B.amp = la.amp[blockIdx.y];
+    B.src_col = B.amp != 0.0f && B.c == P.src_ix;
+    B.inx = B.c >= K && B.c < nxp - K;
+    B.jx4 = -1;
+    if (B.wx > 0 && B.colv) {
+        if (B.c < B.wx) B.jx4 = B.c4;
+        else if (B.c >= nxp - B.wx) B.jx4 = 4 * (B.c - nxp + 2 * B.wx);
+    }
+    B.axc = B.colv ? gp(P.ax)[B.c] : 0.0f;
+    B.bxc = B.colv ? gp(P.bx)[B.c] : 1.0f;
+
+    __shared__ float l_sxx[NT], l_sxz[NT], l_ir[NT], l_vx[NT], l_vz[NT], l_mu0[NT], l_mu1[NT];
+    // Does the block (columns x0-2K .. x0+TX+2K, rows z0-2K .. z1+2K) reach a
+    // CPML strip?  Uniform per block.
+    const int cx0 = x0 - 2 * K, cx1 = x0 + TX + 2 * K;
+    const int rz0 = B.z0 - 2 * K, rz1 = B.z1 + 2 * K;
+    const bool strip = (B.wx > 0 && (cx0 < B.wx || cx1 > nxp - B.wx)) ||
+                       (B.wzt > 0 && rz0 < B.wzt) || (B.wzb > 0 && rz1 > nzp - B.wzb);
+    if (strip)
+        el_loop<K, NT, true>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+    else
+        el_loop<K, NT, false>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+}
+
+template <int K, int NT>
+static int blocks_for(const ErtmProp& P, int zc) {
+    constexpr int TX = NT - 4 * K;
+    return ((P.nxp + TX - 1) / TX) * ((P.nzp + zc - 1) / zc);
+}
+
+template <int K, int NT>
+static void launch_fused_k(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
+                           int zc, hipStream_t s) {
+    int nb = 0;
+    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la.base + k], zc));
+    hipLaunchKernelGGL((k_elastic_fused<K, NT>), dim3(nb, la.n), dim3(NT), 0, s, dprops, la, zc);
+}
+
+// variant: 1 -> 256 threads, 2 -> 128 threads, 3 -> 64 threads (z chunk zc rows,
+// zc % 9 == 1 so that zc + 8 rows form whole unrolled trips)
+bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
+                       int order, int variant, int zc, hipStream_t s) {
+    if (order != 8 || zc % 9 != 1) return false;
+    for (int k = 0; k < la.n; ++k)
+        if (4 * hprops[la.base + k].state_floats >= ((int64_t)1 << 31)) return false;
+    switch (variant) {
+        case 1: launch_fused_k<4, 256>(dprops, hprops, la, zc, s); return true;
+        case 2: launch_fused_k<4, 128>(dprops, hprops, la, zc, s); return true;
+        case 3: launch_fused_k<4, 64>(dprops, hprops, la, zc, s); return true;
+        default: return false;
+    }
+}
+
+}  // namespace ertm

@@ -107,12 +107,12 @@ __device__ __forceinline__ float wave_value(const ErtmProp& P, const float* S, i
     }
     // divcurl: plain derivatives, no CPML memory (ertm/propagation.py:949-963)
     if (wave == ERTM_WAVE_P) {
-        float a = dminus<K>(vx, o, 1, P.coef, P.dx);
-        float b = dminus<K>(vz, o, P.ld, P.coef, P.dz);
+        float a = dminus<K>(vx, o, 1, P.coef, P.dx, P.rdx);
+        float b = dminus<K>(vz, o, P.ld, P.coef, P.dz, P.rdz);
         return a + b;
     }
-    float a = dminus<K>(vx, o, P.ld, P.coef, P.dz);
-    float b = dminus<K>(vz, o, 1, P.coef, P.dx);
+    float a = dminus<K>(vx, o, P.ld, P.coef, P.dz, P.rdz);
+    float b = dminus<K>(vz, o, 1, P.coef, P.dx, P.rdx);
     return a - b;
 }
 

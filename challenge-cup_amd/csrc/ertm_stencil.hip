@@ -59,9 +59,9 @@ __global__ void __launch_bounds__(256) k_velocity(const ErtmProp* __restrict__ p
     if (P.engine != ERTM_ELASTIC) {
         // acoustic / viscoacoustic: vx += (dt*bx) * D'x+ p ; vz += (dt*bz) * D'z+ p
         const float* p = S + (int64_t)ERTM_AC_P * P.fstride;
-        float dx = inx ? dplus<K>(p, o, 1, P.coef, P.dx) : 0.0f;
+        float dx = inx ? dplus<K>(p, o, 1, P.coef, P.dx, P.rdx) : 0.0f;
         if (px >= 0) dx = cpml(dx, S[P.psix[0] + px], P.ax[j], P.bx[j], &T[P.psix[0] + px]);
-        float dz = inz ? dplus<K>(p, o, ld, P.coef, P.dz) : 0.0f;
+        float dz = inz ? dplus<K>(p, o, ld, P.coef, P.dz, P.rdz) : 0.0f;
         if (pz >= 0) dz = cpml(dz, S[P.psiz[0] + pz], P.az[i], P.bz[i], &T[P.psiz[0] + pz]);
         float bxv = dt * avg_x(P.irho, P, i, j);
         float bzv = dt * avg_z(P.irho, P, i, j);
@@ -75,14 +75,14 @@ __global__ void __launch_bounds__(256) k_velocity(const ErtmProp* __restrict__ p
         const float* szz = S + (int64_t)ERTM_EL_SZZ * P.fstride;
         const float* sxz = S + (int64_t)ERTM_EL_SXZ * P.fstride;
         // vx += (dt*bx) * (D'x+ sxx + D'z- sxz)
-        float a = inx ? dplus<K>(sxx, o, 1, P.coef, P.dx) : 0.0f;
+        float a = inx ? dplus<K>(sxx, o, 1, P.coef, P.dx, P.rdx) : 0.0f;
         if (px >= 0) a = cpml(a, S[P.psix[0] + px], P.ax[j], P.bx[j], &T[P.psix[0] + px]);
-        float b = inz ? dminus<K>(sxz, o, ld, P.coef, P.dz) : 0.0f;
+        float b = inz ? dminus<K>(sxz, o, ld, P.coef, P.dz, P.rdz) : 0.0f;
         if (pz >= 0) b = cpml(b, S[P.psiz[0] + pz], P.az[i], P.bz[i], &T[P.psiz[0] + pz]);
         // vz += (dt*bz) * (D'x- sxz + D'z+ szz)
-        float cc = inx ? dminus<K>(sxz, o, 1, P.coef, P.dx) : 0.0f;
+        float cc = inx ? dminus<K>(sxz, o, 1, P.coef, P.dx, P.rdx) : 0.0f;
         if (px >= 0) cc = cpml(cc, S[P.psix[1] + px], P.ax[j], P.bx[j], &T[P.psix[1] + px]);
-        float d = inz ? dplus<K>(szz, o, ld, P.coef, P.dz) : 0.0f;
+        float d = inz ? dplus<K>(szz, o, ld, P.coef, P.dz, P.rdz) : 0.0f;
         if (pz >= 0) d = cpml(d, S[P.psiz[1] + pz], P.az[i], P.bz[i], &T[P.psiz[1] + pz]);
         float bxv = dt * avg_x(P.irho, P, i, j);
         float bzv = dt * avg_z(P.irho, P, i, j);
@@ -121,13 +121,13 @@ __global__ void __launch_bounds__(256) k_stress(const ErtmProp* __restrict__ pro
     if (P.engine == ERTM_ELASTIC) {
         const float* vx = T + (int64_t)ERTM_EL_VX * P.fstride;  // new velocities
         const float* vz = T + (int64_t)ERTM_EL_VZ * P.fstride;
-        float ex = inx ? dminus<K>(vx, o, 1, P.coef, P.dx) : 0.0f;
+        float ex = inx ? dminus<K>(vx, o, 1, P.coef, P.dx, P.rdx) : 0.0f;
         if (px >= 0) ex = cpml(ex, S[P.psix[2] + px], P.ax[j], P.bx[j], &T[P.psix[2] + px]);
-        float ez = inz ? dminus<K>(vz, o, ld, P.coef, P.dz) : 0.0f;
+        float ez = inz ? dminus<K>(vz, o, ld, P.coef, P.dz, P.rdz) : 0.0f;
         if (pz >= 0) ez = cpml(ez, S[P.psiz[2] + pz], P.az[i], P.bz[i], &T[P.psiz[2] + pz]);
-        float e = inz ? dplus<K>(vx, o, ld, P.coef, P.dz) : 0.0f;
+        float e = inz ? dplus<K>(vx, o, ld, P.coef, P.dz, P.rdz) : 0.0f;
         if (pz >= 0) e = cpml(e, S[P.psiz[3] + pz], P.az[i], P.bz[i], &T[P.psiz[3] + pz]);
-        float f = inx ? dplus<K>(vz, o, 1, P.coef, P.dx) : 0.0f;
+        float f = inx ? dplus<K>(vz, o, 1, P.coef, P.dx, P.rdx) : 0.0f;
         if (px >= 0) f = cpml(f, S[P.psix[3] + px], P.ax[j], P.bx[j], &T[P.psix[3] + px]);
         const float lam = P.lam[o];
         const float mu = P.mu[o];
@@ -169,9 +169,9 @@ __global__ void __launch_bounds__(256) k_stress(const ErtmProp* __restrict__ pro
 
     const float* vx = T + (int64_t)ERTM_AC_VX * P.fstride;
     const float* vz = T + (int64_t)ERTM_AC_VZ * P.fstride;
-    float dxv = inx ? dminus<K>(vx, o, 1, P.coef, P.dx) : 0.0f;
+    float dxv = inx ? dminus<K>(vx, o, 1, P.coef, P.dx, P.rdx) : 0.0f;
     if (px >= 0) dxv = cpml(dxv, S[P.psix[1] + px], P.ax[j], P.bx[j], &T[P.psix[1] + px]);
-    float dzv = inz ? dminus<K>(vz, o, ld, P.coef, P.dz) : 0.0f;
+    float dzv = inz ? dminus<K>(vz, o, ld, P.coef, P.dz, P.rdz) : 0.0f;
     if (pz >= 0) dzv = cpml(dzv, S[P.psiz[1] + pz], P.az[i], P.bz[i], &T[P.psiz[1] + pz]);
     const float div = dxv + dzv;
     const float kap = P.kappa[o];

@@ -41,7 +41,7 @@ class ErtmProp(C.Structure):
         ("free_surface", C.c_int32), ("nmech", C.c_int32),
         ("wx", C.c_int32), ("wzt", C.c_int32), ("wzb", C.c_int32), ("nfields", C.c_int32),
         ("dx", C.c_float), ("dz", C.c_float), ("dt", C.c_float), ("coef", C.c_float * 4),
-        ("dt64", C.c_double),
+        ("dt64", C.c_double), ("rdx", C.c_double), ("rdz", C.c_double),
         ("fstride", C.c_int64), ("psix", C.c_int64 * 4), ("psiz", C.c_int64 * 4),
         ("state_floats", C.c_int64),
         ("kappa", C.c_void_p), ("irho", C.c_void_p), ("lam", C.c_void_p), ("mu", C.c_void_p),
@@ -101,7 +101,7 @@ EXPORTS = [
                                     C.c_int32, C.POINTER(C.c_void_p)]),
     ("ertm_batch_update_image", C.c_int, [C.c_void_p, C.POINTER(ErtmImage)]),
     ("ertm_batch_destroy", C.c_int, [C.c_void_p]),
-    ("ertm_batch_set_variant", C.c_int, [C.c_void_p, C.c_int32]),
+    ("ertm_batch_set_variant", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     ("ertm_step", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.c_void_p]),
     ("ertm_stabilize", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                  C.c_void_p, C.c_void_p]),

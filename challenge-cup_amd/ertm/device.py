@@ -149,6 +149,8 @@ class Core:
         for k, c in enumerate(num.coef_f32(self.order)):
             d.coef[k] = c
         d.dt64 = self.dt
+        d.rdx = 1.0 / float(np.float32(self.dx))
+        d.rdz = 1.0 / float(np.float32(self.dz))
         d.fstride = self.fstride
         for k in range(4):
             d.psix[k] = self.psix[k]
@@ -419,7 +421,8 @@ class ImageBuffers:
 class Batch:
     """A set of propagators advanced together (one ertm_batch_t)."""
 
-    def __init__(self, cores: list, recvs: list | None = None, imgs: list | None = None):
+    def __init__(self, cores: list, recvs: list | None = None, imgs: list | None = None,
+                 variant: tuple[int, int] | None = None):
         lib = L.lib()
         n = len(cores)
         self.cores = list(cores)
@@ -433,6 +436,19 @@ class Batch:
                 "ertm_batch_create")
         self.h = h
         self.n = n
+        from . import tuning
+
+        c0 = cores[0]
+        if variant is None:
+            same = all((c.engine, c.order) == (c0.engine, c0.order) for c in cores)
+            variant = (tuning.lookup(c0.engine, c0.order, max(c.nzp for c in cores),
+                                     max(c.nxp for c in cores), n) if same else (0, 64))
+        self.set_variant(*variant)
+
+    def set_variant(self, variant: int, zc: int = 64) -> None:
+        L.check(L.lib().ertm_batch_set_variant(self.h, int(variant), int(zc)),
+                "ertm_batch_set_variant")
+        self.variant = (int(variant), int(zc))
 
     def update_images(self, imgs: list):
         self.imgs = imgs

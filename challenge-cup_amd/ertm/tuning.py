@@ -1,0 +1,106 @@
+"""Stencil-kernel variant selection and autotuning.
+
+Variants (see ertm_batch_set_variant): 0 = two launches per step (any
+engine/order); 1/2/3 = fused single launch, 256/128/64 threads per block, each
+block streaming a chunk of ``zc`` rows.  Every variant gives bit-identical
+results, so the choice is purely a performance decision: the heuristic picks
+the widest block that still yields enough blocks to keep all 256 CUs busy,
+and ``autotune`` times the candidates on the real shapes.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+import threading
+
+FUSED_THREADS = {1: 256, 2: 128, 3: 64}
+_cache: dict = {}
+_lock = threading.Lock()
+CACHE_FILE = os.environ.get("ERTM_TUNE_CACHE", "")
+
+
+def fused_supported(engine: str, order: int) -> bool:
+    return engine == "elastic" and int(order) == 8
+
+
+def blocks(variant: int, zc: int, nzp: int, nxp: int, nbatch: int) -> int:
+    tx = FUSED_THREADS[variant] - 16
+    return math.ceil(nxp / tx) * math.ceil(nzp / zc) * nbatch
+
+
+def heuristic(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
+              n_cu: int = 256) -> tuple[int, int]:
+    if not fused_supported(engine, order):
+        return (0, 64)
+    for variant in (1, 2, 3):
+        for zc in (127, 64, 37, 19, 10):
+            if blocks(variant, zc, nzp, nxp, nbatch) >= 4 * n_cu:
+                return (variant, zc)
+    return (3, 10)
+
+
+def key(engine, order, nzp, nxp, nbatch) -> str:
+    return f"{engine}:{order}:{nzp}x{nxp}:b{nbatch}"
+
+
+def lookup(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1) -> tuple[int, int]:
+    k = key(engine, order, nzp, nxp, nbatch)
+    with _lock:
+        if not _cache and CACHE_FILE and os.path.exists(CACHE_FILE):
+            try:
+                _cache.update({kk: tuple(v) for kk, v in json.load(open(CACHE_FILE)).items()})
+            except (OSError, ValueError):
+                pass
+        if k in _cache:
+            return _cache[k]
+    return heuristic(engine, order, nzp, nxp, nbatch)
+
+
+def record(engine, order, nzp, nxp, nbatch, choice) -> None:
+    with _lock:
+        _cache[key(engine, order, nzp, nxp, nbatch)] = tuple(choice)
+        if CACHE_FILE:
+            try:
+                json.dump({k: list(v) for k, v in _cache.items()}, open(CACHE_FILE, "w"))
+            except OSError:
+                pass
+
+
+def candidates(engine: str, order: int) -> list[tuple[int, int]]:
+    if not fused_supported(engine, order):
+        return [(0, 64)]
+    return [(0, 64)] + [(v, zc) for v in (1, 2, 3) for zc in (10, 19, 28, 37, 64, 127)]
+
+
+def autotune(batch, steps: int = 10, cands=None) -> tuple[tuple[int, int], dict]:
+    """Time ``steps`` steps of each candidate on ``batch`` (its state is
+    advanced; results do not depend on the variant).  Returns the fastest
+    (variant, zc) and all timings in ms/step; records the choice."""
+    import torch
+
+    c0 = batch.cores[0]
+    cands = cands or candidates(c0.engine, c0.order)
+    times = {}
+    amps = [0.0] * batch.n
+    for cand in cands:
+        batch.set_variant(*cand)
+        for _ in range(2):
+            batch.step(c0.cur, amps)
+            for c in batch.cores:
+                c.cur ^= 1
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            batch.step(c0.cur, amps)
+            for c in batch.cores:
+                c.cur ^= 1
+        e1.record()
+        torch.cuda.synchronize()
+        times[cand] = e0.elapsed_time(e1) / steps
+    best = min(times, key=times.get)
+    batch.set_variant(*best)
+    record(c0.engine, c0.order, c0.nzp, c0.nxp, batch.n, best)
+    return best, times

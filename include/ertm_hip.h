@@ -93,6 +93,7 @@ typedef struct ErtmProp {
     float dx, dz, dt;      /* float32 values of the spacing and time step */
     float coef[4];         /* float32 stencil weights, zero beyond order/2 */
     double dt64;           /* the time step as given (float64), for the GSLS b_l */
+    double rdx, rdz;       /* float64 reciprocals of the float32 dx, dz */
     int64_t fstride;       /* floats between field slabs */
     int64_t psix[4];       /* offsets of the x-memories (see ertm_step) */
     int64_t psiz[4];       /* offsets of the z-memories */
@@ -211,9 +212,11 @@ int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmIma
                       int32_t n, ertm_batch_t* out);
 int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img);
 int ertm_batch_destroy(ertm_batch_t b);
-/* Kernel variant / tile shape of the stencil kernels (0 = library default);
- * see ertm_kernel_variants for the list. */
-int ertm_batch_set_variant(ertm_batch_t b, int32_t variant);
+/* Stencil kernel variant: 0 = two launches per step (velocity, then
+ * pressure/stress; every engine and order); 1, 2, 3 = single fused launch per
+ * step with 256, 128, 64 threads per block streaming chunks of zc rows,
+ * zc % 9 == 1 (elastic, order 8).  Results are bit-identical across variants. */
+int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc);
 
 /*
  * Per-launch time table: propagator i of the batch is at time index t[i] and

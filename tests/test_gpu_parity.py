@@ -227,3 +227,60 @@ def test_acoustic_c1_shape_forward_matches_oracle():
     ref = ortm.forward(op, w, ("p",))["p"]
     assert np.array_equal(got, ref)
     assert np.array_equal(p.p, op.p)
+
+
+@pytest.mark.parametrize("variant,zc", [(1, 64), (1, 10), (2, 19), (3, 10), (3, 64), (2, 4096)])
+@pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force"])
+def test_fused_elastic_variants_bit_identical(tag, variant, zc):
+    """Every fused launch shape reproduces the two-launch kernels (and thus the
+    reference) bit for bit, across chunk seams, CPML strips and all sources."""
+    _say("fused variant", tag, variant, zc)
+    from ertm import device as dev
+    from ertm.propagation import source_wavelet
+    from product_cases import source_of
+
+    c = forward_case(tag)
+    if c["order"] != 8:
+        pytest.skip("fused kernels are order 8")
+    p, spec = propagator_of(c)
+    p._batches["step"] = dev.Batch([p._core], variant=(variant, zc))
+    w = source_wavelet(source_of(c), spec.dt, spec.nt)
+    for s in range(spec.nt):
+        p.step(float(w[s]))
+    for name, ref in c["state"].items():
+        assert np.array_equal(getattr(p, name), ref), name
+    psi = {k: p._core.expand_psi(k, v.cpu().numpy()) for k, v in p._core.psi_views().items()}
+    for name, ref in c["psi"].items():
+        assert np.array_equal(psi.get(name, np.zeros_like(ref)), ref), name
+
+
+def test_fused_variant_large_random_model_matches_two_launch():
+    """600 x 1200 elastic (config-2 shape), random medium, CPML on all sides:
+    fused (every variant) == two-launch after 60 steps."""
+    _say("fused vs two-launch C2 shape")
+    import torch
+    from ertm import device as dev
+    from ertm.propagation import PropagationSpec, build_propagator
+    from ertm.types import ElasticModel, GridSpec, SourceSpec
+
+    rng = np.random.default_rng(4)
+    nz, nx = 600, 1200
+    vp = rng.uniform(2200, 4500, (nz, nx)).astype(np.float32)
+    m = ElasticModel(GridSpec(nx=nx, nz=nz, dx=5.0, dz=5.0), vp=vp,
+                     vs=(vp / rng.uniform(1.6, 2.0, (nz, nx))).astype(np.float32),
+                     rho=rng.uniform(1800, 2600, (nz, nx)).astype(np.float32),
+                     qp=np.full_like(vp, 100.0), qs=np.full_like(vp, 50.0))
+    spec = PropagationSpec(dt=4e-4, nt=60, order=8, npml=20, boundary="cpml")
+    src = SourceSpec(x=3000.0, z=1500.0, f0=30.0, delay=0.01)
+    w = np.sin(np.arange(60) * 0.3).astype(np.float32) * 1e6
+    outs = []
+    for v in [(0, 64), (1, 64), (1, 10), (2, 28), (3, 127)]:
+        p = build_propagator(m, spec, engine="elastic")
+        p.set_source(src)
+        p._batches["step"] = dev.Batch([p._core], variant=v)
+        for s in range(60):
+            p.step(float(w[s]))
+        torch.cuda.synchronize()
+        outs.append(p._core.state[p._core.cur].clone())
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])

@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Batched elastic P-SV RTM benchmark on one MI355X.
+
+Workload: 64 surface shots of elastic reverse-time migration (PP and PS
+images, divcurl separation, normalized imaging condition) on a 600 x 1200
+grid at 5 m (free surface on top), nt = 4000 at dt = 0.4 ms, 1200 receivers
+recording vx and vz.  The model has six layers (vp 2200-4500 m/s,
+vs = vp/1.8, Gardner density), a circular cavity (r = 40 m, vp 1500 m/s,
+vs 1 m/s) and a 60-degree dipping fault zone.  Records are modelled on the
+GPU before timing (30-cell CPML), uploaded to HBM, and every timed step
+migrates all 64 shots, advanced together with one kernel launch per time
+step, from resident records to final (post-processed) images.
+
+One JSON line on stdout:
+  value          shots/s over the timed steps (whole job)
+  roofline       the step kernel: 52 algorithmic bytes per physical
+                 cell-update (state read+write 40 B, lambda/mu/buoyancy 12 B)
+                 divided by its mean launch time, measured with HIP events
+                 on the launch stream during the timed steps, against 8 TB/s
+  cpu_baseline   the NumPy oracle, one core, on a truncated sample of one
+                 shot, scaled to shots/s
+
+  python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--shots 64] [--nt 4000]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+# the CPU baseline is a one-core figure
+for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+    os.environ[_v] = "1"
+
+import numpy as np  # noqa: E402
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "challenge-cup_amd"))
+
+NZ, NX, DX = 600, 1200, 5.0
+DT, NT = 4.0e-4, 4000
+F0, DELAY = 30.0, 0.04
+BYTES_PER_CELL = 52  # elastic: 5 state fields read+written, 3 media fields read
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "elastic_fused_rtm64_traffic.txt")
+
+
+def layered_model():
+    """Six layers, a cavity and a dipping fault zone on the 600 x 1200 grid."""
+    from ertm.types import ElasticModel, GridSpec
+
+    z = (np.arange(NZ) * DX)[:, None] * np.ones((1, NX))
+    x = np.ones((NZ, 1)) * (np.arange(NX) * DX)[None, :]
+    tops = [0.0, 500.0, 1000.0, 1500.0, 2000.0, 2500.0]
+    vps = [2200.0, 2660.0, 3120.0, 3580.0, 4040.0, 4500.0]
+    vp = np.zeros((NZ, NX))
+    for top, v in zip(tops, vps):
+        vp[z >= top] = v
+    vs = vp / 1.8
+    rho = 310.0 * vp ** 0.25
+    # dipping fault zone: |distance to the 60-degree line through (4000, 1750)| <= 10 m
+    dip = np.deg2rad(60.0)
+    dist = np.abs((x - 4000.0) * np.sin(dip) - (z - 1750.0) * np.cos(dip))
+    band = dist <= 10.0
+    vp = np.where(band, vp * 0.6, vp)
+    vs = np.where(band, vs * 0.6, vs)
+    rho = np.where(band, rho * 0.85, rho)
+    # cavity: vs = 1 m/s keeps the model inside the validation contract (vs > 0)
+    cav = (x - 3000.0) ** 2 + (z - 1500.0) ** 2 <= 40.0 ** 2
+    vp = np.where(cav, 1500.0, vp)
+    vs = np.where(cav, 1.0, vs)
+    rho = np.where(cav, 310.0 * 1500.0 ** 0.25, rho)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    return ElasticModel(grid=GridSpec(nx=NX, nz=NZ, dx=DX, dz=DX), vp=f32(vp), vs=f32(vs),
+                        rho=f32(rho), qp=f32(np.full((NZ, NX), 100.0)),
+                        qs=f32(np.full((NZ, NX), 50.0)))
+
+
+def acquisition(nshots):
+    from ertm.types import ReceiverSpec, SourceSpec
+
+    sources = [SourceSpec(x=150.0 + 90.0 * i, z=10.0, f0=F0, delay=DELAY, amplitude=1.0,
+                          mechanism="explosive") for i in range(nshots)]
+    receivers = ReceiverSpec(x=np.arange(NX) * DX, z=np.full(NX, 10.0))
+    return sources, receivers
+
+
+def model_records(model, sources, receivers, nt):
+    """Forward-model every shot on the GPU (30-cell CPML), untimed."""
+    from ertm.config import GpuConfig, SolverConfig
+    from ertm.solver import build_solver
+
+    solver = build_solver(SolverConfig(nt=nt, dt=DT, order=8, boundary="free", engine="elastic",
+                                       npml=30), gpu_config=GpuConfig(strict_cfl=False))
+    shots = []
+    for s in sources:
+        rec = solver.forward(model, s, receivers, components=("vx", "vz"))
+        shots.append(rec)
+    return shots
+
+
+def read_traffic():
+    """HBM bytes per step launch from the committed FETCH_SIZE/WRITE_SIZE
+    profile summary (a line 'traffic_bytes_per_launch: N'), else None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def cpu_baseline(model, sources, receivers, sample_nt):
+    """The NumPy oracle on one core: one PP migration of one shot truncated to
+    ``sample_nt`` steps (forward + replay + backward with imaging every step),
+    scaled by nt/sample_nt and by 2 because the reference images PP and PS
+    with two migrate calls."""
+    from oracle import rtm as orc
+
+    m = orc.Model(nz=NZ, nx=NX, dx=DX, dz=DX, vp=np.asarray(model.vp), vs=np.asarray(model.vs),
+                  rho=np.asarray(model.rho), qp=np.asarray(model.qp))
+    s = sources[len(sources) // 2]
+    src = dict(x=s.x, z=s.z, f0=s.f0, delay=s.delay, amplitude=s.amplitude, kind="ricker",
+               mechanism=s.mechanism)
+    rng = np.random.default_rng(0)
+    rec = {c: rng.standard_normal((sample_nt, NX)) * 1e-9 for c in ("vx", "vz")}
+    cfg = orc.Imaging(mode="pp", condition="normalized", separation="divcurl")
+    t0 = time.perf_counter()
+    orc.migrate("elastic", m, DT, sample_nt, 8, "free", cfg, rec, src, receivers.x,
+                receivers.z)
+    secs = time.perf_counter() - t0
+    per_shot = secs * (NT / sample_nt) * 2.0
+    return {"value": 1.0 / per_shot, "unit": "shots/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle elastic migrate (pp, normalized, divcurl) of one {NZ}x{NX} shot "
+                       f"truncated to nt={sample_nt} ({secs:.1f} s), scaled x{NT}/{sample_nt} "
+                       f"and x2 for the separate PS call")}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=NT)
+    ap.add_argument("--cpu-sample-nt", type=int, default=240)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(shots are independent: run one process per GPU and add the shots/s).",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU (HIP device)", file=sys.stderr)
+        return 1
+    from ertm import device as dev
+    from ertm.config import GpuConfig, ImagingConfig, SolverConfig
+    from ertm.imaging import RTMImager
+
+    model = layered_model()
+    sources, receivers = acquisition(args.shots)
+    print(f"# modelling {args.shots} records, nt={args.nt}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    shots = model_records(model, sources, receivers, args.nt)
+    print(f"# records ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    imaging = ImagingConfig(mode="pp", condition="normalized", separation="divcurl")
+    solver = SolverConfig(nt=args.nt, dt=DT, order=8, boundary="free", engine="elastic", npml=30)
+    imager = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False,
+                                                             replay_budget_gb=64.0))
+    t0 = time.perf_counter()
+    job = imager.prepare_batch(shots, model, modes=("pp", "ps"))
+    torch.cuda.synchronize()
+    upload_s = time.perf_counter() - t0
+    print(f"# records aligned and resident in HBM ({upload_s:.1f} s)", file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        imager.run_batch(job)
+        print(f"# warmup {w + 1}/{args.warmup}: {imager.last_timing['run_s']:.2f} s",
+              file=sys.stderr, flush=True)
+
+    torch.cuda.synchronize()
+    parts = {"forward_s": 0.0, "replay_backward_s": 0.0, "postprocess_s": 0.0}
+    with dev.StepTimer():
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            out = imager.run_batch(job)
+            for key in parts:
+                parts[key] += imager.last_timing[key]
+            print(f"# step {k + 1}/{args.steps}: {imager.last_timing['run_s']:.2f} s",
+                  file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+    kt = dev.StepTimer.read()
+    assert all(np.isfinite(o["pp"].image).all() and np.isfinite(o["ps"].image).all() for o in out)
+
+    nshots = args.shots * args.steps
+    shots_s = nshots / elapsed
+    # RTM cell-updates: forward + replay + backward = 3 nt nz nx per shot
+    gcells = 3.0 * args.nt * NZ * NX * nshots / elapsed / 1e9
+    avg_ms = kt["total_ms"] / max(kt["launches"], 1)
+    cells_per_launch = kt["cell_updates"] / max(kt["launches"], 1)
+    achieved = BYTES_PER_CELL * cells_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    line = {
+        "metric": "RTM shots/s & Gcell-updates/s (% HBM roofline) on 1 GPU; p50/p99 shot latency",
+        "value": round(shots_s, 4),
+        "unit": "shots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (layered model with cavity and fault; records modelled on the GPU)",
+        "config": {"workload": "elastic P-SV RTM, PP+PS, 64 shots batched",
+                   "grid": [NZ, NX], "dx": DX, "nt": args.nt, "dt": DT, "shots": args.shots,
+                   "receivers": NX, "components": ["vx", "vz"], "condition": "normalized",
+                   "separation": "divcurl", "boundary": "free"},
+        "gcell_updates_per_s": round(gcells, 2),
+        "gcell_roofline_frac": round(gcells * BYTES_PER_CELL / HBM_PEAK_GBS, 4),
+        "phase_s": {k: round(v / args.steps, 3) for k, v in parts.items()},
+        "upload_s": round(upload_s, 3),
+        "roofline": {
+            "kernel": "k_elastic_fused" if kt["fused"] == 1 else "k_velocity+k_stress",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "launches": kt["launches"],
+            "avg_launch_us": round(avg_ms * 1e3, 2),
+            "bytes_per_launch": BYTES_PER_CELL * cells_per_launch,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        print(f"# cpu baseline: oracle, nt={args.cpu_sample_nt}", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(model, sources, receivers, args.cpu_sample_nt)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -12,6 +12,7 @@
 #include <map>
 #include <string>
 #include <utility>
+#include <mutex>
 #include <vector>
 
 #include "ertm_kernels.h"
@@ -261,18 +262,60 @@ int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc) {
 // ---------------------------------------------------------------------------
 // launch helpers (chunks of ERTM_MAX_LAUNCH propagators)
 // ---------------------------------------------------------------------------
+// Step-kernel timing: while enabled, every step launch is bracketed by a pair
+// of HIP events recorded on the launch stream; reading sums their elapsed
+// times.  Event pairs are pooled and reused across reads.
+namespace {
+struct StepTimer {
+    std::mutex m;
+    bool on = false;
+    std::vector<hipEvent_t> ev;  // 2 per recorded launch
+    size_t used = 0;             // events in use
+    int64_t cells = 0;           // physical cell-updates of the recorded launches
+    int32_t fused = -1;          // 1 fused kernel, 0 two-launch, 2 both seen
+};
+StepTimer g_timer;
+
+int timer_event(hipEvent_t* out) {
+    if (g_timer.used == g_timer.ev.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        g_timer.ev.push_back(e);
+    }
+    *out = g_timer.ev[g_timer.used++];
+    return ERTM_OK;
+}
+}  // namespace
+
 static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s) {
     const int n = (int)b->props.size();
+    std::unique_lock<std::mutex> lk(g_timer.m, std::defer_lock);
+    const bool timed = g_timer.on;
+    if (timed) lk.lock();
     for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
         la.base = base;
         la.n = std::min(ERTM_MAX_LAUNCH, n - base);
         la.cur = cur;
         for (int k = 0; k < la.n; ++k) la.amp[k] = amp ? amp[base + k] : 0.0f;
-        if (b->variant == 0 ||
-            !launch_step_fused(b->dprops, b->props.data(), la, b->order, b->variant, b->zc, s))
-            launch_step(b->dprops, la, b->order, b->max_tiles, b->any_force, s);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) {
+            int rc = timer_event(&e0);
+            if (rc == ERTM_OK) rc = timer_event(&e1);
+            if (rc != ERTM_OK) return rc;
+            HIPCHK(hipEventRecord(e0, s));
+        }
+        bool fused = b->variant != 0 &&
+                     launch_step_fused(b->dprops, b->props.data(), la, b->order, b->variant, b->zc, s);
+        if (!fused) launch_step(b->dprops, la, b->order, b->max_tiles, b->any_force, s);
         LAUNCHCHK("step");
+        if (timed) {
+            HIPCHK(hipEventRecord(e1, s));
+            for (int k = 0; k < la.n; ++k)
+                g_timer.cells += (int64_t)b->props[base + k].nz * b->props[base + k].nx;
+            const int32_t f = fused ? 1 : 0;
+            g_timer.fused = g_timer.fused < 0 ? f : (g_timer.fused == f ? f : 2);
+        }
     }
     return ERTM_OK;
 }
@@ -381,6 +424,36 @@ static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int
 
 extern "C" {
 
+int ertm_step_timing(int32_t enable) {
+    std::lock_guard<std::mutex> lk(g_timer.m);
+    if (enable) {
+        g_timer.used = 0;
+        g_timer.cells = 0;
+        g_timer.fused = -1;
+    }
+    g_timer.on = enable != 0;
+    return ERTM_OK;
+}
+
+int ertm_step_timing_read(double* total_ms, int64_t* launches, int64_t* cell_updates,
+                          int32_t* fused) {
+    if (!total_ms || !launches || !cell_updates || !fused)
+        return fail(ERTM_EINVAL, "step_timing_read: bad arguments");
+    std::lock_guard<std::mutex> lk(g_timer.m);
+    double sum = 0.0;
+    for (size_t i = 0; i + 1 < g_timer.used; i += 2) {
+        HIPCHK(hipEventSynchronize(g_timer.ev[i + 1]));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, g_timer.ev[i], g_timer.ev[i + 1]));
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = (int64_t)(g_timer.used / 2);
+    *cell_updates = g_timer.cells;
+    *fused = g_timer.fused;
+    return ERTM_OK;
+}
+
 int ertm_step(ertm_batch_t b, int32_t cur, const float* amp, void* stream) {
     if (!b || (cur & ~1)) return fail(ERTM_EINVAL, "step: bad arguments");
     return do_step(b, cur, amp, (hipStream_t)stream);
@@ -432,6 +505,15 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
     if (r->phase != 2 && r->ck_interval > 0 && !r->save_arenas)
         return fail(ERTM_EINVAL, "run: checkpoint arenas missing");
     if (r->stabilize && !b->all_visco) return fail(ERTM_EINVAL, "run: stabilize needs visco");
+    for (int k = 0; k < n; ++k) {
+        const int64_t sf = b->props[k].state_floats;
+        if (r->phase == 1 && r->restore_stride < sf)
+            return fail(ERTM_EINVAL, "run: restore_stride %lld below state size %lld of prop %d",
+                        (long long)r->restore_stride, (long long)sf, k);
+        if (r->phase != 2 && r->ck_interval > 0 && r->save_stride < sf)
+            return fail(ERTM_EINVAL, "run: save_stride %lld below state size %lld of prop %d",
+                        (long long)r->save_stride, (long long)sf, k);
+    }
     if (r->op_after > ERTM_IMG_EXC_APPLY) return fail(ERTM_EINVAL, "run: bad image op");
     std::vector<float> amp(n, 0.0f);
     std::vector<int32_t> tt(n, 0);

@@ -159,14 +159,24 @@ class GpuConfig:
                    one shot before falling back to nested (segment) replay.
     replay_slots   maximum image-field slots held per replay segment
                    (0 = one whole checkpoint block).
+    replay_budget_gb  HBM for the replay slots of the shots migrated together
+                   (capped at a quarter of the free HBM); over it a block is
+                   replayed in segments from sub-checkpoints.
+    max_batch      most shots advanced together by ``migrate_batch``.
     """
 
     strict_cfl: bool = True
     device_mem_gb: float = 200.0
     replay_slots: int = 0
+    replay_budget_gb: float = 16.0
+    max_batch: int = 64
 
     def validate(self) -> None:
         if self.device_mem_gb <= 0:
             raise ConfigError("gpu.device_mem_gb must be positive")
         if self.replay_slots < 0:
             raise ConfigError("gpu.replay_slots must not be negative")
+        if self.replay_budget_gb <= 0:
+            raise ConfigError("gpu.replay_budget_gb must be positive")
+        if self.max_batch < 1:
+            raise ConfigError("gpu.max_batch must be at least 1")

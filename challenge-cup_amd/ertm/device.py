@@ -551,3 +551,27 @@ def absmax(t) -> float:
     L.check(lib.ertm_absmax(C.c_void_p(L.ptr(t)), t.numel(), C.c_void_p(L.ptr(out)),
                             L.stream_handle()), "ertm_absmax")
     return float(out.item())
+
+
+class StepTimer:
+    """HIP-event timing of every step-kernel launch while active."""
+
+    def __enter__(self):
+        L.check(L.lib().ertm_step_timing(1), "ertm_step_timing")
+        return self
+
+    def __exit__(self, *exc):
+        L.check(L.lib().ertm_step_timing(0), "ertm_step_timing")
+        return False
+
+    @staticmethod
+    def read() -> dict:
+        ms = C.c_double(0.0)
+        n = C.c_int64(0)
+        cells = C.c_int64(0)
+        fused = C.c_int32(-1)
+        L.check(L.lib().ertm_step_timing_read(C.byref(ms), C.byref(n), C.byref(cells),
+                                              C.byref(fused)), "ertm_step_timing_read")
+        return {"total_ms": float(ms.value), "launches": int(n.value),
+                "cell_updates": int(cells.value), "fused": int(fused.value)}
+

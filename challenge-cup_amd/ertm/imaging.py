@@ -85,7 +85,9 @@ class CheckpointStore:
 
     ``states`` lists (step, arena slot); ``bytes_per_state`` is the reference's
     budgeting size (fields only, see record_source_wavefield); ``arena_bytes``
-    is what the arena really holds (fields + CPML strips per state).
+    is what the arena really holds (fields + CPML strips per state);
+    ``state_floats`` is the arena's slot stride (the largest state of the
+    shots recorded together).
     """
 
     states: list[tuple[int, int]] = field(default_factory=list)
@@ -113,6 +115,21 @@ class CheckpointStore:
         return start, end
 
 
+@dataclass
+class PreparedBatch:
+    """Shots ready to migrate: validated, specs resolved, records aligned to
+    (nt, dt) and resident in HBM as float32 (nt, nrec) tensors."""
+
+    shots: list
+    models: list
+    specs: list
+    records: list          # per shot: {record name: device tensor}
+    waves: list            # per shot: float32 (nt,) source wavelet
+    modes: tuple
+    swave: str
+    rwaves: tuple
+
+
 def checkpoint_interval(nt: int, per_state: int, budget_mb: float) -> int:
     """ertm/imaging.py:121-124."""
     budget = max(float(budget_mb), 1.0) * 1024.0 ** 2
@@ -128,24 +145,80 @@ def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
     step (fields only: the CPML memories do not exist yet), exactly as the
     reference does; the arena itself stores fields and CPML strips.
     """
+    return record_source_wavefields([prop], [wavelet], budget_mb=budget_mb)[0]
+
+
+def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarray], *,
+                             budget_mb: float) -> list[CheckpointStore]:
+    """``record_source_wavefield`` for several shots advanced in one launch per step.
+
+    All shots must share nt and the checkpoint interval (the interval follows
+    from nt and the fields-only state size, so shots on grids of the same
+    size always do).
+    """
     import torch
 
-    nt = int(len(wavelet))
-    per_state = max(int(prop.state_bytes()), 1)
-    interval = checkpoint_interval(nt, per_state, budget_mb)
+    nts = {int(len(w)) for w in wavelets}
+    if len(nts) != 1:
+        raise ImagingError("batched shots must share nt")
+    nt = nts.pop()
+    intervals = {checkpoint_interval(nt, max(int(p.state_bytes()), 1), budget_mb) for p in props}
+    if len(intervals) != 1:
+        raise ImagingError("batched shots must share the checkpoint interval")
+    interval = intervals.pop()
     starts = list(range(0, nt, interval))
-    core = prop._core
-    arena = torch.empty((len(starts), core.state_floats), dtype=torch.float32, device="cuda")
-    b = dev.Batch([core])
-    w = np.asarray(wavelet, dtype=np.float32)
-    core.cur = b.run(0, 0, nt, core.cur, wavelets=[w], ck_interval=interval, ck_origin=0,
-                     save=[arena], save_stride=core.state_floats, step_index0=prop.step_index,
-                     **prop._run_kwargs())
+    cores = [p._core for p in props]
+    # one stride for every arena (grids may differ): the largest state
+    stride = max(c.state_floats for c in cores)
+    arenas = [torch.empty((len(starts), stride), dtype=torch.float32, device="cuda")
+              for _c in cores]
+    per_state = [max(int(p.state_bytes()), 1) for p in props]
+    step0 = {p.step_index for p in props}
+    curs = {c.cur for c in cores}
+    if len(step0) != 1 or len(curs) != 1:
+        raise ImagingError("batched propagators must be at the same step")
+    b = dev.Batch(cores)
+    ws = [np.asarray(w, dtype=np.float32) for w in wavelets]
+    kw = _batch_kwargs(props)
+    cur = b.run(0, 0, nt, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
+                save=arenas, save_stride=stride, step_index0=step0.pop(), **kw)
     b.close()
-    prop.pml.touched = True
-    prop.step_index += nt
-    return CheckpointStore(states=[(s, k) for k, s in enumerate(starts)], interval=interval,
-                           bytes_per_state=per_state, arena=arena, state_floats=core.state_floats)
+    _after_batch(props, kw)
+    stores = []
+    for p, c, arena, ps in zip(props, cores, arenas, per_state):
+        c.cur = cur
+        p.pml.touched = True
+        p.step_index += nt
+        stores.append(CheckpointStore(states=[(s, k) for k, s in enumerate(starts)],
+                                      interval=interval, bytes_per_state=ps, arena=arena,
+                                      state_floats=stride))
+    return stores
+
+
+def _common(values: set, what: str):
+    if len(values) != 1:
+        raise ImagingError(f"batched shots must share the {what}")
+    return next(iter(values))
+
+
+def _batch_kwargs(props: list[Propagator2D]) -> dict:
+    """ertm_run options of a batch: the viscoacoustic stabiliser needs one
+    control record (4 doubles) per propagator in one contiguous buffer."""
+    kws = [p._run_kwargs() for p in props]
+    if not kws[0]:
+        return {}
+    import torch
+
+    kw = dict(kws[0])
+    if len(props) > 1:
+        kw["ctl"] = torch.cat([k["ctl"] for k in kws])
+    return kw
+
+
+def _after_batch(props: list[Propagator2D], kw: dict) -> None:
+    if len(props) > 1 and "ctl" in kw:
+        for k, p in enumerate(props):
+            p._run_kwargs()["ctl"].copy_(kw["ctl"][4 * k: 4 * k + 4])
 
 
 def replay_block(prop: Propagator2D, wavelet: np.ndarray, store: CheckpointStore,
@@ -230,22 +303,46 @@ class RTMImager:
         Each image equals what ``migrate`` returns for that mode alone; the
         forward, replay and backward propagations are shared.
         """
+        return self.migrate_batch([shot], [model], modes=modes,
+                                  wavelets=None if wavelet is None else [wavelet],
+                                  preprocess=preprocess)[0]
+
+    def migrate_batch(self, shots: list[ShotRecord], models: list[ElasticModel] | ElasticModel,
+                      *, modes: tuple[str, ...] | None = None,
+                      wavelets: list[np.ndarray] | None = None,
+                      preprocess: Callable[[np.ndarray, float], np.ndarray] | None = None
+                      ) -> list[dict[str, ImageResult]]:
+        """Migrate many shots; shots that share nt and the checkpoint interval
+        are advanced together, up to ``gpu.max_batch`` per launch, on grids of
+        any (also different) sizes.  Every result equals what
+        ``migrate_modes`` returns for that shot alone."""
         t_start = time.perf_counter()
-        try:
-            shot.validate()
-            model.validate()
-        except ValueError as exc:
-            raise ImagingError(f"invalid imaging input: {exc}") from exc
-        spec = PropagationSpec.from_config(self.solver_config, model)
-        try:
-            if self.gpu_config.strict_cfl:
-                check_stability(model, spec)
-            else:
-                check_hard_stability(model, spec)
-        except Exception as exc:  # SolverError -> ImagingError
-            raise ImagingError(str(exc)) from exc
+        job = self.prepare_batch(shots, models, modes=modes, wavelets=wavelets,
+                                 preprocess=preprocess)
+        t_ready = time.perf_counter()
+        out = self.run_batch(job)
+        self.last_timing["prepare_s"] = t_ready - t_start
+        self.last_timing["total_s"] = time.perf_counter() - t_start
+        for res in out:
+            for r in res.values():
+                r.meta["perf"] = dict(self.last_timing)
+        return out
+
+    def prepare_batch(self, shots: list[ShotRecord], models: list[ElasticModel] | ElasticModel,
+                      *, modes: tuple[str, ...] | None = None,
+                      wavelets: list[np.ndarray] | None = None,
+                      preprocess: Callable[[np.ndarray, float], np.ndarray] | None = None
+                      ) -> "PreparedBatch":
+        """Validate, align the records to (nt, dt) and upload them to HBM
+        (ertm/imaging.py:205-245, :386-423); ``run_batch`` does the rest."""
+        if isinstance(models, ElasticModel):
+            models = [models] * len(shots)
+        if len(models) != len(shots):
+            raise ImagingError("one model per shot expected")
+        if wavelets is not None and len(wavelets) != len(shots):
+            raise ImagingError("one wavelet per shot expected")
+        modes = tuple(str(m).lower() for m in (modes or (self.config.mode,)))
         engine = self.solver_config.engine
-        modes = tuple(str(m).lower() for m in modes)
         for m in modes:
             if m not in WAVE_MODES:
                 raise ImagingError(f"unsupported imaging mode {m!r}")
@@ -254,28 +351,86 @@ class RTMImager:
         swaves = {WAVE_MODES[m][0] for m in modes}
         if len(swaves) != 1 or len(modes) > 2:
             raise ImagingError("fused modes must share the source wave (at most two modes)")
-        swave = swaves.pop()
-        rwaves = tuple(WAVE_MODES[m][1] for m in modes)
-        condition = str(self.config.condition).lower()
-        separation = self.config.separation
+        specs, records, waves = [], [], []
+        for i, (shot, model) in enumerate(zip(shots, models)):
+            try:
+                shot.validate()
+                model.validate()
+            except ValueError as exc:
+                raise ImagingError(f"invalid imaging input: {exc}") from exc
+            spec = PropagationSpec.from_config(self.solver_config, model)
+            try:
+                if self.gpu_config.strict_cfl:
+                    check_stability(model, spec)
+                else:
+                    check_hard_stability(model, spec)
+            except Exception as exc:  # SolverError -> ImagingError
+                raise ImagingError(str(exc)) from exc
+            record = self._prepare_record(shot, spec, preprocess)
+            if not record:
+                raise ImagingError("empty shot record: nothing to back-propagate")
+            specs.append(spec)
+            records.append({nm: dev.to_device(np.asarray(v, dtype=np.float32))
+                            for nm, v in record.items()})
+            waves.append(self._prepare_wavelet(shot, spec,
+                                               None if wavelets is None else wavelets[i]))
+        return PreparedBatch(shots=list(shots), models=list(models), specs=specs,
+                             records=records, waves=waves, modes=modes, swave=swaves.pop(),
+                             rwaves=tuple(WAVE_MODES[m][1] for m in modes))
 
-        record = self._prepare_record(shot, spec, preprocess)
-        if not record:
-            raise ImagingError("empty shot record: nothing to back-propagate")
-        wave = self._prepare_wavelet(shot, spec, wavelet)
-        img, info = self._run_shot(model, spec, engine, shot, record, wave, swave, rwaves,
-                                   condition, separation)
-        results = {}
-        for k, m in enumerate(modes):
-            image = self._postprocess(img[k], model, spec)
-            res = ImageResult(image=np.asarray(image, dtype=np.float32), grid=model.grid, mode=m,
-                              meta=self._meta(spec, engine, shot, info, condition, m))
-            res.validate()
-            results[m] = res
-        self.last_timing["total_s"] = time.perf_counter() - t_start
-        for r in results.values():
-            r.meta["perf"] = dict(self.last_timing)
+    def run_batch(self, job: "PreparedBatch") -> list[dict[str, ImageResult]]:
+        """Forward, replay, backward and imaging of prepared shots; records are
+        already in HBM.  Returns one {mode: ImageResult} per shot."""
+        t_start = time.perf_counter()
+        engine = self.solver_config.engine
+        condition = str(self.config.condition).lower()
+        groups: dict[tuple[int, int], list[int]] = {}
+        for i, (model, spec) in enumerate(zip(job.models, job.specs)):
+            groups.setdefault(self._batch_key(model, spec), []).append(i)
+        results: list[dict[str, ImageResult] | None] = [None] * len(job.shots)
+        timing = {"forward_s": 0.0, "replay_backward_s": 0.0, "postprocess_s": 0.0,
+                  "cell_updates": 0.0, "shots": 0.0, "launch_groups": 0.0}
+        cap = int(self.gpu_config.max_batch)
+        for members in groups.values():
+            for c0 in range(0, len(members), cap):
+                idx = members[c0:c0 + cap]
+                imgs, infos = self._run_shots(
+                    [job.models[i] for i in idx], [job.specs[i] for i in idx], engine,
+                    [job.shots[i] for i in idx], [job.records[i] for i in idx],
+                    [job.waves[i] for i in idx], job.swave, job.rwaves, condition,
+                    self.config.separation)
+                for key in ("forward_s", "replay_backward_s", "cell_updates", "shots"):
+                    timing[key] += self.last_timing.get(key, 0.0)
+                timing["launch_groups"] += 1.0
+                t_post = time.perf_counter()
+                for j, i in enumerate(idx):
+                    spec = job.specs[i]
+                    out = {}
+                    for k, m in enumerate(job.modes):
+                        image = self._postprocess(imgs[j][k], job.models[i], spec)
+                        res = ImageResult(image=np.asarray(image, dtype=np.float32),
+                                          grid=job.models[i].grid, mode=m,
+                                          meta=self._meta(spec, engine, job.shots[i], infos[j],
+                                                          condition, m))
+                        res.validate()
+                        out[m] = res
+                    results[i] = out
+                timing["postprocess_s"] += time.perf_counter() - t_post
+        timing["run_s"] = time.perf_counter() - t_start
+        self.last_timing = timing
         return results
+
+    def _batch_key(self, model: ElasticModel, spec: PropagationSpec) -> tuple[int, int]:
+        """(nt, checkpoint interval): shots with equal keys run in one launch.
+        The interval follows the reference's rule on the fields-only state
+        size (ertm/imaging.py:121-124)."""
+        engine = self.solver_config.engine
+        nf = {"acoustic": 3, "elastic": 5}.get(engine, 4 + int(self.q_config.n_mechanisms))
+        pad = num.ghost_width(int(spec.order))
+        nz, nx = model.shape
+        per_state = nf * (nz + 2 * pad) * (nx + 2 * pad) * 4
+        return int(spec.nt), checkpoint_interval(int(spec.nt), per_state,
+                                                 float(self.config.checkpoint_mb))
 
     def image_condition(self, source_wavefield: np.ndarray, receiver_wavefield: np.ndarray):
         """Per-time-step integrand (ertm/imaging.py:351-368)."""
@@ -294,134 +449,166 @@ class RTMImager:
             return "vz"
         return name
 
-    def _run_shot(self, model, spec, engine, shot, record, wave, swave, rwaves, condition,
-                  separation):
+    def _run_shots(self, models, specs, engine, shots, records, waves, swave, rwaves, condition,
+                   separation):
+        """Forward, replay and backward passes of several shots, each pass
+        advancing every shot with one launch per time step.  The shots share
+        nt and the checkpoint interval; grids may differ in size.  Each
+        shot's result is the one it gets alone."""
         import torch
 
         timing = {}
         t0 = time.perf_counter()
-        nt = int(spec.nt)
-        nz, nx = model.shape
-        fwd = build_propagator(model, spec, engine=engine, q_options=self.q_options(compensation=False))
-        fwd.set_source(shot.source)
-        fwd.set_receivers(shot.receivers)
-        store = record_source_wavefield(fwd, wave, budget_mb=float(self.config.checkpoint_mb))
-        back = build_propagator(model, spec, engine=engine, q_options=self.q_options(compensation=True))
-        back.set_source(shot.source)
-        back.set_receivers(shot.receivers)
-        names = list(record)
-        recs = [dev.to_device(np.asarray(record[n], dtype=np.float32)) for n in names]
-        tab = dev.RecvTables()
-        tab.build_injection(back._core, shot.receivers.x, shot.receivers.z,
-                            [self._routing(n) for n in names], back._k_relaxed_host())
-        tab.set_records(recs)
-        start_step = int(np.floor((float(shot.source.delay) + float(self.config.start_periods)
-                                   / max(float(shot.source.f0), 1e-6)) / spec.dt))
-        start_step = int(np.clip(start_step, 0, nt))
+        n = len(shots)
+        nt = _common({int(sp.nt) for sp in specs}, "number of time steps")
+        fwds, backs, tabs = [], [], []
+        for model, spec, shot in zip(models, specs, shots):
+            fwd = build_propagator(model, spec, engine=engine,
+                                   q_options=self.q_options(compensation=False))
+            fwd.set_source(shot.source)
+            fwd.set_receivers(shot.receivers)
+            fwds.append(fwd)
+        stores = record_source_wavefields(fwds, waves, budget_mb=float(self.config.checkpoint_mb))
+        for model, spec, shot, record in zip(models, specs, shots, records):
+            back = build_propagator(model, spec, engine=engine,
+                                    q_options=self.q_options(compensation=True))
+            back.set_source(shot.source)
+            back.set_receivers(shot.receivers)
+            names = list(record)
+            recs = [record[nm] for nm in names]
+            tab = dev.RecvTables()
+            tab.build_injection(back._core, shot.receivers.x, shot.receivers.z,
+                                [self._routing(nm) for nm in names], back._k_relaxed_host())
+            tab.set_records(recs)
+            backs.append(back)
+            tabs.append(tab)
+        starts = []
+        for shot, spec in zip(shots, specs):
+            st = int(np.floor((float(shot.source.delay) + float(self.config.start_periods)
+                               / max(float(shot.source.f0), 1e-6)) / spec.dt))
+            starts.append(int(np.clip(st, 0, nt)))
         illum = condition == "normalized" or bool(self.config.receiver_normalization)
-        fcore, bcore = fwd._core, back._core
-        bkw = back._run_kwargs()
+        fcores = [f._core for f in fwds]
+        bcores = [bk._core for bk in backs]
+        bkw = _batch_kwargs(backs)
         back_steps = 0
-        nblocks = len(store.states)
-        w = np.asarray(wave, dtype=np.float32)
+        store0 = stores[0]
+        nblocks = len(store0.states)
+        ws = [np.asarray(w, dtype=np.float32) for w in waves]
+        arenas = [st.arena for st in stores]
+        astride = _common({st.state_floats for st in stores}, "state size")
+        fcur = _common({c.cur for c in fcores}, "state parity")
+        bcur = _common({c.cur for c in bcores}, "state parity")
         replayed = 0
         torch.cuda.synchronize()
         timing["forward_s"] = time.perf_counter() - t0
         t1 = time.perf_counter()
         if condition == "excitation":
-            ib = dev.ImageBuffers(nz, nx, separation, "P", rwaves[:1], start_step, False,
-                                  excitation=True)
-            fb = dev.Batch([fcore], imgs=[ib])
+            ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, "P", rwaves[:1], st,
+                                    False, excitation=True) for m, st in zip(models, starts)]
+            fb = dev.Batch(fcores, imgs=ibs)
             for blk in range(nblocks):
-                s0, s1 = store.bounds(blk, nt)
-                fcore.cur = fb.run(1, s0, s1, fcore.cur, wavelets=[w], op_after=L.IMG_EXC_PEAK,
-                                   restore=[store.arena], restore_stride=store.state_floats,
-                                   restore_slot=store.states[blk][1])
+                s0, s1 = store0.bounds(blk, nt)
+                fcur = fb.run(1, s0, s1, fcur, wavelets=ws, op_after=L.IMG_EXC_PEAK,
+                              restore=arenas, restore_stride=astride,
+                              restore_slot=store0.states[blk][1])
                 replayed += s1 - s0
             fb.close()
-            bb = dev.Batch([bcore], recvs=[tab], imgs=[ib])
+            bb = dev.Batch(bcores, recvs=tabs, imgs=ibs)
             for blk in range(nblocks - 1, -1, -1):
-                s0, s1 = store.bounds(blk, nt)
-                bcore.cur = bb.run(2, s0, s1, bcore.cur, op_after=L.IMG_EXC_APPLY,
-                                   step_index0=back_steps, **bkw)
+                s0, s1 = store0.bounds(blk, nt)
+                bcur = bb.run(2, s0, s1, bcur, op_after=L.IMG_EXC_APPLY,
+                              step_index0=back_steps, **bkw)
                 back_steps += s1 - s0
             bb.close()
-            num_t = ib.num
-            ill = ib.peak_amp ** 2
-            rill = ib.rill
+            accs = [(ib.num, ib.peak_amp ** 2, ib.rill) for ib in ibs]
         else:
-            slots = self._replay_slots(store.interval, nz * nx)
-            rbuf = torch.empty((slots, nz, nx), dtype=torch.float32, device="cuda")
-            ib = dev.ImageBuffers(nz, nx, separation, swave, rwaves, start_step, illum, rbuf=rbuf,
-                                  nslots=slots)
-            fimg = dev.ImageBuffers(nz, nx, separation, swave, (), 0, False, rbuf=rbuf,
-                                    nslots=slots)
-            fb = dev.Batch([fcore], imgs=[fimg])
-            bb = dev.Batch([bcore], recvs=[tab], imgs=[ib])
-            sub = None
-            if slots < store.interval:
-                nsub = int(math.ceil(store.interval / slots))
-                sub = torch.empty((nsub, fcore.state_floats), dtype=torch.float32, device="cuda")
+            cells = sum(m.shape[0] * m.shape[1] for m in models)
+            slots = self._replay_slots(store0.interval, cells)
+            rbufs = [torch.empty((slots, m.shape[0], m.shape[1]), dtype=torch.float32,
+                                 device="cuda") for m in models]
+            ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, rwaves, st, illum,
+                                    rbuf=rb, nslots=slots)
+                   for m, st, rb in zip(models, starts, rbufs)]
+            fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, False,
+                                      rbuf=rb, nslots=slots) for m, rb in zip(models, rbufs)]
+            fb = dev.Batch(fcores, imgs=fimgs)
+            bb = dev.Batch(bcores, recvs=tabs, imgs=ibs)
+            subs = None
+            if slots < store0.interval:
+                nsub = int(math.ceil(store0.interval / slots))
+                subs = [torch.empty((nsub, astride), dtype=torch.float32, device="cuda")
+                        for _c in fcores]
             for blk in range(nblocks - 1, -1, -1):
-                s0, s1 = store.bounds(blk, nt)
+                s0, s1 = store0.bounds(blk, nt)
+                slot = store0.states[blk][1]
                 segs = [(a, min(a + slots, s1)) for a in range(s0, s1, slots)]
                 if len(segs) > 1:
                     # sub-checkpoints: replay state just before w[a] of each later segment
-                    fcore.cur = fb.run(1, s0, segs[-1][0], fcore.cur, wavelets=[w],
-                                       ck_interval=slots, ck_origin=s0 + slots - 1, save=[sub],
-                                       save_stride=fcore.state_floats, restore=[store.arena],
-                                       restore_stride=store.state_floats,
-                                       restore_slot=store.states[blk][1])
+                    fcur = fb.run(1, s0, segs[-1][0], fcur, wavelets=ws, ck_interval=slots,
+                                  ck_origin=s0 + slots - 1, save=subs, save_stride=astride,
+                                  restore=arenas, restore_stride=astride, restore_slot=slot)
                     replayed += segs[-1][0] - s0
                 for k in range(len(segs) - 1, -1, -1):
                     a, e = segs[k]
-                    fimg.s.t0 = a
-                    ib.s.t0 = a
-                    fb.update_images([fimg])
-                    bb.update_images([ib])
+                    for fi, ib in zip(fimgs, ibs):
+                        fi.s.t0 = a
+                        ib.s.t0 = a
+                    fb.update_images(fimgs)
+                    bb.update_images(ibs)
                     if k == 0:
-                        src = dict(restore=[store.arena], restore_stride=store.state_floats,
-                                   restore_slot=store.states[blk][1])
+                        src = dict(restore=arenas, restore_stride=astride, restore_slot=slot)
                     else:
-                        src = dict(restore=[sub], restore_stride=fcore.state_floats,
-                                   restore_slot=k - 1)
-                    fcore.cur = fb.run(1, a, e, fcore.cur, wavelets=[w], op_after=L.IMG_EXTRACT,
-                                       **src)
+                        src = dict(restore=subs, restore_stride=astride, restore_slot=k - 1)
+                    fcur = fb.run(1, a, e, fcur, wavelets=ws, op_after=L.IMG_EXTRACT, **src)
                     replayed += e - a
-                    bcore.cur = bb.run(2, a, e, bcore.cur, op_after=L.IMG_CORRELATE,
-                                       step_index0=back_steps, **bkw)
+                    bcur = bb.run(2, a, e, bcur, op_after=L.IMG_CORRELATE,
+                                  step_index0=back_steps, **bkw)
                     back_steps += e - a
             fb.close()
             bb.close()
-            num_t, ill, rill = ib.num, ib.ill, ib.rill
+            accs = [(ib.num, ib.ill, ib.rill) for ib in ibs]
+        _after_batch(backs, bkw)
         torch.cuda.synchronize()
         timing["replay_backward_s"] = time.perf_counter() - t1
-        back.step_index = back_steps
-        back.pml.touched = True
+        for c in fcores:
+            c.cur = fcur
+        for c in bcores:
+            c.cur = bcur
         eps = float(self.config.eps)
-        outs = []
-        ill_h = ill.cpu().numpy()
-        for k in range(len(rwaves)):
-            image = num_t[k].cpu().numpy()
-            if condition == "normalized" and self.config.illumination:
-                image = image / (ill_h + eps)
-            if self.config.receiver_normalization:
-                image = image / (rill[k].cpu().numpy() + eps)
-            outs.append(image)
-        info = dict(store=store, start_step=start_step, prop=fwd, back=back,
-                    replayed_steps=replayed, nt=nt, cell_updates=(nt + replayed + back_steps) * nz * nx)
-        timing["cell_updates"] = float(info["cell_updates"])
+        all_outs, infos = [], []
+        for j in range(n):
+            num_t, ill, rill = accs[j]
+            backs[j].step_index = back_steps
+            backs[j].pml.touched = True
+            outs = []
+            ill_h = ill.cpu().numpy()
+            for k in range(len(rwaves)):
+                image = num_t[k].cpu().numpy()
+                if condition == "normalized" and self.config.illumination:
+                    image = image / (ill_h + eps)
+                if self.config.receiver_normalization:
+                    image = image / (rill[k].cpu().numpy() + eps)
+                outs.append(image)
+            nz, nx = models[j].shape
+            all_outs.append(outs)
+            infos.append(dict(store=stores[j], start_step=starts[j], prop=fwds[j], back=backs[j],
+                              replayed_steps=replayed, nt=nt,
+                              cell_updates=(nt + replayed + back_steps) * nz * nx))
+        timing["cell_updates"] = float(sum(i["cell_updates"] for i in infos))
+        timing["shots"] = float(n)
         self.last_timing = timing
-        return outs, info
+        return all_outs, infos
 
     def _replay_slots(self, interval: int, cells: int) -> int:
-        """Replay slots per segment: a whole block unless over the replay budget."""
+        """Replay slots per segment: a whole block unless over the replay
+        budget (``cells`` summed over the shots advanced together)."""
         import torch
 
         if self.gpu_config.replay_slots > 0:
             return max(1, min(interval, int(self.gpu_config.replay_slots)))
         free, _total = torch.cuda.mem_get_info()
-        budget = min(16 * 1024 ** 3, 0.25 * free)
+        budget = min(float(self.gpu_config.replay_budget_gb) * 1024 ** 3, 0.25 * free)
         return max(1, min(interval, int(budget // (4 * cells))))
 
     # ------------------------------------------------------------ helpers

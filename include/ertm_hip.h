@@ -306,6 +306,18 @@ int ertm_absmax(const float* x, int64_t n, float* out, void* stream);
 /* Block until all work on the stream is done. */
 int ertm_sync(void* stream);
 
+/* ---- measurement (no counterpart in the reference) ------------------------ */
+/* Step-kernel timing.  While enabled, each step launch of ertm_step / ertm_run
+ * (one launch per group of up to ERTM_MAX_LAUNCH propagators) is bracketed by
+ * two HIP events recorded on the launch stream.  Enabling clears earlier
+ * records.  ertm_step_timing_read waits for the recorded events and returns
+ * the summed kernel time (ms), the number of launches, the physical
+ * cell-updates (nz*nx per propagator) they covered and whether they were the
+ * fused kernel (1), the two-launch pair (0), both (2) or none (-1). */
+int ertm_step_timing(int32_t enable);
+int ertm_step_timing_read(double* total_ms, int64_t* launches, int64_t* cell_updates,
+                          int32_t* fused);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,127 @@
+"""Batched migration: shots advanced together give each shot its alone result.
+
+``RTMImager.migrate_batch`` advances every shot of a launch group with one
+kernel launch per time step.  These tests check, bit for bit, that a shot's
+image inside a batch equals the reference golden image (for the golden shot)
+and equals the image the same shot gets from a plain ``migrate`` call (for
+the others), for equal and for different grid sizes in one launch.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from golden_cases import migrate_case
+from product_cases import migrate_objects, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+VISCO_COMP_TOL_IMAGE = 1e-4
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _engine():
+    from ertm import _lib
+
+    _lib.lib()
+
+
+def _moved(shot, dx):
+    src = dataclasses.replace(shot.source, x=float(shot.source.x) + dx)
+    return dataclasses.replace(shot, source=src)
+
+
+@pytest.mark.parametrize("tag", ["ac_zero_lag", "ac_excitation", "el_pp_divcurl", "el_ss_exc",
+                                 "vi_noq", "vi_norm"])
+def test_batch_equals_alone_and_golden(tag):
+    _say("batch vs alone", tag)
+    from ertm.imaging import RTMImager
+
+    c = migrate_case(tag)
+    shot, model, imaging, solver, q = migrate_objects(c)
+    imager = RTMImager(imaging, solver, q)
+    shots = [shot, _moved(shot, 2 * model.grid.dx), _moved(shot, -3 * model.grid.dx)]
+    out = imager.migrate_batch(shots, model)
+    assert imager.last_timing["launch_groups"] == 1.0
+    mode = imaging.mode
+    if c["engine"] == "viscoacoustic" and c["q"]["enabled"]:
+        # compensated viscoacoustic images carry the hipFFT low-pass
+        assert rel_l2(out[0][mode].image, c["image"]) < VISCO_COMP_TOL_IMAGE
+    else:
+        assert np.array_equal(out[0][mode].image, c["image"])
+    for s, o in zip(shots[1:], out[1:]):
+        alone = RTMImager(imaging, solver, q).migrate(s, model)
+        assert np.array_equal(o[mode].image, alone.image)
+        assert o[mode].meta["checkpoints"] == alone.meta["checkpoints"]
+
+
+def test_batch_of_uneven_grids_in_one_launch():
+    """Two grids of different size with the same checkpoint interval."""
+    _say("batch uneven grids")
+    from ertm.imaging import RTMImager
+    from ertm.types import ElasticModel, GridSpec
+
+    c = migrate_case("ac_zero_lag")
+    shot, model, imaging, solver, q = migrate_objects(c)
+    nz, nx = model.shape
+    g = model.grid
+    pad = lambda a: np.pad(np.asarray(a), ((0, 1), (0, 2)), mode="edge")  # noqa: E731
+    big = ElasticModel(grid=GridSpec(nx=nx + 2, nz=nz + 1, dx=g.dx, dz=g.dz, x0=g.x0, z0=g.z0),
+                       vp=pad(model.vp), vs=pad(model.vs), rho=pad(model.rho), qp=pad(model.qp),
+                       qs=pad(model.qs))
+    imager = RTMImager(imaging, solver, q)
+    from ertm.propagation import PropagationSpec
+
+    k0 = imager._batch_key(model, PropagationSpec.from_config(solver, model))
+    k1 = imager._batch_key(big, PropagationSpec.from_config(solver, big))
+    if k0 != k1:
+        pytest.skip(f"grids do not share a checkpoint interval: {k0} vs {k1}")
+    out = imager.migrate_batch([shot, shot], [model, big])
+    assert imager.last_timing["launch_groups"] == 1.0
+    assert np.array_equal(out[0]["pp"].image, c["image"])
+    alone = RTMImager(imaging, solver, q).migrate(shot, big)
+    assert out[1]["pp"].image.shape == (nz + 1, nx + 2)
+    assert np.array_equal(out[1]["pp"].image, alone.image)
+
+
+def test_batch_nested_replay_and_pp_ps():
+    """Segmented replay inside a batch and fused pp+ps keep alone results."""
+    _say("batch nested pp+ps")
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case("el_pp_divcurl")
+    shot, model, imaging, solver, q = migrate_objects(c)
+    shots = [shot, _moved(shot, 4 * model.grid.dx)]
+    out = RTMImager(imaging, solver, q, GpuConfig(replay_slots=5)).migrate_batch(
+        shots, model, modes=("pp", "ps"))
+    assert np.array_equal(out[0]["pp"].image, c["image"])
+    assert np.array_equal(out[0]["ps"].image, migrate_case("el_ps_divcurl")["image"])
+    alone = RTMImager(imaging, solver, q).migrate_modes(shots[1], model, ("pp", "ps"))
+    for m in ("pp", "ps"):
+        assert np.array_equal(out[1][m].image, alone[m].image)
+
+
+def test_batch_max_batch_splits_groups():
+    _say("batch max_batch")
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case("ac_zero_lag")
+    shot, model, imaging, solver, q = migrate_objects(c)
+    shots = [_moved(shot, k * model.grid.dx) for k in range(3)]
+    imager = RTMImager(imaging, solver, q, GpuConfig(max_batch=2))
+    out = imager.migrate_batch(shots, model)
+    assert imager.last_timing["launch_groups"] == 2.0
+    ref = RTMImager(imaging, solver, q).migrate_batch(shots, model)
+    for a, b in zip(out, ref):
+        assert np.array_equal(a["pp"].image, b["pp"].image)

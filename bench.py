@@ -96,11 +96,7 @@ def model_records(model, sources, receivers, nt):
 
     solver = build_solver(SolverConfig(nt=nt, dt=DT, order=8, boundary="free", engine="elastic",
                                        npml=30), gpu_config=GpuConfig(strict_cfl=False))
-    shots = []
-    for s in sources:
-        rec = solver.forward(model, s, receivers, components=("vx", "vz"))
-        shots.append(rec)
-    return shots
+    return solver.forward_batch(model, sources, receivers, components=("vx", "vz"))
 
 
 def read_traffic():
@@ -151,6 +147,9 @@ def main(argv=None) -> int:
     ap.add_argument("--nt", type=int, default=NT)
     ap.add_argument("--cpu-sample-nt", type=int, default=240)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a plain float32 copy kernel (out = in * 1) over this many "
+                         "MiB, three times: the reference reading for HBM byte counters")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -166,6 +165,17 @@ def main(argv=None) -> int:
     from ertm import device as dev
     from ertm.config import GpuConfig, ImagingConfig, SolverConfig
     from ertm.imaging import RTMImager
+
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(f"# copy probe: {n * 4} bytes read and written per launch", file=sys.stderr,
+              flush=True)
+        del src, dst
 
     model = layered_model()
     sources, receivers = acquisition(args.shots)

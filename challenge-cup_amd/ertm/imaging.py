@@ -42,6 +42,8 @@ from .propagation import (
     PropagationSpec,
     Propagator2D,
     QOptions,
+    batch_after_run,
+    batch_run_kwargs,
     build_propagator,
     check_hard_stability,
     check_stability,
@@ -179,11 +181,11 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
         raise ImagingError("batched propagators must be at the same step")
     b = dev.Batch(cores)
     ws = [np.asarray(w, dtype=np.float32) for w in wavelets]
-    kw = _batch_kwargs(props)
+    kw = batch_run_kwargs(props)
     cur = b.run(0, 0, nt, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
                 save=arenas, save_stride=stride, step_index0=step0.pop(), **kw)
     b.close()
-    _after_batch(props, kw)
+    batch_after_run(props, kw)
     stores = []
     for p, c, arena, ps in zip(props, cores, arenas, per_state):
         c.cur = cur
@@ -199,26 +201,6 @@ def _common(values: set, what: str):
     if len(values) != 1:
         raise ImagingError(f"batched shots must share the {what}")
     return next(iter(values))
-
-
-def _batch_kwargs(props: list[Propagator2D]) -> dict:
-    """ertm_run options of a batch: the viscoacoustic stabiliser needs one
-    control record (4 doubles) per propagator in one contiguous buffer."""
-    kws = [p._run_kwargs() for p in props]
-    if not kws[0]:
-        return {}
-    import torch
-
-    kw = dict(kws[0])
-    if len(props) > 1:
-        kw["ctl"] = torch.cat([k["ctl"] for k in kws])
-    return kw
-
-
-def _after_batch(props: list[Propagator2D], kw: dict) -> None:
-    if len(props) > 1 and "ctl" in kw:
-        for k, p in enumerate(props):
-            p._run_kwargs()["ctl"].copy_(kw["ctl"][4 * k: 4 * k + 4])
 
 
 def replay_block(prop: Propagator2D, wavelet: np.ndarray, store: CheckpointStore,
@@ -490,7 +472,7 @@ class RTMImager:
         illum = condition == "normalized" or bool(self.config.receiver_normalization)
         fcores = [f._core for f in fwds]
         bcores = [bk._core for bk in backs]
-        bkw = _batch_kwargs(backs)
+        bkw = batch_run_kwargs(backs)
         back_steps = 0
         store0 = stores[0]
         nblocks = len(store0.states)
@@ -568,7 +550,7 @@ class RTMImager:
             fb.close()
             bb.close()
             accs = [(ib.num, ib.ill, ib.rill) for ib in ibs]
-        _after_batch(backs, bkw)
+        batch_after_run(backs, bkw)
         torch.cuda.synchronize()
         timing["replay_backward_s"] = time.perf_counter() - t1
         for c in fcores:

@@ -624,14 +624,92 @@ def run_forward(model: ElasticModel, source: SourceSpec, receivers: ReceiverSpec
     return prop.run_steps(np.asarray(wavelet, dtype=np.float32), tuple(names), record_every)
 
 
-def forward_shot(model: ElasticModel, source: SourceSpec, receivers: ReceiverSpec,
-                 spec: PropagationSpec, *, engine: str = "acoustic",
-                 components: tuple[str, ...] | None = None,
-                 q_options: QOptions | None = None) -> ShotRecord:
-    """Forward shot as a ShotRecord (ertm/propagation.py:1055-1093)."""
+def batch_run_kwargs(props: list[Propagator2D]) -> dict:
+    """ertm_run options of several propagators run together: the viscoacoustic
+    stabiliser needs one control record (4 doubles) per propagator in one
+    contiguous buffer."""
+    kws = [p._run_kwargs() for p in props]
+    if not kws[0]:
+        return {}
+    import torch
+
+    kw = dict(kws[0])
+    if len(props) > 1:
+        kw["ctl"] = torch.cat([k["ctl"] for k in kws])
+    return kw
+
+
+def batch_after_run(props: list[Propagator2D], kw: dict) -> None:
+    """Hand the batch's control records back to each propagator."""
+    if len(props) > 1 and "ctl" in kw:
+        for k, p in enumerate(props):
+            p._run_kwargs()["ctl"].copy_(kw["ctl"][4 * k: 4 * k + 4])
+
+
+def run_steps_batch(props: list[Propagator2D], wavelets: list[np.ndarray],
+                    components: tuple[str, ...]) -> list[dict[str, np.ndarray]]:
+    """``run_steps`` of several propagators advanced together (one launch per
+    step for all of them); each result equals its propagator's ``run_steps``."""
+    import torch
+
+    nts = {int(len(w)) for w in wavelets}
+    if len(nts) != 1 or len(props) != len(wavelets):
+        raise SolverError("batched propagators need one wavelet each, all of one length")
+    nt = nts.pop()
+    steps = {p.step_index for p in props}
+    curs = {p._core.cur for p in props}
+    if len(steps) != 1 or len(curs) != 1:
+        raise SolverError("batched propagators must be at the same step")
+    names = tuple(components)
+    tabs, recs_all, ws = [], [], []
+    for p, w in zip(props, wavelets):
+        nrec = p._receivers.nrec if p._receivers is not None else 0
+        if not nrec or not names:
+            raise SolverError("batched forward modelling needs receivers and components")
+        recs = [torch.zeros((nt, nrec), dtype=torch.float32, device="cuda") for _ in names]
+        tab = dev.RecvTables()
+        tab.build_sampling(p._core, p._receivers.x, p._receivers.z, names)
+        tab.set_records(recs)
+        tabs.append(tab)
+        recs_all.append(recs)
+        w = np.asarray(w, dtype=np.float32)
+        ws.append(w if p._source_index is not None else np.zeros_like(w))
+    b = dev.Batch([p._core for p in props], tabs)
+    kw = batch_run_kwargs(props)
+    cur = b.run(0, 0, nt, curs.pop(), wavelets=ws, do_sample=True, step_index0=steps.pop(), **kw)
+    b.close()
+    batch_after_run(props, kw)
+    out = []
+    for p, recs in zip(props, recs_all):
+        p._core.cur = cur
+        p.pml.touched = p.pml.touched or nt > 0
+        p.step_index += nt
+        out.append({n: r.cpu().numpy() for n, r in zip(names, recs)})
+    return out
+
+
+def forward_shots(model: ElasticModel | list[ElasticModel], sources: list[SourceSpec],
+                  receivers: ReceiverSpec | list[ReceiverSpec], spec: PropagationSpec, *,
+                  engine: str = "acoustic", components: tuple[str, ...] | None = None,
+                  q_options: QOptions | None = None) -> list[ShotRecord]:
+    """``forward_shot`` for many sources, advanced together; every record
+    equals the one ``forward_shot`` returns for that source alone."""
+    n = len(sources)
+    models = model if isinstance(model, list) else [model] * n
+    recvs = receivers if isinstance(receivers, list) else [receivers] * n
     names = components or (("p",) if engine in ("acoustic", "viscoacoustic") else ("vx", "vz", "sxz"))
-    data = run_forward(model, source, receivers, spec, engine=engine, components=names,
-                       q_options=q_options)
+    props = []
+    for m, s, r in zip(models, sources, recvs):
+        p = build_propagator(m, spec, engine=engine, q_options=q_options)
+        p.set_source(s)
+        p.set_receivers(r)
+        props.append(p)
+    data = run_steps_batch(props, [source_wavelet(s, spec.dt, spec.nt) for s in sources],
+                           tuple(names))
+    return [_shot_record(d, names, spec, engine, s, r) for d, s, r in zip(data, sources, recvs)]
+
+
+def _shot_record(data, names, spec, engine, source, receivers) -> ShotRecord:
     first = names[0]
     t = np.arange(data[first].shape[0], dtype=np.float64) * spec.dt
     meta: dict[str, Any] = {
@@ -645,3 +723,14 @@ def forward_shot(model: ElasticModel, source: SourceSpec, receivers: ReceiverSpe
                       components={n: data[n].astype(np.float32) for n in names})
     shot.validate()
     return shot
+
+
+def forward_shot(model: ElasticModel, source: SourceSpec, receivers: ReceiverSpec,
+                 spec: PropagationSpec, *, engine: str = "acoustic",
+                 components: tuple[str, ...] | None = None,
+                 q_options: QOptions | None = None) -> ShotRecord:
+    """Forward shot as a ShotRecord (ertm/propagation.py:1055-1093)."""
+    names = components or (("p",) if engine in ("acoustic", "viscoacoustic") else ("vx", "vz", "sxz"))
+    data = run_forward(model, source, receivers, spec, engine=engine, components=names,
+                       q_options=q_options)
+    return _shot_record(data, names, spec, engine, source, receivers)

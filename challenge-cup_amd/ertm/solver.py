@@ -18,6 +18,7 @@ from .propagation import (
     check_hard_stability,
     check_stability,
     forward_shot,
+    forward_shots,
     source_wavelet,
     stability_report,
 )
@@ -88,6 +89,36 @@ class FiniteDifferenceSolver:
             "npml": self.config.npml, "dt": spec.dt, "nt": spec.nt, "components": list(names),
         })
         return record
+
+    def forward_batch(self, model: ElasticModel | list[ElasticModel], sources: list[SourceSpec],
+                      receivers: ReceiverSpec | list[ReceiverSpec], *,
+                      components: tuple[str, ...] | None = None) -> list[ShotRecord]:
+        """``forward`` for many sources, advanced together with one launch per
+        step; every record equals what ``forward`` returns for that source."""
+        models = model if isinstance(model, list) else [model] * len(sources)
+        if len(models) != len(sources):
+            raise SolverError("one model per source expected")
+        specs = []
+        for m in models:
+            try:
+                m.validate()
+            except ValueError as exc:
+                raise SolverError(f"invalid forward model: {exc}") from exc
+            specs.append(self.spec(m))
+        if len({(s.nt, s.npml) for s in specs}) != 1:
+            raise SolverError("batched forward modelling needs one nt and npml")
+        names = components or self.components()
+        records = forward_shots(models, list(sources), receivers, specs[0],
+                                engine=self.config.engine, components=names,
+                                q_options=self.q_options())
+        for rec, spec in zip(records, specs):
+            rec.meta.update({
+                "placeholder": False, "solver": self.name, "engine": self.config.engine,
+                "order": self.config.order, "boundary": self.config.boundary,
+                "npml": self.config.npml, "dt": spec.dt, "nt": spec.nt,
+                "components": list(names),
+            })
+        return records
 
     def wavelet(self, source: SourceSpec, model: ElasticModel) -> np.ndarray:
         spec = self.spec(model)

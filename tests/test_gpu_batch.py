@@ -125,3 +125,34 @@ def test_batch_max_batch_splits_groups():
     ref = RTMImager(imaging, solver, q).migrate_batch(shots, model)
     for a, b in zip(out, ref):
         assert np.array_equal(a["pp"].image, b["pp"].image)
+
+
+@pytest.mark.parametrize("tag", ["ac_cpml", "el_free", "vi_fwd"])
+def test_forward_batch_equals_single_shots(tag):
+    _say("forward batch", tag)
+    from golden_cases import forward_case
+    from product_cases import model_of, receivers_of, source_of
+    from ertm.config import GpuConfig, QConfig, SolverConfig
+    from ertm.solver import build_solver
+
+    c = forward_case(tag)
+    model = model_of(c)
+    q = None
+    if c["engine"] == "viscoacoustic":
+        qq = c["q"]
+        q = QConfig(enabled=True, model="gsls", f_ref=qq["f_ref"], n_mechanisms=qq["n"],
+                    band_ratio=qq["band_ratio"], alpha_c=qq["alpha_c"],
+                    amplitude_cap=qq["amplitude_cap"], lowpass_interval=qq["lowpass_interval"],
+                    f_max=qq["f_max"], gain_budget_db=qq["gain_budget_db"])
+    solver = build_solver(SolverConfig(nt=c["nt"], dt=c["dt"], order=c["order"],
+                                       boundary=c["boundary"], engine=c["engine"],
+                                       npml=c["npml"]), q, GpuConfig(strict_cfl=False))
+    src = source_of(c)
+    rcv = receivers_of(c)
+    sources = [src] + [dataclasses.replace(src, x=float(src.x) + k * c["dx"]) for k in (1, -2)]
+    comps = tuple(str(k) for k in c["components"])
+    batch = solver.forward_batch(model, sources, rcv, components=comps)
+    for s, rec in zip(sources, batch):
+        alone = solver.forward(model, s, rcv, components=comps)
+        for name in comps:
+            assert np.array_equal(rec.components[name], alone.components[name])

@@ -246,12 +246,12 @@ int ertm_batch_destroy(ertm_batch_t b) {
 
 int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc) {
     if (!b) return fail(ERTM_EINVAL, "set_variant: NULL batch");
-    if (variant < 0 || variant > 3) return fail(ERTM_EINVAL, "unknown kernel variant %d", variant);
+    const int w = variant % 10, pd = variant / 10 + 1;
+    if (variant < 0 || (variant != 0 && (w < 1 || w > 3 || (pd != 1 && pd != 2 && pd != 4))))
+        return fail(ERTM_EINVAL, "unknown kernel variant %d", variant);
     if (variant != 0 && (!b->all_elastic || b->order != 8))
         return fail(ERTM_EINVAL, "fused variants need an all-elastic, order-8 batch");
     if (zc < 8 || zc > 4096) return fail(ERTM_EINVAL, "z chunk %d out of [8, 4096]", zc);
-    if (variant != 0 && zc % 9 != 1)
-        return fail(ERTM_EINVAL, "fused variants need a z chunk with zc %% 9 == 1, got %d", zc);
     b->variant = variant;
     b->zc = zc;
     return ERTM_OK;

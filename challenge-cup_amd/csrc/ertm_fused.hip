@@ -82,25 +82,40 @@ struct ElBlock {
 // between slabs return finite values that only feed lanes whose results are
 // discarded.  The launcher only selects this kernel when a state buffer is
 // smaller than 2 GiB.
-// Running per-lane byte offsets of one row iteration k (each advances by one
-// row per iteration): stress loads of row k+2K, velocity row m = k+K (old
-// loads of the next row are prefetched from here + one row), 1/rho row
-// m+1, mu row k+1, lambda / stress stores row k.
+// Running per-lane byte offsets (each advances by one row per iteration).
+// At iteration k the loads fetch the inputs of iteration k+PD: stress row
+// k+PD+2K, velocity row k+PD+K, 1/rho row k+PD+K+1, mu row k+PD+1, lambda
+// row k+PD; the stores write velocity row m = k+K and stress row k.
 struct ElOffs {
-    int sxx, szz, sxz, vx, vz, ir, mu, lam, txx, tzz, txz;
+    int sxx, szz, sxz, vx, vz, ir, mu, lam;  // loads
+    int svx, svz, txx, tzz, txz;             // stores
     __device__ __forceinline__ void advance(int d) {
         sxx += d; szz += d; sxz += d; vx += d; vz += d; ir += d; mu += d; lam += d;
-        txx += d; tzz += d; txz += d;
+        svx += d; svz += d; txx += d; tzz += d; txz += d;
     }
 };
 
-template <int K, int NT>
+// PD = prefetch distance in rows: the inputs of the next PD iterations are
+// in flight while a row is computed (more bytes in flight per wave).  PD
+// divides 2K.
+template <int K, int PD>
 struct ElState {
     float q_sxx[2 * K + 1], q_szz[2 * K + 1], q_sxz[2 * K + 1];  // rows k .. k+2K
     float q_vx[2 * K];                                          // rows k-K+1 .. k+K
     float q_vz[2 * K + 1];                                      // rows k-K .. k+K
     float ir_m, mu_k;                                           // own column, rows m, k
-    float n_sxx, n_szz, n_sxz, n_vx, n_vz, n_ir, n_mu, n_lam;   // prefetched inputs
+    float n_sxx[PD], n_szz[PD], n_sxz[PD], n_vx[PD], n_vz[PD];  // prefetched inputs of
+    float n_ir[PD], n_mu[PD], n_lam[PD];                        // iterations k .. k+PD-1
+    __device__ __forceinline__ void fetch(const ElBlock& B, const ElOffs& O, int p) {
+        n_sxx[p] = bload(B.rS, O.sxx);
+        n_szz[p] = bload(B.rS, O.szz);
+        n_sxz[p] = bload(B.rS, O.sxz);
+        n_vx[p] = bload(B.rS, O.vx);
+        n_vz[p] = bload(B.rS, O.vz);
+        n_ir[p] = bload(B.rIR, O.ir);
+        n_mu[p] = bload(B.rMU, O.mu);
+        n_lam[p] = bload(B.rLAM, O.lam);
+    }
 };
 
 // One row iteration.  STRESS = false: velocity only (the 2K prologue rows
@@ -109,36 +124,33 @@ struct ElState {
 // so the wait for the prefetched loads leaves earlier stores in flight
 // (vmcnt also counts stores on this architecture).  STRIP adds the CPML
 // memory reads/writes in per-lane branches.
-template <int K, int NT, bool STRIP, bool STRESS>
-__device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, NT>& X, ElOffs& O, int k,
+// The prefetched inputs form a ring of PD slots: iteration k reads slot
+// SLOT = k mod PD (rows loaded PD iterations earlier) and refills it with the
+// inputs of iteration k+PD.  SLOT is a compile-time constant (the row loop is
+// unrolled by PD), so no register moves touch loads still in flight.
+template <int K, int NT, int PD, int SLOT, bool STRIP, bool STRESS>
+__device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD>& X, ElOffs& O, int k,
                                         float* l_sxx, float* l_sxz, float* l_ir, float* l_vx,
                                         float* l_vz, float* l_mu0, float* l_mu1) {
     constexpr int QS = 2 * K + 1, QX = 2 * K, QZ = 2 * K + 1;
     const int tid = B.tid, ld4 = B.ld4, nzp = B.nzp;
     const int m = k + K;
-    const int om = O.vx, okk = O.lam;  // current-row offsets before advancing
-    const int om_z = O.vz, otxx = O.txx, otzz = O.tzz, otxz = O.txz;
-    const float vx_old = X.n_vx, vz_old = X.n_vz, ir_m1 = X.n_ir, mu_k1 = X.n_mu;
-    const float lam_k = X.n_lam;
+    const int om = O.svx, om_z = O.svz;  // this iteration's store offsets
+    const int otxx = O.txx, otzz = O.tzz, otxz = O.txz;
+    const float vx_old = X.n_vx[SLOT], vz_old = X.n_vz[SLOT], ir_m1 = X.n_ir[SLOT];
+    const float mu_k1 = X.n_mu[SLOT], lam_k = X.n_lam[SLOT];
 #pragma unroll
     for (int q = 0; q < QS - 1; ++q) {
         X.q_sxx[q] = X.q_sxx[q + 1];
         X.q_szz[q] = X.q_szz[q + 1];
         X.q_sxz[q] = X.q_sxz[q + 1];
     }
-    X.q_sxx[QS - 1] = X.n_sxx;
-    X.q_szz[QS - 1] = X.n_szz;
-    X.q_sxz[QS - 1] = X.n_sxz;
+    X.q_sxx[QS - 1] = X.n_sxx[SLOT];
+    X.q_szz[QS - 1] = X.n_szz[SLOT];
+    X.q_sxz[QS - 1] = X.n_sxz[SLOT];
+    // prefetch the inputs of iteration k+PD; in flight during the next PD rows
+    X.fetch(B, O, SLOT);
     O.advance(ld4);
-    // prefetch the next iteration's inputs; in flight during this row's compute
-    X.n_sxx = bload(B.rS, O.sxx);
-    X.n_szz = bload(B.rS, O.szz);
-    X.n_sxz = bload(B.rS, O.sxz);
-    X.n_vx = bload(B.rS, O.vx);
-    X.n_vz = bload(B.rS, O.vz);
-    X.n_ir = bload(B.rIR, O.ir);
-    X.n_mu = bload(B.rMU, O.mu);
-    X.n_lam = bload(B.rLAM, O.lam);
 
     // ---- phase A: x-neighbour rows for the velocity update of row m
     l_sxx[tid] = X.q_sxx[K];
@@ -292,9 +304,23 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, NT>& X, ElO
         bstore(B.rT, own_k ? otzz : ERTM_NOWRITE, szz);
         bstore(B.rT, own_k ? otxz : ERTM_NOWRITE, sxz);
     }
-    (void)okk;
     X.ir_m = ir_m1;
     X.mu_k = mu_k1;
+}
+
+// Iterations k .. k+cnt-1 (cnt <= PD, block-uniform) with ring slots 0..cnt-1.
+template <int K, int NT, int PD, bool STRIP, bool STRESS, int SLOT = 0>
+__device__ __forceinline__ void el_group(const ElBlock& B, ElState<K, PD>& X, ElOffs& O, int k,
+                                         int cnt, float* l_sxx, float* l_sxz, float* l_ir,
+                                         float* l_vx, float* l_vz, float* l_mu0, float* l_mu1) {
+    if constexpr (SLOT < PD) {
+        if (SLOT < cnt) {
+            el_iter<K, NT, PD, SLOT, STRIP, STRESS>(B, X, O, k + SLOT, l_sxx, l_sxz, l_ir, l_vx,
+                                                    l_vz, l_mu0, l_mu1);
+            el_group<K, NT, PD, STRIP, STRESS, SLOT + 1>(B, X, O, k, cnt, l_sxx, l_sxz, l_ir,
+                                                         l_vx, l_vz, l_mu0, l_mu1);
+        }
+    }
 }
 
 // One chunk: 2K prologue rows (velocity only), then zc rows (both updates).
@@ -304,12 +330,12 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, NT>& X, ElO
 // between slabs return finite values that only feed lanes whose results are
 // discarded.  The launcher only selects this kernel when a state buffer is
 // smaller than 2 GiB.
-template <int K, int NT, bool STRIP>
+template <int K, int NT, int PD, bool STRIP>
 __device__ __forceinline__ void el_loop(const ElBlock& B, float* l_sxx, float* l_sxz, float* l_ir,
                                         float* l_vx, float* l_vz, float* l_mu0, float* l_mu1) {
     constexpr int QX = 2 * K, QZ = 2 * K + 1;
     const int ld4 = B.ld4;
-    ElState<K, NT> X;
+    ElState<K, PD> X;
 #pragma unroll
     for (int q = 0; q < QX; ++q) X.q_vx[q] = 0.0f;
 #pragma unroll
@@ -325,7 +351,7 @@ __device__ __forceinline__ void el_loop(const ElBlock& B, float* l_sxx, float* l
     }
     X.ir_m = bload(B.rIR, o0 + K * ld4);
     X.mu_k = bload(B.rMU, o0);
-    ElOffs O;  // offsets of iteration kbeg
+    ElOffs O;  // load offsets of iteration kbeg, store offsets of iteration kbeg
     O.sxx = o0 + 2 * K * ld4 + B.fXX;
     O.szz = o0 + 2 * K * ld4 + B.fZZ;
     O.sxz = o0 + 2 * K * ld4 + B.fXZ;
@@ -334,34 +360,49 @@ __device__ __forceinline__ void el_loop(const ElBlock& B, float* l_sxx, float* l
     O.ir = o0 + (K + 1) * ld4;
     O.mu = o0 + ld4;
     O.lam = o0;
+    O.svx = O.vx;
+    O.svz = O.vz;
     O.txx = o0 + B.fXX;
     O.tzz = o0 + B.fZZ;
     O.txz = o0 + B.fXZ;
-    X.n_sxx = bload(B.rS, O.sxx);
-    X.n_szz = bload(B.rS, O.szz);
-    X.n_sxz = bload(B.rS, O.sxz);
-    X.n_vx = bload(B.rS, O.vx);
-    X.n_vz = bload(B.rS, O.vz);
-    X.n_ir = bload(B.rIR, O.ir);
-    X.n_mu = bload(B.rMU, O.mu);
-    X.n_lam = bload(B.rLAM, O.lam);
+    // inputs of iterations kbeg .. kbeg+PD-1; the load offsets then point at
+    // iteration kbeg+PD (the store offsets stay at kbeg)
+#pragma unroll
+    for (int p = 0; p < PD; ++p) {
+        X.fetch(B, O, p);
+        O.sxx += ld4; O.szz += ld4; O.sxz += ld4; O.vx += ld4; O.vz += ld4;
+        O.ir += ld4; O.mu += ld4; O.lam += ld4;
+    }
+    // 2K prologue iterations (a whole number of PD-groups: PD divides 2K),
+    // then the chunk's rows in PD-groups and a remainder of < PD iterations,
+    // which continues the slot ring from slot 0.
     int k = kbeg;
-    for (; k < B.z0; ++k)
-        el_iter<K, NT, STRIP, false>(B, X, O, k, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
-    for (; k < B.z1; ++k)
-        el_iter<K, NT, STRIP, true>(B, X, O, k, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+    for (; k < B.z0; k += PD)
+        el_group<K, NT, PD, STRIP, false>(B, X, O, k, PD, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0,
+                                          l_mu1);
+    for (; k + PD <= B.z1; k += PD)
+        el_group<K, NT, PD, STRIP, true>(B, X, O, k, PD, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0,
+                                         l_mu1);
+    if (k < B.z1)
+        el_group<K, NT, PD, STRIP, true>(B, X, O, k, B.z1 - k, l_sxx, l_sxz, l_ir, l_vx, l_vz,
+                                         l_mu0, l_mu1);
 }
 
-template <int K, int NT>
+template <int K, int NT, int PD>
 __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict__ props,
                                                       LaunchArgs la, int zc) {
     constexpr int TX = NT - 4 * K;
-    const int slot = la.base + blockIdx.y;
+    // The hardware deals workgroups to the XCDs by linear id (x fastest), so
+    // the remap works on the linear id of the whole (blocks x propagators)
+    // grid: each XCD gets a contiguous run of one propagator's blocks.
+    const int lin = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int ys = lin / gridDim.x;
+    const int bid = lin - ys * gridDim.x;
+    const int slot = la.base + ys;
     const ErtmProp& P = props[slot];
     const int nzp = P.nzp, nxp = P.nxp;
     const int nbands = (nxp + TX - 1) / TX;
     const int nchunks = (nzp + zc - 1) / zc;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
     if (bid >= nbands * nchunks) return;
     const int band = bid / nchunks, chunk = bid - band * nchunks;
     ElBlock B;
@@ -414,7 +455,7 @@ __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict
     B.rhz = P.rdz;
 #pragma unroll
     for (int q = 0; q < 4; ++q) B.cf[q] = P.coef[q];
-    B.amp = la.amp[blockIdx.y];
+    B.amp = la.amp[ys];
     B.src_col = B.amp != 0.0f && B.c == P.src_ix;
     B.inx = B.c >= K && B.c < nxp - K;
     B.jx4 = -1;
@@ -433,9 +474,9 @@ __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict
     const bool strip = (B.wx > 0 && (cx0 < B.wx || cx1 > nxp - B.wx)) ||
                        (B.wzt > 0 && rz0 < B.wzt) || (B.wzb > 0 && rz1 > nzp - B.wzb);
     if (strip)
-        el_loop<K, NT, true>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+        el_loop<K, NT, PD, true>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
     else
-        el_loop<K, NT, false>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+        el_loop<K, NT, PD, false>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
 }
 
 template <int K, int NT>
@@ -444,25 +485,37 @@ static int blocks_for(const ErtmProp& P, int zc) {
     return ((P.nxp + TX - 1) / TX) * ((P.nzp + zc - 1) / zc);
 }
 
-template <int K, int NT>
+template <int K, int NT, int PD>
 static void launch_fused_k(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
                            int zc, hipStream_t s) {
     int nb = 0;
     for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la.base + k], zc));
-    hipLaunchKernelGGL((k_elastic_fused<K, NT>), dim3(nb, la.n), dim3(NT), 0, s, dprops, la, zc);
+    hipLaunchKernelGGL((k_elastic_fused<K, NT, PD>), dim3(nb, la.n), dim3(NT), 0, s, dprops, la,
+                       zc);
 }
 
-// variant: 1 -> 256 threads, 2 -> 128 threads, 3 -> 64 threads (z chunk zc rows,
-// zc % 9 == 1 so that zc + 8 rows form whole unrolled trips)
+// variant = 10 * (PD - 1) + w: w = 1, 2, 3 -> 256, 128, 64 threads per block;
+// PD = 1, 2, 4 rows of prefetch (PD divides the 2K = 8 prologue rows).
+// zc = rows per chunk.
 bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
                        int order, int variant, int zc, hipStream_t s) {
-    if (order != 8 || zc % 9 != 1) return false;
-    for (int k = 0; k < la.n; ++k)
-        if (4 * hprops[la.base + k].state_floats >= ((int64_t)1 << 31)) return false;
+    if (order != 8 || zc < 8) return false;
+    // every byte offset, including prefetches up to 32 rows past the grid,
+    // must fit an int
+    for (int k = 0; k < la.n; ++k) {
+        const ErtmProp& P = hprops[la.base + k];
+        if (4 * P.state_floats + 128 * (int64_t)P.ld >= ((int64_t)1 << 31)) return false;
+    }
     switch (variant) {
-        case 1: launch_fused_k<4, 256>(dprops, hprops, la, zc, s); return true;
-        case 2: launch_fused_k<4, 128>(dprops, hprops, la, zc, s); return true;
-        case 3: launch_fused_k<4, 64>(dprops, hprops, la, zc, s); return true;
+        case 1: launch_fused_k<4, 256, 1>(dprops, hprops, la, zc, s); return true;
+        case 2: launch_fused_k<4, 128, 1>(dprops, hprops, la, zc, s); return true;
+        case 3: launch_fused_k<4, 64, 1>(dprops, hprops, la, zc, s); return true;
+        case 11: launch_fused_k<4, 256, 2>(dprops, hprops, la, zc, s); return true;
+        case 12: launch_fused_k<4, 128, 2>(dprops, hprops, la, zc, s); return true;
+        case 13: launch_fused_k<4, 64, 2>(dprops, hprops, la, zc, s); return true;
+        case 31: launch_fused_k<4, 256, 4>(dprops, hprops, la, zc, s); return true;
+        case 32: launch_fused_k<4, 128, 4>(dprops, hprops, la, zc, s); return true;
+        case 33: launch_fused_k<4, 64, 4>(dprops, hprops, la, zc, s); return true;
         default: return false;
     }
 }

@@ -137,13 +137,13 @@ __global__ void k_image(const ErtmProp* __restrict__ props, const ErtmImage* __r
         case ERTM_IMG_CORRELATE: {
             const float sv = I.rbuf[(int64_t)(t - I.t0) * ncell + q];
             const bool cross = t >= I.start_step;
-            if (I.illum) {
+            if (I.illum & ERTM_ILLUM_SRC) {
                 float ss = sv * sv;
                 I.ill[q] = I.ill[q] + (double)ss;
             }
             for (int w = 0; w < I.nrw; ++w) {
                 const float rv = wave_value<K>(P, S, I.separation, I.rwave[w], o);
-                if (I.illum) {
+                if (I.illum & ERTM_ILLUM_REC) {
                     float rr = rv * rv;
                     I.rill[w][q] = I.rill[w][q] + (double)rr;
                 }
@@ -168,8 +168,10 @@ __global__ void k_image(const ErtmProp* __restrict__ props, const ErtmImage* __r
             if (t >= I.start_step && I.peak_step[q] == t) {
                 const float rv = wave_value<K>(P, S, I.separation, I.rwave[0], o);
                 I.num[0][q] = I.num[0][q] + I.peak_amp[q] * (double)rv;
-                float rr = rv * rv;
-                I.rill[0][q] = I.rill[0][q] + (double)rr;
+                if (I.illum & ERTM_ILLUM_REC) {
+                    float rr = rv * rv;
+                    I.rill[0][q] = I.rill[0][q] + (double)rr;
+                }
             }
             break;
         }

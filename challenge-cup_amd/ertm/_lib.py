@@ -25,6 +25,7 @@ SRC_EXPLOSIVE, SRC_SHEAR, SRC_FORCE_Z = 0, 1, 2
 WAVE_P, WAVE_S = 0, 1
 SEP_STRESS, SEP_DIVCURL = 0, 1
 IMG_EXTRACT, IMG_CORRELATE, IMG_EXC_PEAK, IMG_EXC_APPLY = 0, 1, 2, 3
+ILLUM_SRC, ILLUM_REC = 1, 2
 
 ENGINE_CODE = {"acoustic": ACOUSTIC, "elastic": ELASTIC, "viscoacoustic": VISCO}
 MECH_CODE = {"explosive": SRC_EXPLOSIVE, "shear": SRC_SHEAR, "force_z": SRC_FORCE_Z}

@@ -384,7 +384,7 @@ class ImageBuffers:
     """Accumulators and replay slots behind one ErtmImage."""
 
     def __init__(self, nz: int, nx: int, separation: str, swave: str, rwaves, start_step: int,
-                 illum: bool, excitation: bool = False, rbuf=None, nslots: int = 0):
+                 illum: int, excitation: bool = False, rbuf=None, nslots: int = 0):
         torch = torch_mod()
         self.s = L.ErtmImage()
         s = self.s
@@ -394,7 +394,7 @@ class ImageBuffers:
         for k, w in enumerate(rwaves):
             s.rwave[k] = L.WAVE_S if w == "S" else L.WAVE_P
         s.start_step = int(start_step)
-        s.illum = int(bool(illum))
+        s.illum = int(illum)  # ILLUM_SRC | ILLUM_REC bits
         z = lambda dt: torch.zeros((nz, nx), dtype=dt, device="cuda")  # noqa: E731
         self.num = [z(torch.float64) for _ in rwaves]
         self.rill = [z(torch.float64) for _ in rwaves]

@@ -27,7 +27,9 @@ replay budget, the block is replayed in segments from sub-checkpoints
 from __future__ import annotations
 
 import math
+import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Any, Callable
 
@@ -195,6 +197,10 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
                                       interval=interval, bytes_per_state=ps, arena=arena,
                                       state_floats=stride))
     return stores
+
+
+def _host_workers(n_items: int) -> int:
+    return max(1, min(n_items, 16, os.cpu_count() or 1))
 
 
 def _common(values: set, what: str):
@@ -385,18 +391,23 @@ class RTMImager:
                     timing[key] += self.last_timing.get(key, 0.0)
                 timing["launch_groups"] += 1.0
                 t_post = time.perf_counter()
-                for j, i in enumerate(idx):
-                    spec = job.specs[i]
-                    out = {}
-                    for k, m in enumerate(job.modes):
-                        image = self._postprocess(imgs[j][k], job.models[i], spec)
-                        res = ImageResult(image=np.asarray(image, dtype=np.float32),
-                                          grid=job.models[i].grid, mode=m,
-                                          meta=self._meta(spec, engine, job.shots[i], infos[j],
-                                                          condition, m))
-                        res.validate()
-                        out[m] = res
-                    results[i] = out
+                # normalisation and post-processing per image, in host threads
+                # (NumPy releases the GIL in these array operations; every
+                # image is computed exactly as in a serial loop)
+                work = [(j, i, k) for j, i in enumerate(idx) for k in range(len(job.modes))]
+                with ThreadPoolExecutor(max_workers=_host_workers(len(work))) as pool:
+                    images = list(pool.map(
+                        lambda w: self._finish_image(imgs[w[0]], w[2], condition,
+                                                     job.models[w[1]], job.specs[w[1]]), work))
+                for (j, i, k), image in zip(work, images):
+                    m = job.modes[k]
+                    res = ImageResult(image=image, grid=job.models[i].grid, mode=m,
+                                      meta=self._meta(job.specs[i], engine, job.shots[i],
+                                                      infos[j], condition, m))
+                    res.validate()
+                    if results[i] is None:
+                        results[i] = {}
+                    results[i][m] = res
                 timing["postprocess_s"] += time.perf_counter() - t_post
         timing["run_s"] = time.perf_counter() - t_start
         self.last_timing = timing
@@ -469,7 +480,10 @@ class RTMImager:
             st = int(np.floor((float(shot.source.delay) + float(self.config.start_periods)
                                / max(float(shot.source.f0), 1e-6)) / spec.dt))
             starts.append(int(np.clip(st, 0, nt)))
-        illum = condition == "normalized" or bool(self.config.receiver_normalization)
+        # accumulate only the illuminations the normalisation reads
+        rec_illum = L.ILLUM_REC if self.config.receiver_normalization else 0
+        illum = rec_illum | (L.ILLUM_SRC if condition == "normalized" and self.config.illumination
+                             else 0)
         fcores = [f._core for f in fwds]
         bcores = [bk._core for bk in backs]
         bkw = batch_run_kwargs(backs)
@@ -487,7 +501,7 @@ class RTMImager:
         t1 = time.perf_counter()
         if condition == "excitation":
             ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, "P", rwaves[:1], st,
-                                    False, excitation=True) for m, st in zip(models, starts)]
+                                    rec_illum, excitation=True) for m, st in zip(models, starts)]
             fb = dev.Batch(fcores, imgs=ibs)
             for blk in range(nblocks):
                 s0, s1 = store0.bounds(blk, nt)
@@ -512,7 +526,7 @@ class RTMImager:
             ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, rwaves, st, illum,
                                     rbuf=rb, nslots=slots)
                    for m, st, rb in zip(models, starts, rbufs)]
-            fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, False,
+            fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, 0,
                                       rbuf=rb, nslots=slots) for m, rb in zip(models, rbufs)]
             fb = dev.Batch(fcores, imgs=fimgs)
             bb = dev.Batch(bcores, recvs=tabs, imgs=ibs)
@@ -557,23 +571,18 @@ class RTMImager:
             c.cur = fcur
         for c in bcores:
             c.cur = bcur
-        eps = float(self.config.eps)
+        use_ill = condition == "normalized" and bool(self.config.illumination)
+        use_rill = bool(self.config.receiver_normalization)
         all_outs, infos = [], []
         for j in range(n):
             num_t, ill, rill = accs[j]
             backs[j].step_index = back_steps
             backs[j].pml.touched = True
-            outs = []
-            ill_h = ill.cpu().numpy()
-            for k in range(len(rwaves)):
-                image = num_t[k].cpu().numpy()
-                if condition == "normalized" and self.config.illumination:
-                    image = image / (ill_h + eps)
-                if self.config.receiver_normalization:
-                    image = image / (rill[k].cpu().numpy() + eps)
-                outs.append(image)
+            # raw float64 accumulators on the host; normalised by _finish_image
+            all_outs.append(([x.cpu().numpy() for x in num_t],
+                             ill.cpu().numpy() if use_ill else None,
+                             [r.cpu().numpy() for r in rill] if use_rill else None))
             nz, nx = models[j].shape
-            all_outs.append(outs)
             infos.append(dict(store=stores[j], start_step=starts[j], prop=fwds[j], back=backs[j],
                               replayed_steps=replayed, nt=nt,
                               cell_updates=(nt + replayed + back_steps) * nz * nx))
@@ -636,6 +645,19 @@ class RTMImager:
         padded = np.zeros((nt, data.shape[1]), dtype=data.dtype)
         padded[: data.shape[0]] = data
         return padded
+
+    def _finish_image(self, raw, k: int, condition: str, model: ElasticModel,
+                      spec: PropagationSpec) -> np.ndarray:
+        """Normalise receiver wave k's accumulator and post-process it
+        (ertm/imaging.py:326-349)."""
+        nums, ill, rill = raw
+        eps = float(self.config.eps)
+        image = nums[k]
+        if condition == "normalized" and self.config.illumination:
+            image = image / (ill + eps)
+        if self.config.receiver_normalization:
+            image = image / (rill[k] + eps)
+        return np.asarray(self._postprocess(image, model, spec), dtype=np.float32)
 
     def _postprocess(self, image: np.ndarray, model: ElasticModel, spec: PropagationSpec):
         """Laplacian, wavenumber low-pass, PML mask (ertm/imaging.py:476-490)."""

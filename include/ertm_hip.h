@@ -175,13 +175,20 @@ enum {
  *   ill       source illumination (float64); rill[2] receiver illumination
  *   peak_step int32, peak_amp / running float64 (excitation)
  */
+/* ErtmImage.illum bits.  The reference accumulates both illuminations
+ * whenever either is used (ertm/imaging.py:300-306); each is only read by its
+ * own normalisation (:326-331), so only the ones used are accumulated. */
+#define ERTM_ILLUM_SRC 1
+#define ERTM_ILLUM_REC 2
+
 typedef struct ErtmImage {
     int32_t separation;    /* ERTM_SEP_* (elastic) */
     int32_t swave;         /* source-side wave (ERTM_WAVE_*) */
     int32_t nrw;           /* receiver waves accumulated (1 or 2) */
     int32_t rwave[2];
     int32_t start_step;    /* cross term only for t >= start_step */
-    int32_t illum;         /* 1: accumulate ill and rill every step */
+    int32_t illum;         /* ERTM_ILLUM_* bits: accumulate ill (source) and/or
+                              rill (receiver) illumination every step */
     int32_t t0;            /* time index of replay slot 0 */
     int32_t nslots;        /* K */
     int32_t _pad0;
@@ -213,9 +220,11 @@ int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmIma
 int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img);
 int ertm_batch_destroy(ertm_batch_t b);
 /* Stencil kernel variant: 0 = two launches per step (velocity, then
- * pressure/stress; every engine and order); 1, 2, 3 = single fused launch per
- * step with 256, 128, 64 threads per block streaming chunks of zc rows,
- * zc % 9 == 1 (elastic, order 8).  Results are bit-identical across variants. */
+ * pressure/stress; every engine and order); 10 * (PD - 1) + w with w = 1, 2, 3
+ * and PD = 1, 2, 4 = single fused launch per step (elastic, order 8) with
+ * 256, 128, 64 threads per block streaming chunks of zc rows (8 <= zc <=
+ * 4096) and the inputs of PD rows ahead in flight.  Results are
+ * bit-identical across variants. */
 int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc);
 
 /*

@@ -229,7 +229,9 @@ def test_acoustic_c1_shape_forward_matches_oracle():
     assert np.array_equal(p.p, op.p)
 
 
-@pytest.mark.parametrize("variant,zc", [(1, 64), (1, 10), (2, 19), (3, 10), (3, 64), (2, 4096)])
+@pytest.mark.parametrize("variant,zc", [(1, 64), (1, 10), (2, 19), (3, 10), (3, 64), (2, 4096),
+                                        (12, 19), (11, 37), (13, 8), (32, 24), (31, 9),
+                                        (33, 4096)])
 @pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force"])
 def test_fused_elastic_variants_bit_identical(tag, variant, zc):
     """Every fused launch shape reproduces the two-launch kernels (and thus the
@@ -274,7 +276,7 @@ def test_fused_variant_large_random_model_matches_two_launch():
     src = SourceSpec(x=3000.0, z=1500.0, f0=30.0, delay=0.01)
     w = np.sin(np.arange(60) * 0.3).astype(np.float32) * 1e6
     outs = []
-    for v in [(0, 64), (1, 64), (1, 10), (2, 28), (3, 127)]:
+    for v in [(0, 64), (1, 64), (1, 10), (2, 28), (3, 127), (2, 19), (12, 19), (31, 33)]:
         p = build_propagator(m, spec, engine="elastic")
         p.set_source(src)
         p._batches["step"] = dev.Batch([p._core], variant=v)

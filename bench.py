@@ -147,6 +147,9 @@ def main(argv=None) -> int:
     ap.add_argument("--nt", type=int, default=NT)
     ap.add_argument("--cpu-sample-nt", type=int, default=240)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-shots", type=int, default=4,
+                    help="also time this many shots migrated one at a time (the plain per-shot "
+                         "path) for the batched/plain ratio; 0 = skip")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain float32 copy kernel (out = in * 1) over this many "
                          "MiB, three times: the reference reading for HBM byte counters")
@@ -257,6 +260,22 @@ def main(argv=None) -> int:
         },
         "cpu_baseline": None,
     }
+    if args.plain_shots > 0:
+        # plain per-shot path: the same shots, one launch group per shot
+        plain = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False,
+                                                                replay_budget_gb=64.0,
+                                                                max_batch=1))
+        pjob = plain.prepare_batch(shots[:args.plain_shots], model, modes=("pp", "ps"))
+        plain.run_batch(plain.prepare_batch(shots[:1], model, modes=("pp", "ps")))  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain.run_batch(pjob)
+        torch.cuda.synchronize()
+        plain_s = args.plain_shots / (time.perf_counter() - t0)
+        line["plain_per_shot"] = {"value": round(plain_s, 4), "unit": "shots/s",
+                                  "shots": args.plain_shots}
+        line["batched_over_plain"] = round(shots_s / plain_s, 3)
+        print(f"# plain per-shot path: {plain_s:.3f} shots/s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print(f"# cpu baseline: oracle, nt={args.cpu_sample_nt}", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(model, sources, receivers, args.cpu_sample_nt)

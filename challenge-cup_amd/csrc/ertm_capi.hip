@@ -52,6 +52,41 @@ struct FftCache {
     void* buf = nullptr;
 };
 
+// Descriptor tables come from a process-wide pool of device blocks that are
+// never freed: hipFree waits for the whole device, which would couple batches
+// running on different streams.  Blocks are reused by size class (powers of
+// two from 4 KiB).
+std::mutex g_pool_m;
+std::multimap<size_t, void*> g_pool;
+
+size_t pool_class(size_t bytes) {
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+void* pool_get(size_t bytes) {
+    const size_t c = pool_class(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_pool_m);
+        auto it = g_pool.find(c);
+        if (it != g_pool.end()) {
+            void* p = it->second;
+            g_pool.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, c) != hipSuccess) return nullptr;
+    return p;
+}
+
+void pool_put(void* p, size_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_m);
+    g_pool.emplace(pool_class(bytes), p);
+}
+
 }  // namespace
 
 struct ertm_batch_s {
@@ -69,12 +104,17 @@ struct ertm_batch_s {
     int variant = 0;
     int zc = 64;
     float* scratch = nullptr;  // per propagator: peak, scale
+    void* block = nullptr;     // pool block holding dprops, drecv, dimg, scratch
+    size_t block_bytes = 0;
+    std::vector<hipStream_t> streams;  // streams this batch has enqueued work on
     std::map<std::pair<int, int>, FftCache> fft;
+    void note(hipStream_t s) {
+        for (hipStream_t t : streams)
+            if (t == s) return;
+        streams.push_back(s);
+    }
     ~ertm_batch_s() {
-        if (dprops) (void)hipFree(dprops);
-        if (drecv) (void)hipFree(drecv);
-        if (dimg) (void)hipFree(dimg);
-        if (scratch) (void)hipFree(scratch);
+        pool_put(block, block_bytes);
         for (auto& kv : fft) {
             if (kv.second.plan) hipfftDestroy(kv.second.plan);
             if (kv.second.buf) (void)hipFree(kv.second.buf);
@@ -209,14 +249,20 @@ int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmIma
         if (P.engine != ERTM_VISCO) b->all_visco = false;
         if (P.engine != ERTM_ELASTIC) b->all_elastic = false;
     }
-    hipError_t e1 = hipMalloc((void**)&b->dprops, sizeof(ErtmProp) * n);
-    hipError_t e2 = hipMalloc((void**)&b->drecv, sizeof(ErtmRecv) * n);
-    hipError_t e3 = hipMalloc((void**)&b->dimg, sizeof(ErtmImage) * n);
-    hipError_t e4 = hipMalloc((void**)&b->scratch, sizeof(float) * 2 * n + 64);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t sp = up(sizeof(ErtmProp) * n), sr = up(sizeof(ErtmRecv) * n);
+    const size_t si = up(sizeof(ErtmImage) * n), ss = up(sizeof(float) * 2 * n + 64);
+    b->block_bytes = sp + sr + si + ss;
+    b->block = pool_get(b->block_bytes);
+    if (!b->block) {
         delete b;
         return fail(ERTM_ENOMEM, "batch_create: device allocation failed");
     }
+    char* base = (char*)b->block;
+    b->dprops = (ErtmProp*)base;
+    b->drecv = (ErtmRecv*)(base + sp);
+    b->dimg = (ErtmImage*)(base + sp + sr);
+    b->scratch = (float*)(base + sp + sr + si);
     hipError_t c1 = hipMemcpy(b->dprops, b->props.data(), sizeof(ErtmProp) * n, hipMemcpyHostToDevice);
     hipError_t c2 = hipMemcpy(b->drecv, b->recv.data(), sizeof(ErtmRecv) * n, hipMemcpyHostToDevice);
     hipError_t c3 = hipMemcpy(b->dimg, b->img.data(), sizeof(ErtmImage) * n, hipMemcpyHostToDevice);
@@ -228,18 +274,22 @@ int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmIma
     return ERTM_OK;
 }
 
-int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img) {
+int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img, void* stream) {
     if (!b || !img) return fail(ERTM_EINVAL, "batch_update_image: bad arguments");
     const int n = (int)b->props.size();
-    HIPCHK(hipDeviceSynchronize());
+    hipStream_t s = (hipStream_t)stream;
+    // the old table may be read by work already enqueued on the batch's streams
+    for (hipStream_t t : b->streams) HIPCHK(hipStreamSynchronize(t));
     b->img.assign(img, img + n);
-    HIPCHK(hipMemcpy(b->dimg, b->img.data(), sizeof(ErtmImage) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(b->dimg, b->img.data(), sizeof(ErtmImage) * n, hipMemcpyHostToDevice,
+                          s));
+    HIPCHK(hipStreamSynchronize(s));
     return ERTM_OK;
 }
 
 int ertm_batch_destroy(ertm_batch_t b) {
     if (!b) return ERTM_OK;
-    (void)hipDeviceSynchronize();
+    for (hipStream_t t : b->streams) (void)hipStreamSynchronize(t);
     delete b;
     return ERTM_OK;
 }
@@ -456,28 +506,33 @@ int ertm_step_timing_read(double* total_ms, int64_t* launches, int64_t* cell_upd
 
 int ertm_step(ertm_batch_t b, int32_t cur, const float* amp, void* stream) {
     if (!b || (cur & ~1)) return fail(ERTM_EINVAL, "step: bad arguments");
+    b->note((hipStream_t)stream);
     return do_step(b, cur, amp, (hipStream_t)stream);
 }
 
 int ertm_stabilize(ertm_batch_t b, int32_t cur, int32_t step_index, double cap, int32_t interval,
                    double* ctl, void* stream) {
     if (!b || (cur & ~1)) return fail(ERTM_EINVAL, "stabilize: bad arguments");
+    b->note((hipStream_t)stream);
     return do_stabilize(b, cur, step_index, cap, interval, ctl, (hipStream_t)stream);
 }
 
 int ertm_sample(ertm_batch_t b, int32_t cur, const int32_t* t, void* stream) {
     if (!b || !t || (cur & ~1)) return fail(ERTM_EINVAL, "sample: bad arguments");
+    b->note((hipStream_t)stream);
     return do_sample(b, cur, t, (hipStream_t)stream);
 }
 
 int ertm_inject(ertm_batch_t b, int32_t cur, const int32_t* t, void* stream) {
     if (!b || !t || (cur & ~1)) return fail(ERTM_EINVAL, "inject: bad arguments");
+    b->note((hipStream_t)stream);
     return do_inject(b, cur, t, (hipStream_t)stream);
 }
 
 int ertm_image(ertm_batch_t b, int32_t cur, const int32_t* t, int32_t op, void* stream) {
     if (!b || !t || (cur & ~1) || op < ERTM_IMG_EXTRACT || op > ERTM_IMG_EXC_APPLY)
         return fail(ERTM_EINVAL, "image: bad arguments");
+    b->note((hipStream_t)stream);
     return do_image(b, cur, t, op, (hipStream_t)stream);
 }
 
@@ -491,6 +546,7 @@ int ertm_copy_state(float* dst, const float* src, int64_t nfloats, void* stream)
 int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
     if (!b || !r) return fail(ERTM_EINVAL, "run: bad arguments");
     hipStream_t s = (hipStream_t)stream;
+    b->note(s);
     const int n = (int)b->props.size();
     if (r->t_begin < 0 || r->t_end < r->t_begin) return fail(ERTM_EINVAL, "run: bad time range");
     if (r->phase < 0 || r->phase > 2) return fail(ERTM_EINVAL, "run: bad phase %d", r->phase);

@@ -16,7 +16,7 @@ from .exceptions import ERTMError, SolverError
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libertm_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_LAUNCH = 64
 MAX_MECH = 8
 
@@ -100,7 +100,7 @@ EXPORTS = [
     ("ertm_device_info", C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]),
     ("ertm_batch_create", C.c_int, [C.POINTER(ErtmProp), C.POINTER(ErtmRecv), C.POINTER(ErtmImage),
                                     C.c_int32, C.POINTER(C.c_void_p)]),
-    ("ertm_batch_update_image", C.c_int, [C.c_void_p, C.POINTER(ErtmImage)]),
+    ("ertm_batch_update_image", C.c_int, [C.c_void_p, C.POINTER(ErtmImage), C.c_void_p]),
     ("ertm_batch_destroy", C.c_int, [C.c_void_p]),
     ("ertm_batch_set_variant", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     ("ertm_step", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.c_void_p]),

@@ -453,7 +453,8 @@ class Batch:
     def update_images(self, imgs: list):
         self.imgs = imgs
         self._img = (L.ErtmImage * self.n)(*[i.s for i in imgs])
-        L.check(L.lib().ertm_batch_update_image(self.h, self._img), "ertm_batch_update_image")
+        L.check(L.lib().ertm_batch_update_image(self.h, self._img, L.stream_handle()),
+                "ertm_batch_update_image")
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:

@@ -496,7 +496,7 @@ class RTMImager:
         fcur = _common({c.cur for c in fcores}, "state parity")
         bcur = _common({c.cur for c in bcores}, "state parity")
         replayed = 0
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()
         timing["forward_s"] = time.perf_counter() - t0
         t1 = time.perf_counter()
         if condition == "excitation":
@@ -565,7 +565,7 @@ class RTMImager:
             bb.close()
             accs = [(ib.num, ib.ill, ib.rill) for ib in ibs]
         batch_after_run(backs, bkw)
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()
         timing["replay_backward_s"] = time.perf_counter() - t1
         for c in fcores:
             c.cur = fcur

@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define ERTM_ABI_VERSION 1
+#define ERTM_ABI_VERSION 2
 
 enum { ERTM_OK = 0, ERTM_EINVAL = 1, ERTM_EHIP = 2, ERTM_ENOMEM = 3, ERTM_ESTATE = 4 };
 
@@ -217,7 +217,10 @@ int ertm_device_info(int32_t* device, int32_t* n_cu, char* name, size_t name_len
  * entry may have nsamp=ncells=0 / all-NULL buffers). */
 int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmImage* img,
                       int32_t n, ertm_batch_t* out);
-int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img);
+/* Replace the imaging descriptors.  Waits for the work already enqueued with
+ * this batch (on any stream), then uploads on `stream` and waits for it. */
+int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img, void* stream);
+/* Waits for the work enqueued with this batch, then releases it. */
 int ertm_batch_destroy(ertm_batch_t b);
 /* Stencil kernel variant: 0 = two launches per step (velocity, then
  * pressure/stress; every engine and order); 10 * (PD - 1) + w with w = 1, 2, 3

@@ -1,0 +1,75 @@
+"""Serving-loop host logic without a GPU: admission order, HBM budget,
+same-key batching, and the request price."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from ertm.config import GpuConfig, ImagingConfig, SolverConfig
+from ertm.serving import Admission, ShotRequest, request_bytes
+from ertm.types import ElasticModel, GridSpec, ReceiverSpec, ShotRecord, SourceSpec
+
+
+def _item(name, nbytes, key):
+    return (name, None, nbytes, key, 0.0, None)
+
+
+def test_admission_is_first_come_first_served_within_budget():
+    adm = Admission(budget=100, streams=4, max_group=8)
+    for name, nb, key in [("a", 60, 1), ("b", 50, 2), ("c", 30, 1), ("d", 10, 3)]:
+        adm.add(_item(name, nb, key))
+    g1 = adm.take()
+    # "a" starts with its same-key companion "c" (60 + 30 <= 100)
+    assert [i[0] for i in g1] == ["a", "c"]
+    # "b" (50) does not fit next to 90 reserved; "d" must not overtake it
+    assert adm.take() is None
+    adm.release(g1)
+    g2 = adm.take()
+    assert [i[0] for i in g2] == ["b"]
+    g3 = adm.take()
+    assert [i[0] for i in g3] == ["d"]
+    assert adm.peak <= 100
+
+
+def test_admission_respects_streams_and_group_cap():
+    adm = Admission(budget=10 ** 9, streams=2, max_group=2)
+    for k in range(5):
+        adm.add(_item(k, 1, "same"))
+    g1, g2 = adm.take(), adm.take()
+    assert [i[0] for i in g1] == [0, 1] and [i[0] for i in g2] == [2, 3]
+    assert adm.take() is None          # no idle stream
+    adm.release(g1)
+    assert [i[0] for i in adm.take()] == [4]
+
+
+def _request(nz, nx, nt, engine="elastic", checkpoint_mb=256.0):
+    g = GridSpec(nx=nx, nz=nz, dx=5.0, dz=5.0)
+    vp = np.full((nz, nx), 3000.0, np.float32)
+    model = ElasticModel(g, vp=vp, vs=vp / 1.8, rho=np.full_like(vp, 2200.0),
+                         qp=np.full_like(vp, 100.0), qs=np.full_like(vp, 50.0))
+    rx = np.arange(0, nx, 2) * 5.0
+    rec = ReceiverSpec(x=rx, z=np.full(rx.size, 10.0))
+    comps = ("vx", "vz") if engine == "elastic" else ("p",)
+    data = {c: np.zeros((nt, rx.size), np.float32) for c in comps}
+    shot = ShotRecord(data=data[comps[0]], dt=4e-4, t=np.arange(nt) * 4e-4, receivers=rec,
+                      source=SourceSpec(x=100.0, z=10.0), components=data)
+    return ShotRequest(shot=shot, model=model,
+                       solver=SolverConfig(nt=nt, dt=4e-4, engine=engine, boundary="free"),
+                       imaging=ImagingConfig(checkpoint_mb=checkpoint_mb))
+
+
+def test_request_price_covers_the_large_parts_and_grows_with_the_job():
+    small = request_bytes(_request(256, 256, 1500))
+    big = request_bytes(_request(600, 1200, 4000))
+    assert big > small > 0
+    r = _request(600, 1200, 4000)
+    nzp, ld = 610, 1216
+    state = 5 * nzp * ld * 4
+    # two propagators x two ping-pong states, the checkpoint arena (18 states
+    # at the 256 MB rule) and a whole block of replay slots (interval 223)
+    floor = 4 * state + 18 * state + 223 * 600 * 1200 * 4
+    assert request_bytes(r) >= floor
+    # a replay-slot cap shrinks the price
+    assert request_bytes(r, GpuConfig(replay_slots=8)) < request_bytes(r)
+    # acoustic requests are cheaper than elastic ones of the same shape
+    assert request_bytes(_request(600, 1200, 4000, "acoustic")) < request_bytes(r)

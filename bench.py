@@ -99,6 +99,90 @@ def model_records(model, sources, receivers, nt):
     return solver.forward_batch(model, sources, receivers, components=("vx", "vz"))
 
 
+def serving_request(base_seed, i, scale):
+    """One serving request (the mixed acoustic/elastic request stream), grid
+    sizes and nt scaled by ``scale``: engine acoustic or elastic (p = 0.5),
+    nz U{256..2000}, nx U{256..4000}, dx U[2.5, 10], nt U{1500..8000}; 2-8
+    layers with sorted vp U[1800, 5000], vs = vp/1.8, Gardner density; 0-3
+    elliptical anomalies (radii U[20, 200] m, vp and vs x0.6, density x0.85);
+    f0 = clip(vmin / (12.5 dx), 5, 40); dt from the CFL rule with safety 0.5;
+    a surface source at a random x; receivers every 2 cells at z = 2 dz;
+    zero_lag imaging; a seeded random record."""
+    from ertm import numerics as num
+    from ertm.config import ImagingConfig, SolverConfig
+    from ertm.serving import ShotRequest
+    from ertm.types import ElasticModel, GridSpec, ReceiverSpec, ShotRecord, SourceSpec
+
+    rng = np.random.default_rng([base_seed, i])
+    engine = "acoustic" if rng.random() < 0.5 else "elastic"
+    nz = max(64, int(rng.integers(256, 2001) * scale))
+    nx = max(64, int(rng.integers(256, 4001) * scale))
+    dx = float(rng.uniform(2.5, 10.0))
+    nt = max(300, int(rng.integers(1500, 8001) * scale))
+    z = (np.arange(nz) * dx)[:, None]
+    x = (np.arange(nx) * dx)[None, :]
+    nl = int(rng.integers(2, 9))
+    vps = np.sort(rng.uniform(1800.0, 5000.0, nl))
+    tops = np.concatenate([[0.0], np.sort(rng.uniform(0.0, nz * dx, nl - 1))])
+    vp = np.zeros((nz, nx))
+    for top, v in zip(tops, vps):
+        vp[np.broadcast_to(z >= top, (nz, nx))] = v
+    vs = vp / 1.8
+    rho = 310.0 * vp ** 0.25
+    for _ in range(int(rng.integers(0, 4))):
+        cx, cz = rng.uniform(0, nx * dx), rng.uniform(0, nz * dx)
+        rx, rz = rng.uniform(20.0, 200.0, 2)
+        m = ((x - cx) / rx) ** 2 + ((z - cz) / rz) ** 2 <= 1.0
+        vp, vs, rho = np.where(m, vp * 0.6, vp), np.where(m, vs * 0.6, vs), np.where(m, rho * 0.85, rho)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    model = ElasticModel(GridSpec(nx=nx, nz=nz, dx=dx, dz=dx), vp=f32(vp), vs=f32(vs),
+                         rho=f32(rho), qp=f32(np.full((nz, nx), 100.0)),
+                         qs=f32(np.full((nz, nx), 50.0)))
+    dt = float(num.cfl_dt(float(vp.max()), dx, dx, 8, 0.5))
+    f0 = float(np.clip(float(vp.min()) / (12.5 * dx), 5.0, 40.0))
+    src = SourceSpec(x=float(rng.uniform(0.0, (nx - 1) * dx)), z=2 * dx, f0=f0, delay=1.2 / f0)
+    rxs = np.arange(0, nx, 2) * dx
+    recv = ReceiverSpec(x=rxs, z=np.full(rxs.size, 2 * dx))
+    comps = ("p",) if engine == "acoustic" else ("vx", "vz")
+    data = {c: (rng.standard_normal((nt, rxs.size)) * 1e-9).astype(np.float32) for c in comps}
+    shot = ShotRecord(data=data[comps[0]], dt=dt, t=np.arange(nt) * dt, receivers=recv,
+                      source=src, components=data)
+    return ShotRequest(shot=shot, model=model,
+                       solver=SolverConfig(nt=nt, dt=dt, order=8, boundary="cpml",
+                                           engine=engine, npml=20),
+                       imaging=ImagingConfig(condition="zero_lag"), rid=i), 3 * nt * nz * nx
+
+
+def serving_leg(n, scale, load, gcells_per_s, seed=2024):
+    """Poisson arrivals at ``load`` x the rate the measured RTM cell-update
+    throughput would sustain; returns the loop's shots/s and latencies."""
+    from ertm.config import GpuConfig
+    from ertm.serving import ServingLoop
+
+    reqs = [serving_request(seed, i, scale) for i in range(n)]
+    mean_cells = float(np.mean([c for _, c in reqs]))
+    rate = load * gcells_per_s * 1e9 / mean_cells
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, n)
+    with ServingLoop(hbm_budget_gb=200.0, streams=4,
+                     gpu_config=GpuConfig(replay_budget_gb=4.0)) as loop:
+        t0 = time.perf_counter()
+        due = t0
+        futs = []
+        for (req, _), gap in zip(reqs, gaps):
+            due += gap
+            time.sleep(max(0.0, due - time.perf_counter()))
+            futs.append(loop.submit(req, arrival=due))
+        res = [f.result() for f in futs]
+        stats = loop.stats()
+    return {"requests": n, "scale": scale, "load": load, "rate_per_s": round(rate, 4),
+            "mean_cell_updates": mean_cells, "completed": stats["completed"],
+            "failed": stats["failed"], "shots_per_s": round(stats.get("shots_per_s", 0.0), 4),
+            "p50_s": round(stats.get("p50_s", float("nan")), 3),
+            "p99_s": round(stats.get("p99_s", float("nan")), 3),
+            "mean_group": round(stats.get("mean_group", 0.0), 3),
+            "errors": sorted({r.error for r in res if not r.ok})[:3]}
+
+
 def read_traffic():
     """HBM bytes per step launch from the committed FETCH_SIZE/WRITE_SIZE
     profile summary (a line 'traffic_bytes_per_launch: N'), else None."""
@@ -147,6 +231,12 @@ def main(argv=None) -> int:
     ap.add_argument("--nt", type=int, default=NT)
     ap.add_argument("--cpu-sample-nt", type=int, default=240)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve", type=int, default=16,
+                    help="serving leg: this many Poisson-arriving mixed requests (0 = skip)")
+    ap.add_argument("--serve-scale", type=float, default=0.5,
+                    help="serving leg: grid sizes and nt scaled by this factor")
+    ap.add_argument("--serve-load", type=float, default=0.5,
+                    help="serving leg: arrival rate as a fraction of the measured throughput")
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="also time this many shots migrated one at a time (the plain per-shot "
                          "path) for the batched/plain ratio; 0 = skip")
@@ -276,6 +366,10 @@ def main(argv=None) -> int:
                                   "shots": args.plain_shots}
         line["batched_over_plain"] = round(shots_s / plain_s, 3)
         print(f"# plain per-shot path: {plain_s:.3f} shots/s", file=sys.stderr, flush=True)
+    if args.serve > 0:
+        print(f"# serving leg: {args.serve} requests", file=sys.stderr, flush=True)
+        line["serving"] = serving_leg(args.serve, args.serve_scale, args.serve_load, gcells)
+        print(f"# serving: {line['serving']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print(f"# cpu baseline: oracle, nt={args.cpu_sample_nt}", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(model, sources, receivers, args.cpu_sample_nt)

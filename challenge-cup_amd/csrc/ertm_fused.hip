@@ -49,6 +49,24 @@ __device__ __forceinline__ void bstore(rsrc_t r, int off, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0, 0);
 }
 
+__device__ __forceinline__ double bload64(rsrc_t r, int off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void bstore64(rsrc_t r, int off, double v) {
+    using u64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, v), r, off, 0, 0);
+}
+
+// Imaging inside the step (IMG template argument): 0 = none; 1 = extract the
+// source-side field of the new state into the replay slot; 2 = correlate it
+// with the receiver-side fields of the new state (ertm/imaging.py:279-324).
+// Same values and float64 accumulation order as k_image: divcurl P = Dx-vx +
+// Dz-vz and S = Dz-vx - Dx-vz from the plain derivatives (no CPML memory),
+// stress P = (sxx + szz)/2 and S = sxz of the stored new stresses.
+#define ERTM_FIMG_NONE 0
+#define ERTM_FIMG_EXTRACT 1
+#define ERTM_FIMG_CORRELATE 2
+
 // Per-block constants of the fused elastic step.
 struct ElBlock {
     rsrc_t rS, rT, rIR, rLAM, rMU;
@@ -63,6 +81,12 @@ struct ElBlock {
     float dt, hx, hz, amp, axc, bxc;
     double rhx, rhz;
     float cf[4];
+    // imaging (IMG != 0): replay slot and accumulators of this propagator
+    rsrc_t rR, rA;
+    int pad, nzph, nxph, jcol;          // jcol: physical column of the lane, -1 outside
+    int o_num0, o_num1, o_ill, o_rill0, o_rill1;  // byte offsets in rA
+    int illum, nrw, rw0, rw1, swave, sep;
+    bool cross, need_s;
 };
 
 // Byte offset that every buffer range check rejects: stores to it are dropped.
@@ -98,12 +122,12 @@ struct ElOffs {
 // PD = prefetch distance in rows: the inputs of the next PD iterations are
 // in flight while a row is computed (more bytes in flight per wave).  PD
 // divides 2K.
-template <int K, int PD>
+template <int K, int PD, int QXL = 2 * K>
 struct ElState {
     float q_sxx[2 * K + 1], q_szz[2 * K + 1], q_sxz[2 * K + 1];  // rows k .. k+2K
-    float q_vx[2 * K];                                          // rows k-K+1 .. k+K
+    float q_vx[QXL];                                   // rows k+K-QXL+1 .. k+K
     float q_vz[2 * K + 1];                                      // rows k-K .. k+K
-    float ir_m, mu_k;                                           // own column, rows m, k
+    float ir_c, mu_c;  // own column, carried from the previous row (see el_iter)
     float n_sxx[PD], n_szz[PD], n_sxz[PD], n_vx[PD], n_vz[PD];  // prefetched inputs of
     float n_ir[PD], n_mu[PD], n_lam[PD];                        // iterations k .. k+PD-1
     __device__ __forceinline__ void fetch(const ElBlock& B, const ElOffs& O, int p) {
@@ -128,17 +152,38 @@ struct ElState {
 // SLOT = k mod PD (rows loaded PD iterations earlier) and refills it with the
 // inputs of iteration k+PD.  SLOT is a compile-time constant (the row loop is
 // unrolled by PD), so no register moves touch loads still in flight.
-template <int K, int NT, int PD, int SLOT, bool STRIP, bool STRESS>
-__device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD>& X, ElOffs& O, int k,
-                                        float* l_sxx, float* l_sxz, float* l_ir, float* l_vx,
-                                        float* l_vz, float* l_mu0, float* l_mu1) {
-    constexpr int QS = 2 * K + 1, QX = 2 * K, QZ = 2 * K + 1;
+// vx queue length: Dz+vx at row k reads rows k-K+1 .. k+K, i.e. K rows ahead
+// of a downward stream but K rows behind an upward one; streaming down the
+// queue ends at the velocity front m = k+K (2K rows), streaming up it must
+// reach back to row k+K (2K+1 rows).  S imaging (Dz-vx, rows k-K .. k+K-1)
+// needs 2K+1 rows either way.
+template <int K, int IMG, int D>
+constexpr int qx_len() { return (IMG == ERTM_FIMG_NONE && D > 0) ? 2 * K : 2 * K + 1; }
+
+//
+// D = +1 streams the chunk top to bottom (rows increasing), D = -1 bottom to
+// top.  Register queues are ordered along the stream: q[j] holds row
+// (first row) + j*D, so a z-derivative at queue index i is dplus/dminus with
+// stride D -- the same operands in the same order as in row space, hence the
+// same rounding.  Velocity runs K rows ahead of stress along the stream:
+// m = k + D*K.  1/rho is needed at rows m and m+1 and mu at rows k and k+1:
+// the row reached first along the stream is carried, the other one loaded.
+template <int K, int NT, int PD, int SLOT, bool STRIP, bool STRESS, int IMG, int D>
+__device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD, qx_len<K, IMG, D>()>& X,
+                                        ElOffs& O, int k, float* l_sxx, float* l_sxz,
+                                        float* l_ir, float* l_vx, float* l_vz, float* l_mu0,
+                                        float* l_mu1) {
+    constexpr int QS = 2 * K + 1, QX = qx_len<K, IMG, D>(), QZ = 2 * K + 1;
+    constexpr int XO = QX - 2 * K;  // q_vx[i] holds row k + (i - (K - 1 + XO)) * D
     const int tid = B.tid, ld4 = B.ld4, nzp = B.nzp;
-    const int m = k + K;
+    const int m = k + D * K;
     const int om = O.svx, om_z = O.svz;  // this iteration's store offsets
     const int otxx = O.txx, otzz = O.tzz, otxz = O.txz;
-    const float vx_old = X.n_vx[SLOT], vz_old = X.n_vz[SLOT], ir_m1 = X.n_ir[SLOT];
-    const float mu_k1 = X.n_mu[SLOT], lam_k = X.n_lam[SLOT];
+    const float vx_old = X.n_vx[SLOT], vz_old = X.n_vz[SLOT], lam_k = X.n_lam[SLOT];
+    const float ir_m = D > 0 ? X.ir_c : X.n_ir[SLOT];   // 1/rho, row m
+    const float ir_m1 = D > 0 ? X.n_ir[SLOT] : X.ir_c;  // row m+1
+    const float mu_k = D > 0 ? X.mu_c : X.n_mu[SLOT];   // mu, row k
+    const float mu_k1 = D > 0 ? X.n_mu[SLOT] : X.mu_c;  // row k+1
 #pragma unroll
     for (int q = 0; q < QS - 1; ++q) {
         X.q_sxx[q] = X.q_sxx[q + 1];
@@ -150,12 +195,12 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD>& X, ElO
     X.q_sxz[QS - 1] = X.n_sxz[SLOT];
     // prefetch the inputs of iteration k+PD; in flight during the next PD rows
     X.fetch(B, O, SLOT);
-    O.advance(ld4);
+    O.advance(D * ld4);
 
     // ---- phase A: x-neighbour rows for the velocity update of row m
     l_sxx[tid] = X.q_sxx[K];
     l_sxz[tid] = X.q_sxz[K];
-    l_ir[tid] = X.ir_m;
+    l_ir[tid] = ir_m;
     __syncthreads();
 
     // velocity of row m on every lane (lanes outside the band and rows
@@ -163,9 +208,9 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD>& X, ElO
     const bool inz_m = m >= K && m < nzp - K;
     const bool own_m = B.own && m >= B.z0 && m < B.z1;
     float a = dplus<K>(l_sxx, tid, 1, B.cf, B.hx, B.rhx);
-    float b = dminus<K>(X.q_sxz, K, 1, B.cf, B.hz, B.rhz);
+    float b = dminus<K>(X.q_sxz, K, D, B.cf, B.hz, B.rhz);
     float cc = dminus<K>(l_sxz, tid, 1, B.cf, B.hx, B.rhx);
-    float d = dplus<K>(X.q_szz, K, 1, B.cf, B.hz, B.rhz);
+    float d = dplus<K>(X.q_szz, K, D, B.cf, B.hz, B.rhz);
     a = B.inx ? a : 0.0f;
     cc = B.inx ? cc : 0.0f;
     b = inz_m ? b : 0.0f;
@@ -194,10 +239,10 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD>& X, ElO
     }
     float irx, irz;
     {
-        float s = X.ir_m + l_ir[tid + 1];
-        irx = (B.c < B.nxp - 1) ? 0.5f * s : X.ir_m;
-        float t = X.ir_m + ir_m1;
-        irz = (m < nzp - 1) ? 0.5f * t : X.ir_m;
+        float s = ir_m + l_ir[tid + 1];
+        irx = (B.c < B.nxp - 1) ? 0.5f * s : ir_m;
+        float t = ir_m + ir_m1;
+        irz = (m < nzp - 1) ? 0.5f * t : ir_m;
     }
     const float bxv = B.dt * irx;
     const float bzv = B.dt * irz;
@@ -222,9 +267,9 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD>& X, ElO
 
     // ---- phase B: x-neighbour rows for the stress update of row k
     if (STRESS) {
-        l_vx[tid] = X.q_vx[K - 1];
+        l_vx[tid] = X.q_vx[K - 1 + XO];
         l_vz[tid] = X.q_vz[K];
-        l_mu0[tid] = X.mu_k;
+        l_mu0[tid] = mu_k;
         l_mu1[tid] = mu_k1;
     }
     __syncthreads();
@@ -232,8 +277,9 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD>& X, ElO
         const bool inz_k = k >= K && k < nzp - K;
         const bool own_k = B.own;
         float ex = dminus<K>(l_vx, tid, 1, B.cf, B.hx, B.rhx);
-        float ez = dminus<K>(X.q_vz, K, 1, B.cf, B.hz, B.rhz);
-        float e = dplus<K>(X.q_vx, K - 1, 1, B.cf, B.hz, B.rhz);
+        float ez = dminus<K>(X.q_vz, K, D, B.cf, B.hz, B.rhz);
+        float e = dplus<K>(X.q_vx, K - 1 + XO, D, B.cf, B.hz, B.rhz);
+        const float ex_plain = ex, ez_plain = ez;
         float f = dplus<K>(l_vz, tid, 1, B.cf, B.hx, B.rhx);
         ex = B.inx ? ex : 0.0f;
         f = B.inx ? f : 0.0f;
@@ -261,14 +307,14 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD>& X, ElO
                 }
             }
         }
-        const float two_mu = 2.0f * X.mu_k;
+        const float two_mu = 2.0f * mu_k;
         const float l2m = lam_k + two_mu;
         float mx0, mx1;  // mu on the cell corner: mean_z(mean_x(mu))
         {
-            float s0 = X.mu_k + l_mu0[tid + 1];
+            float s0 = mu_k + l_mu0[tid + 1];
             float s1 = mu_k1 + l_mu1[tid + 1];
             const bool xin = B.c < B.nxp - 1;
-            mx0 = xin ? 0.5f * s0 : X.mu_k;
+            mx0 = xin ? 0.5f * s0 : mu_k;
             mx1 = xin ? 0.5f * s1 : mu_k1;
         }
         float muxz;
@@ -303,22 +349,77 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD>& X, ElO
         bstore(B.rT, own_k ? otxx : ERTM_NOWRITE, sxx);
         bstore(B.rT, own_k ? otzz : ERTM_NOWRITE, szz);
         bstore(B.rT, own_k ? otxz : ERTM_NOWRITE, sxz);
+        if constexpr (IMG != ERTM_FIMG_NONE) {
+            // physical cell (k - pad, jcol) of an owned lane
+            const int i = k - B.pad;
+            const bool cell = own_k && B.jcol >= 0 && i >= 0 && i < B.nzph;
+            const int q = i * B.nxph + B.jcol;
+            float fp, fs = 0.0f;
+            if (B.sep == ERTM_SEP_STRESS) {
+                const float ss = sxx + szz;
+                fp = 0.5f * ss;
+                fs = sxz;
+            } else {
+                fp = ex_plain + ez_plain;
+                if (B.need_s) {
+                    const float dzvx = dminus<K>(X.q_vx, K + XO - 1, D, B.cf, B.hz, B.rhz);
+                    const float dxvz = dminus<K>(l_vz, tid, 1, B.cf, B.hx, B.rhx);
+                    fs = dzvx - dxvz;
+                }
+            }
+            if constexpr (IMG == ERTM_FIMG_EXTRACT) {
+                bstore(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.swave == ERTM_WAVE_S ? fs : fp);
+            } else {
+                const int q4 = cell ? 4 * q : ERTM_NOWRITE, q8 = 8 * q;
+                const float sv = bload(B.rR, q4);
+                if (B.illum & ERTM_ILLUM_SRC) {
+                    const int o = cell ? B.o_ill + q8 : ERTM_NOWRITE;
+                    const float sq = sv * sv;
+                    bstore64(B.rA, o, bload64(B.rA, o) + (double)sq);
+                }
+                const float rv0 = B.rw0 == ERTM_WAVE_S ? fs : fp;
+                if (B.illum & ERTM_ILLUM_REC) {
+                    const int o = cell ? B.o_rill0 + q8 : ERTM_NOWRITE;
+                    const float rr = rv0 * rv0;
+                    bstore64(B.rA, o, bload64(B.rA, o) + (double)rr);
+                }
+                if (B.cross) {
+                    const int o = cell ? B.o_num0 + q8 : ERTM_NOWRITE;
+                    const float sr = sv * rv0;
+                    bstore64(B.rA, o, bload64(B.rA, o) + (double)sr);
+                }
+                if (B.nrw > 1) {
+                    const float rv1 = B.rw1 == ERTM_WAVE_S ? fs : fp;
+                    if (B.illum & ERTM_ILLUM_REC) {
+                        const int o = cell ? B.o_rill1 + q8 : ERTM_NOWRITE;
+                        const float rr = rv1 * rv1;
+                        bstore64(B.rA, o, bload64(B.rA, o) + (double)rr);
+                    }
+                    if (B.cross) {
+                        const int o = cell ? B.o_num1 + q8 : ERTM_NOWRITE;
+                        const float sr = sv * rv1;
+                        bstore64(B.rA, o, bload64(B.rA, o) + (double)sr);
+                    }
+                }
+            }
+        }
     }
-    X.ir_m = ir_m1;
-    X.mu_k = mu_k1;
+    X.ir_c = D > 0 ? ir_m1 : ir_m;  // the next row's m+1 (down) / m (up) is this one's
+    X.mu_c = D > 0 ? mu_k1 : mu_k;
 }
 
 // Iterations k .. k+cnt-1 (cnt <= PD, block-uniform) with ring slots 0..cnt-1.
-template <int K, int NT, int PD, bool STRIP, bool STRESS, int SLOT = 0>
-__device__ __forceinline__ void el_group(const ElBlock& B, ElState<K, PD>& X, ElOffs& O, int k,
-                                         int cnt, float* l_sxx, float* l_sxz, float* l_ir,
-                                         float* l_vx, float* l_vz, float* l_mu0, float* l_mu1) {
+template <int K, int NT, int PD, bool STRIP, bool STRESS, int IMG, int D, int SLOT = 0>
+__device__ __forceinline__ void el_group(const ElBlock& B, ElState<K, PD, qx_len<K, IMG, D>()>& X,
+                                         ElOffs& O, int k, int cnt, float* l_sxx, float* l_sxz,
+                                         float* l_ir, float* l_vx, float* l_vz, float* l_mu0,
+                                         float* l_mu1) {
     if constexpr (SLOT < PD) {
         if (SLOT < cnt) {
-            el_iter<K, NT, PD, SLOT, STRIP, STRESS>(B, X, O, k + SLOT, l_sxx, l_sxz, l_ir, l_vx,
-                                                    l_vz, l_mu0, l_mu1);
-            el_group<K, NT, PD, STRIP, STRESS, SLOT + 1>(B, X, O, k, cnt, l_sxx, l_sxz, l_ir,
-                                                         l_vx, l_vz, l_mu0, l_mu1);
+            el_iter<K, NT, PD, SLOT, STRIP, STRESS, IMG, D>(B, X, O, k + D * SLOT, l_sxx, l_sxz,
+                                                            l_ir, l_vx, l_vz, l_mu0, l_mu1);
+            el_group<K, NT, PD, STRIP, STRESS, IMG, D, SLOT + 1>(B, X, O, k, cnt, l_sxx, l_sxz,
+                                                                 l_ir, l_vx, l_vz, l_mu0, l_mu1);
         }
     }
 }
@@ -330,66 +431,71 @@ __device__ __forceinline__ void el_group(const ElBlock& B, ElState<K, PD>& X, El
 // between slabs return finite values that only feed lanes whose results are
 // discarded.  The launcher only selects this kernel when a state buffer is
 // smaller than 2 GiB.
-template <int K, int NT, int PD, bool STRIP>
+template <int K, int NT, int PD, bool STRIP, int IMG, int D>
 __device__ __forceinline__ void el_loop(const ElBlock& B, float* l_sxx, float* l_sxz, float* l_ir,
                                         float* l_vx, float* l_vz, float* l_mu0, float* l_mu1) {
-    constexpr int QX = 2 * K, QZ = 2 * K + 1;
+    constexpr int QX = qx_len<K, IMG, D>(), QZ = 2 * K + 1;
     const int ld4 = B.ld4;
-    ElState<K, PD> X;
+    const int dl = D * ld4;  // byte step of one row along the stream
+    ElState<K, PD, QX> X;
 #pragma unroll
     for (int q = 0; q < QX; ++q) X.q_vx[q] = 0.0f;
 #pragma unroll
     for (int q = 0; q < QZ; ++q) X.q_vz[q] = 0.0f;
-    const int kbeg = B.z0 - 2 * K;
+    // first stress row of the stream (the 2K prologue rows lie outside the chunk)
+    const int kbeg = D > 0 ? B.z0 - 2 * K : B.z1 - 1 + 2 * K;
     const int o0 = kbeg * ld4 + B.c4;  // byte offset of (kbeg, c) in a slab
 #pragma unroll
-    for (int q = 0; q < 2 * K; ++q) {  // stress rows kbeg .. kbeg+2K-1 -> slots 1..2K
-        const int o = o0 + q * ld4;
+    for (int q = 0; q < 2 * K; ++q) {  // stress rows kbeg + q*D -> slots 1..2K
+        const int o = o0 + q * dl;
         X.q_sxx[q + 1] = bload(B.rS, o + B.fXX);
         X.q_szz[q + 1] = bload(B.rS, o + B.fZZ);
         X.q_sxz[q + 1] = bload(B.rS, o + B.fXZ);
     }
-    X.ir_m = bload(B.rIR, o0 + K * ld4);
-    X.mu_k = bload(B.rMU, o0);
+    X.ir_c = bload(B.rIR, o0 + (D * K + (D > 0 ? 0 : 1)) * ld4);
+    X.mu_c = bload(B.rMU, o0 + (D > 0 ? 0 : 1) * ld4);
     ElOffs O;  // load offsets of iteration kbeg, store offsets of iteration kbeg
-    O.sxx = o0 + 2 * K * ld4 + B.fXX;
-    O.szz = o0 + 2 * K * ld4 + B.fZZ;
-    O.sxz = o0 + 2 * K * ld4 + B.fXZ;
-    O.vx = o0 + K * ld4 + B.fVX;
-    O.vz = o0 + K * ld4 + B.fVZ;
-    O.ir = o0 + (K + 1) * ld4;
-    O.mu = o0 + ld4;
+    O.sxx = o0 + 2 * K * dl + B.fXX;
+    O.szz = o0 + 2 * K * dl + B.fZZ;
+    O.sxz = o0 + 2 * K * dl + B.fXZ;
+    O.vx = o0 + K * dl + B.fVX;
+    O.vz = o0 + K * dl + B.fVZ;
+    O.ir = o0 + (D * K + (D > 0 ? 1 : 0)) * ld4;
+    O.mu = o0 + (D > 0 ? 1 : 0) * ld4;
     O.lam = o0;
     O.svx = O.vx;
     O.svz = O.vz;
     O.txx = o0 + B.fXX;
     O.tzz = o0 + B.fZZ;
     O.txz = o0 + B.fXZ;
-    // inputs of iterations kbeg .. kbeg+PD-1; the load offsets then point at
-    // iteration kbeg+PD (the store offsets stay at kbeg)
+    // inputs of iterations kbeg .. kbeg+(PD-1)D; the load offsets then point
+    // at iteration kbeg+PD*D (the store offsets stay at kbeg)
 #pragma unroll
     for (int p = 0; p < PD; ++p) {
         X.fetch(B, O, p);
-        O.sxx += ld4; O.szz += ld4; O.sxz += ld4; O.vx += ld4; O.vz += ld4;
-        O.ir += ld4; O.mu += ld4; O.lam += ld4;
+        O.sxx += dl; O.szz += dl; O.sxz += dl; O.vx += dl; O.vz += dl;
+        O.ir += dl; O.mu += dl; O.lam += dl;
     }
     // 2K prologue iterations (a whole number of PD-groups: PD divides 2K),
     // then the chunk's rows in PD-groups and a remainder of < PD iterations,
     // which continues the slot ring from slot 0.
     int k = kbeg;
-    for (; k < B.z0; k += PD)
-        el_group<K, NT, PD, STRIP, false>(B, X, O, k, PD, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0,
-                                          l_mu1);
-    for (; k + PD <= B.z1; k += PD)
-        el_group<K, NT, PD, STRIP, true>(B, X, O, k, PD, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0,
-                                         l_mu1);
-    if (k < B.z1)
-        el_group<K, NT, PD, STRIP, true>(B, X, O, k, B.z1 - k, l_sxx, l_sxz, l_ir, l_vx, l_vz,
-                                         l_mu0, l_mu1);
+    for (int p = 0; p < 2 * K; p += PD, k += D * PD)
+        el_group<K, NT, PD, STRIP, false, IMG, D>(B, X, O, k, PD, l_sxx, l_sxz, l_ir, l_vx, l_vz,
+                                                  l_mu0, l_mu1);
+    const int nrows = B.z1 - B.z0;
+    int q = 0;
+    for (; q + PD <= nrows; q += PD, k += D * PD)
+        el_group<K, NT, PD, STRIP, true, IMG, D>(B, X, O, k, PD, l_sxx, l_sxz, l_ir, l_vx, l_vz,
+                                                 l_mu0, l_mu1);
+    if (q < nrows)
+        el_group<K, NT, PD, STRIP, true, IMG, D>(B, X, O, k, nrows - q, l_sxx, l_sxz, l_ir, l_vx,
+                                                 l_vz, l_mu0, l_mu1);
 }
 
-template <int K, int NT, int PD>
+template <int K, int NT, int PD, int IMG>
 __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict__ props,
+                                                      const ErtmImage* __restrict__ imgs,
                                                       LaunchArgs la, int zc) {
     constexpr int TX = NT - 4 * K;
     // The hardware deals workgroups to the XCDs by linear id (x fastest), so
@@ -466,6 +572,37 @@ __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict
     B.axc = B.colv ? gp(P.ax)[B.c] : 0.0f;
     B.bxc = B.colv ? gp(P.bx)[B.c] : 1.0f;
 
+    if constexpr (IMG != ERTM_FIMG_NONE) {
+        const ErtmImage& I = imgs[slot];
+        const int t = la.t[ys];
+        B.pad = P.pad;
+        B.nzph = P.nz;
+        B.nxph = P.nx;
+        B.jcol = (B.own && B.c >= P.pad && B.c < P.pad + P.nx) ? B.c - P.pad : -1;
+        const int64_t ncell = (int64_t)P.nz * P.nx;
+        B.rR = make_rsrc(I.rbuf + (int64_t)(t - I.t0) * ncell, 4 * ncell);
+        B.sep = I.separation;
+        B.swave = I.swave;
+        B.nrw = I.nrw;
+        B.rw0 = I.rwave[0];
+        B.rw1 = I.rwave[1];
+        B.illum = I.illum;
+        B.cross = t >= I.start_step;
+        B.need_s = IMG == ERTM_FIMG_EXTRACT
+                       ? I.swave == ERTM_WAVE_S
+                       : (I.rwave[0] == ERTM_WAVE_S || (I.nrw > 1 && I.rwave[1] == ERTM_WAVE_S));
+        if (IMG == ERTM_FIMG_CORRELATE) {
+            // the accumulators are one allocation starting at num[0] (checked on the host)
+            const char* a0 = (const char*)I.num[0];
+            B.rA = make_rsrc(a0, 8 * ncell * (2 * I.nrw + 1));
+            B.o_num0 = 0;
+            B.o_num1 = I.nrw > 1 ? (int)((const char*)I.num[1] - a0) : 0;
+            B.o_ill = I.ill ? (int)((const char*)I.ill - a0) : 0;
+            B.o_rill0 = I.rill[0] ? (int)((const char*)I.rill[0] - a0) : 0;
+            B.o_rill1 = (I.nrw > 1 && I.rill[1]) ? (int)((const char*)I.rill[1] - a0) : 0;
+        }
+    }
+
     __shared__ float l_sxx[NT], l_sxz[NT], l_ir[NT], l_vx[NT], l_vz[NT], l_mu0[NT], l_mu1[NT];
     // Does the block (columns x0-2K .. x0+TX+2K, rows z0-2K .. z1+2K) reach a
     // CPML strip?  Uniform per block.
@@ -473,10 +610,18 @@ __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict
     const int rz0 = B.z0 - 2 * K, rz1 = B.z1 + 2 * K;
     const bool strip = (B.wx > 0 && (cx0 < B.wx || cx1 > nxp - B.wx)) ||
                        (B.wzt > 0 && rz0 < B.wzt) || (B.wzb > 0 && rz1 > nzp - B.wzb);
-    if (strip)
-        el_loop<K, NT, PD, true>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
-    else
-        el_loop<K, NT, PD, false>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+    // Alternate chunks stream in opposite directions, so the two blocks on
+    // either side of a chunk seam read the seam's halo rows at about the same
+    // time (both at their start, or both at their end) and the second read
+    // hits in L2.
+    const bool up = (chunk & 1) != 0;
+    if (strip) {
+        if (up) el_loop<K, NT, PD, true, IMG, -1>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+        else el_loop<K, NT, PD, true, IMG, 1>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+    } else {
+        if (up) el_loop<K, NT, PD, false, IMG, -1>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+        else el_loop<K, NT, PD, false, IMG, 1>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+    }
 }
 
 template <int K, int NT>
@@ -485,20 +630,35 @@ static int blocks_for(const ErtmProp& P, int zc) {
     return ((P.nxp + TX - 1) / TX) * ((P.nzp + zc - 1) / zc);
 }
 
-template <int K, int NT, int PD>
-static void launch_fused_k(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
-                           int zc, hipStream_t s) {
+template <int K, int NT, int PD, int IMG>
+static void launch_fused_k(const ErtmProp* dprops, const ErtmImage* dimg,
+                           const ErtmProp* hprops, const LaunchArgs& la, int zc, hipStream_t s) {
     int nb = 0;
     for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la.base + k], zc));
-    hipLaunchKernelGGL((k_elastic_fused<K, NT, PD>), dim3(nb, la.n), dim3(NT), 0, s, dprops, la,
-                       zc);
+    hipLaunchKernelGGL((k_elastic_fused<K, NT, PD, IMG>), dim3(nb, la.n), dim3(NT), 0, s, dprops,
+                       dimg, la, zc);
 }
 
-// variant = 10 * (PD - 1) + w: w = 1, 2, 3 -> 256, 128, 64 threads per block;
-// PD = 1, 2, 4 rows of prefetch (PD divides the 2K = 8 prologue rows).
-// zc = rows per chunk.
+template <int K, int NT, int PD>
+static bool launch_fused_img(const ErtmProp* dprops, const ErtmImage* dimg,
+                             const ErtmProp* hprops, const LaunchArgs& la, int zc, int img,
+                             hipStream_t s) {
+    switch (img) {
+        case ERTM_FIMG_NONE: launch_fused_k<K, NT, PD, ERTM_FIMG_NONE>(dprops, dimg, hprops, la, zc, s); return true;
+        case ERTM_FIMG_EXTRACT: launch_fused_k<K, NT, PD, ERTM_FIMG_EXTRACT>(dprops, dimg, hprops, la, zc, s); return true;
+        case ERTM_FIMG_CORRELATE: launch_fused_k<K, NT, PD, ERTM_FIMG_CORRELATE>(dprops, dimg, hprops, la, zc, s); return true;
+        default: return false;
+    }
+}
+
+// variant w = 1, 2, 3 -> 256, 128, 64 threads per block, one row of
+// prefetch; zc = rows per chunk.  img (ERTM_FIMG_*) != NONE also images the
+// new state; la.t then holds each propagator's time index.  (The row loop
+// takes a prefetch distance PD dividing 2K; deeper prefetch measured no
+// faster on gfx950 and is not instantiated.)
 bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
-                       int order, int variant, int zc, hipStream_t s) {
+                       int order, int variant, int zc, const ErtmImage* dimg, int img,
+                       hipStream_t s) {
     if (order != 8 || zc < 8) return false;
     // every byte offset, including prefetches up to 32 rows past the grid,
     // must fit an int
@@ -507,15 +667,9 @@ bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const Lau
         if (4 * P.state_floats + 128 * (int64_t)P.ld >= ((int64_t)1 << 31)) return false;
     }
     switch (variant) {
-        case 1: launch_fused_k<4, 256, 1>(dprops, hprops, la, zc, s); return true;
-        case 2: launch_fused_k<4, 128, 1>(dprops, hprops, la, zc, s); return true;
-        case 3: launch_fused_k<4, 64, 1>(dprops, hprops, la, zc, s); return true;
-        case 11: launch_fused_k<4, 256, 2>(dprops, hprops, la, zc, s); return true;
-        case 12: launch_fused_k<4, 128, 2>(dprops, hprops, la, zc, s); return true;
-        case 13: launch_fused_k<4, 64, 2>(dprops, hprops, la, zc, s); return true;
-        case 31: launch_fused_k<4, 256, 4>(dprops, hprops, la, zc, s); return true;
-        case 32: launch_fused_k<4, 128, 4>(dprops, hprops, la, zc, s); return true;
-        case 33: launch_fused_k<4, 64, 4>(dprops, hprops, la, zc, s); return true;
+        case 1: return launch_fused_img<4, 256, 1>(dprops, dimg, hprops, la, zc, img, s);
+        case 2: return launch_fused_img<4, 128, 1>(dprops, dimg, hprops, la, zc, img, s);
+        case 3: return launch_fused_img<4, 64, 1>(dprops, dimg, hprops, la, zc, img, s);
         default: return false;
     }
 }

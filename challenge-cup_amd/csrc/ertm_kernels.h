@@ -26,7 +26,8 @@ int tiles_of(const ErtmProp& P);
 void launch_step(const ErtmProp* dprops, const LaunchArgs& la, int order, int max_tiles,
                  bool any_force, hipStream_t s);
 bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
-                       int order, int variant, int zc, hipStream_t s);
+                       int order, int variant, int zc, const ErtmImage* dimg, int img,
+                       hipStream_t s);
 void launch_sample(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,
                    int max_samp, hipStream_t s);
 void launch_inject(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,

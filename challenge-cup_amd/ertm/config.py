@@ -163,6 +163,9 @@ class GpuConfig:
                    (capped at a quarter of the free HBM); over it a block is
                    replayed in segments from sub-checkpoints.
     max_batch      most shots advanced together by ``migrate_batch``.
+    kernel_variant (variant, zc) of the step kernels (see
+                   ertm_batch_set_variant); None = the tuner's choice.  Every
+                   variant gives the same results.
     """
 
     strict_cfl: bool = True
@@ -170,6 +173,7 @@ class GpuConfig:
     replay_slots: int = 0
     replay_budget_gb: float = 16.0
     max_batch: int = 64
+    kernel_variant: tuple | None = None
 
     def validate(self) -> None:
         if self.device_mem_gb <= 0:

@@ -396,9 +396,13 @@ class ImageBuffers:
         s.start_step = int(start_step)
         s.illum = int(illum)  # ILLUM_SRC | ILLUM_REC bits
         z = lambda dt: torch.zeros((nz, nx), dtype=dt, device="cuda")  # noqa: E731
-        self.num = [z(torch.float64) for _ in rwaves]
-        self.rill = [z(torch.float64) for _ in rwaves]
-        self.ill = z(torch.float64)
+        # one allocation, num[0..nrw) | ill | rill[0..nrw): the fused step
+        # kernel reaches them all through one buffer descriptor
+        nrw = len(rwaves)
+        self.acc = torch.zeros((2 * nrw + 1, nz, nx), dtype=torch.float64, device="cuda")
+        self.num = [self.acc[k] for k in range(nrw)]
+        self.ill = self.acc[nrw]
+        self.rill = [self.acc[nrw + 1 + k] for k in range(nrw)]
         for k in range(len(rwaves)):
             s.num[k] = L.ptr(self.num[k])
             s.rill[k] = L.ptr(self.rill[k])
@@ -439,10 +443,13 @@ class Batch:
         from . import tuning
 
         c0 = cores[0]
+        same = all((c.engine, c.order) == (c0.engine, c0.order) for c in cores)
+        fusable = same and tuning.fused_supported(c0.engine, c0.order)
         if variant is None:
-            same = all((c.engine, c.order) == (c0.engine, c0.order) for c in cores)
             variant = (tuning.lookup(c0.engine, c0.order, max(c.nzp for c in cores),
                                      max(c.nxp for c in cores), n) if same else (0, 64))
+        elif int(variant[0]) != 0 and not fusable:
+            variant = (0, 64)  # fused kernels exist for elastic order 8 only
         self.set_variant(*variant)
 
     def set_variant(self, variant: int, zc: int = 64) -> None:

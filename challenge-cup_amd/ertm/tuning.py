@@ -1,9 +1,8 @@
 """Stencil-kernel variant selection and autotuning.
 
 Variants (see ertm_batch_set_variant): 0 = two launches per step (any
-engine/order); 10*(PD-1) + w = fused single launch with w = 1/2/3 for
-256/128/64 threads per block, each block streaming a chunk of ``zc`` rows
-with the inputs of PD rows ahead in flight.  Every variant gives bit-identical
+engine/order); 1/2/3 = fused single launch with 256/128/64 threads per
+block, each block streaming a chunk of ``zc`` rows.  Every variant gives bit-identical
 results, so the choice is purely a performance decision: the heuristic picks
 the widest block that still yields enough blocks to keep all 256 CUs busy,
 and ``autotune`` times the candidates on the real shapes.
@@ -20,7 +19,7 @@ FUSED_THREADS = {1: 256, 2: 128, 3: 64}
 
 
 def threads(variant: int) -> int:
-    return FUSED_THREADS[variant % 10]
+    return FUSED_THREADS[variant]
 _cache: dict = {}
 _lock = threading.Lock()
 CACHE_FILE = os.environ.get("ERTM_TUNE_CACHE", "")
@@ -37,13 +36,14 @@ def blocks(variant: int, zc: int, nzp: int, nxp: int, nbatch: int) -> int:
 
 def heuristic(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
               n_cu: int = 256) -> tuple[int, int]:
-    """128-thread fused blocks with short chunks: many short row streams keep
-    enough loads in flight (measured on 64 shots of 610 x 1210 and one shot
-    of 2010 x 6010); batches too small to fill the chip with them use the
-    two-launch kernels, whose 2-D tiles need no halo recompute."""
+    """128-thread fused blocks with the longest chunk that still gives 16
+    blocks per CU: enough row streams keep loads in flight, and long chunks
+    re-read fewer halo rows (measured on 64 shots of 610 x 1210: 48-row
+    chunks); batches too small to fill the chip use the two-launch kernels,
+    whose 2-D tiles need no halo recompute."""
     if not fused_supported(engine, order):
         return (0, 64)
-    for zc in (19, 16, 10):
+    for zc in (48, 32, 19, 16, 10):
         if blocks(2, zc, nzp, nxp, nbatch) >= 16 * n_cu:
             return (2, zc)
     if blocks(2, 10, nzp, nxp, nbatch) >= 4 * n_cu:
@@ -81,7 +81,7 @@ def record(engine, order, nzp, nxp, nbatch, choice) -> None:
 def candidates(engine: str, order: int) -> list[tuple[int, int]]:
     if not fused_supported(engine, order):
         return [(0, 64)]
-    return [(0, 64)] + [(v, zc) for v in (1, 2, 3, 11, 12, 13, 31, 32, 33)
+    return [(0, 64)] + [(v, zc) for v in (1, 2, 3)
                         for zc in (16, 24, 32, 48, 64, 96, 128, 160, 208, 320)]
 
 

@@ -223,11 +223,10 @@ int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img, void* stream);
 /* Waits for the work enqueued with this batch, then releases it. */
 int ertm_batch_destroy(ertm_batch_t b);
 /* Stencil kernel variant: 0 = two launches per step (velocity, then
- * pressure/stress; every engine and order); 10 * (PD - 1) + w with w = 1, 2, 3
- * and PD = 1, 2, 4 = single fused launch per step (elastic, order 8) with
- * 256, 128, 64 threads per block streaming chunks of zc rows (8 <= zc <=
- * 4096) and the inputs of PD rows ahead in flight.  Results are
- * bit-identical across variants. */
+ * pressure/stress; every engine and order); 1, 2, 3 = single fused launch
+ * per step (elastic, order 8) with 256, 128, 64 threads per block streaming
+ * chunks of zc rows (8 <= zc <= 4096), alternate chunks in opposite
+ * directions.  Results are bit-identical across variants. */
 int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc);
 
 /*

@@ -151,6 +151,42 @@ def test_nested_replay_is_bit_identical(tag):
     assert np.array_equal(res.image, c["image"])
 
 
+@pytest.mark.parametrize("kv", [(2, 19), (1, 10), (3, 64), (2, 8)])
+@pytest.mark.parametrize("tag", ["el_pp_divcurl", "el_ps_divcurl", "el_pp_stress", "el_ps_stress",
+                                 "el_sp_zl", "el_ss_exc", "ac_zero_lag"])
+def test_migrate_with_fused_step_imaging_matches_reference(tag, kv):
+    """Fused step kernels that extract and correlate inside the step (PD = 1
+    variants) give the reference images bit for bit."""
+    _say("migrate fused imaging", tag, kv)
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case(tag)
+    shot, model, imaging, solver, q = migrate_objects(c)
+    res = RTMImager(imaging, solver, q, GpuConfig(kernel_variant=kv)).migrate(shot, model)
+    assert np.array_equal(res.image, c["image"])
+
+
+def test_fused_imaging_pp_ps_nested_batch():
+    _say("fused imaging pp+ps nested batch")
+    import dataclasses
+
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case("el_pp_divcurl")
+    shot, model, imaging, solver, q = migrate_objects(c)
+    src2 = dataclasses.replace(shot.source, x=float(shot.source.x) + 3 * model.grid.dx)
+    shots = [shot, dataclasses.replace(shot, source=src2)]
+    out = RTMImager(imaging, solver, q, GpuConfig(kernel_variant=(2, 16), replay_slots=5)) \
+        .migrate_batch(shots, model, modes=("pp", "ps"))
+    assert np.array_equal(out[0]["pp"].image, c["image"])
+    assert np.array_equal(out[0]["ps"].image, migrate_case("el_ps_divcurl")["image"])
+    alone = RTMImager(imaging, solver, q).migrate_modes(shots[1], model, ("pp", "ps"))
+    for m in ("pp", "ps"):
+        assert np.array_equal(out[1][m].image, alone[m].image)
+
+
 def test_fused_pp_ps_equals_separate_runs():
     _say("fused pp+ps")
     from ertm.imaging import RTMImager
@@ -230,8 +266,7 @@ def test_acoustic_c1_shape_forward_matches_oracle():
 
 
 @pytest.mark.parametrize("variant,zc", [(1, 64), (1, 10), (2, 19), (3, 10), (3, 64), (2, 4096),
-                                        (12, 19), (11, 37), (13, 8), (32, 24), (31, 9),
-                                        (33, 4096)])
+                                        (2, 8), (1, 9), (3, 13), (2, 20)])
 @pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force"])
 def test_fused_elastic_variants_bit_identical(tag, variant, zc):
     """Every fused launch shape reproduces the two-launch kernels (and thus the
@@ -276,7 +311,7 @@ def test_fused_variant_large_random_model_matches_two_launch():
     src = SourceSpec(x=3000.0, z=1500.0, f0=30.0, delay=0.01)
     w = np.sin(np.arange(60) * 0.3).astype(np.float32) * 1e6
     outs = []
-    for v in [(0, 64), (1, 64), (1, 10), (2, 28), (3, 127), (2, 19), (12, 19), (31, 33)]:
+    for v in [(0, 64), (1, 64), (1, 10), (2, 28), (3, 127), (2, 19), (2, 8), (3, 33)]:
         p = build_propagator(m, spec, engine="elastic")
         p.set_source(src)
         p._batches["step"] = dev.Batch([p._core], variant=v)

@@ -366,8 +366,42 @@ static bool fused_image_ok(ertm_batch_t b, int op) {
     return true;
 }
 
+// Internal imaging op: the fused step stores the receiver-side fields of the
+// new state into ring slot t mod R; do_flush later folds R of them into the
+// accumulators (same per-cell order as R correlation passes).
+#define ERTM_IMG_RING_STORE 100
+
+static bool ring_ok(ertm_batch_t b) {
+    if (b->variant < 1 || b->variant > 3 || !b->all_elastic || b->order != 8 || b->any_force)
+        return false;
+    for (size_t k = 0; k < b->props.size(); ++k) {
+        const ErtmImage& I = b->img[k];
+        const ErtmProp& P = b->props[k];
+        if (!I.rbuf || !I.rring || I.ring_len < 1 || I.nrw < 1 || I.nrw > 2 || !I.num[0])
+            return false;
+        if ((int64_t)P.nz * P.nx * 4 * I.nrw >= ((int64_t)1 << 31) - 64) return false;
+        if (I.ring_len != b->img[0].ring_len) return false;
+    }
+    return true;
+}
+
 static int img_kind(int op) {
-    return op == ERTM_IMG_EXTRACT ? 1 : op == ERTM_IMG_CORRELATE ? 2 : 0;
+    return op == ERTM_IMG_EXTRACT ? 1 : op == ERTM_IMG_CORRELATE ? 2
+         : op == ERTM_IMG_RING_STORE ? 3 : 0;
+}
+
+static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t s) {
+    const int n = (int)b->props.size();
+    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+        LaunchArgs la{};
+        la.base = base;
+        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
+        la.op = count;
+        for (int k = 0; k < la.n; ++k) la.t[k] = t_hi[base + k];
+        launch_image_flush(b->dprops, b->dimg, la, b->max_img, s);
+        LAUNCHCHK("image_flush");
+    }
+    return ERTM_OK;
 }
 
 // One step; with img_op >= 0 and *imaged set on return, the step launches
@@ -376,7 +410,8 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
                    int img_op = -1, const int32_t* t = nullptr, bool* imaged = nullptr) {
     const int n = (int)b->props.size();
     std::unique_lock<std::mutex> lk(g_timer.m, std::defer_lock);
-    const bool fuse_img = img_op >= 0 && t && fused_image_ok(b, img_op);
+    const bool fuse_img = img_op >= 0 && t &&
+                          (img_op == ERTM_IMG_RING_STORE ? ring_ok(b) : fused_image_ok(b, img_op));
     if (imaged) *imaged = fuse_img;
     // only plain step launches are timed (the 52 B/cell algorithmic figure)
     const bool timed = g_timer.on && !fuse_img;
@@ -630,6 +665,12 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
         }
     }
     const int nsteps = r->t_end - r->t_begin;
+    // backward correlation through the receiver-field ring
+    const bool use_ring = r->phase == 2 && r->op_after == ERTM_IMG_CORRELATE && !r->stabilize &&
+                          ring_ok(b);
+    const int ring_len = use_ring ? b->img[0].ring_len : 0;
+    std::vector<int32_t> t_hi(n, 0);
+    int pending = 0;
     for (int q = 0; q < nsteps; ++q) {
         const int t = r->phase == 2 ? r->t_end - 1 - q : r->t_begin + q;
         for (int k = 0; k < n; ++k) {
@@ -642,9 +683,18 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
         // imaging of the new state inside the step when nothing runs between
         // them that changes the state (the viscoacoustic stabiliser)
         bool imaged = false;
-        const int fop = (!r->stabilize && r->op_after >= 0) ? r->op_after : -1;
+        const int fop = use_ring ? ERTM_IMG_RING_STORE
+                                 : ((!r->stabilize && r->op_after >= 0) ? r->op_after : -1);
         if ((rc = do_step(b, cur, amp.data(), s, fop, tt.data(), &imaged)) != ERTM_OK) return rc;
+        if (use_ring && !imaged) return fail(ERTM_ESTATE, "run: ring imaging launch refused");
         cur ^= 1;
+        if (use_ring) {
+            if (pending == 0) t_hi.assign(tt.begin(), tt.end());
+            if (++pending == ring_len || q == nsteps - 1) {
+                if ((rc = do_flush(b, t_hi.data(), pending, s)) != ERTM_OK) return rc;
+                pending = 0;
+            }
+        }
         if (r->stabilize) {
             if ((rc = do_stabilize(b, cur, step_index, r->cap, r->lp_interval, r->ctl, s)) != ERTM_OK)
                 return rc;

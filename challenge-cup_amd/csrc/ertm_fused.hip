@@ -66,6 +66,7 @@ __device__ __forceinline__ void bstore64(rsrc_t r, int off, double v) {
 #define ERTM_FIMG_NONE 0
 #define ERTM_FIMG_EXTRACT 1
 #define ERTM_FIMG_CORRELATE 2
+#define ERTM_FIMG_RING 3  // store the receiver-side fields into ring slot t mod R
 
 // Per-block constants of the fused elastic step.
 struct ElBlock {
@@ -369,6 +370,11 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD, qx_len<
             }
             if constexpr (IMG == ERTM_FIMG_EXTRACT) {
                 bstore(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.swave == ERTM_WAVE_S ? fs : fp);
+            } else if constexpr (IMG == ERTM_FIMG_RING) {
+                bstore(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.rw0 == ERTM_WAVE_S ? fs : fp);
+                if (B.nrw > 1)
+                    bstore(B.rR, cell ? B.o_num1 + 4 * q : ERTM_NOWRITE,
+                           B.rw1 == ERTM_WAVE_S ? fs : fp);
             } else {
                 const int q4 = cell ? 4 * q : ERTM_NOWRITE, q8 = 8 * q;
                 const float sv = bload(B.rR, q4);
@@ -580,7 +586,14 @@ __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict
         B.nxph = P.nx;
         B.jcol = (B.own && B.c >= P.pad && B.c < P.pad + P.nx) ? B.c - P.pad : -1;
         const int64_t ncell = (int64_t)P.nz * P.nx;
-        B.rR = make_rsrc(I.rbuf + (int64_t)(t - I.t0) * ncell, 4 * ncell);
+        if (IMG == ERTM_FIMG_RING) {
+            // ring slot t mod R: nrw fields of ncell floats; o_num1 = second field
+            B.rR = make_rsrc(I.rring + (int64_t)(t % I.ring_len) * I.nrw * ncell,
+                             4 * ncell * I.nrw);
+            B.o_num1 = (int)(4 * ncell);
+        } else {
+            B.rR = make_rsrc(I.rbuf + (int64_t)(t - I.t0) * ncell, 4 * ncell);
+        }
         B.sep = I.separation;
         B.swave = I.swave;
         B.nrw = I.nrw;
@@ -591,6 +604,7 @@ __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict
         B.need_s = IMG == ERTM_FIMG_EXTRACT
                        ? I.swave == ERTM_WAVE_S
                        : (I.rwave[0] == ERTM_WAVE_S || (I.nrw > 1 && I.rwave[1] == ERTM_WAVE_S));
+        (void)ncell;
         if (IMG == ERTM_FIMG_CORRELATE) {
             // the accumulators are one allocation starting at num[0] (checked on the host)
             const char* a0 = (const char*)I.num[0];
@@ -647,6 +661,7 @@ static bool launch_fused_img(const ErtmProp* dprops, const ErtmImage* dimg,
         case ERTM_FIMG_NONE: launch_fused_k<K, NT, PD, ERTM_FIMG_NONE>(dprops, dimg, hprops, la, zc, s); return true;
         case ERTM_FIMG_EXTRACT: launch_fused_k<K, NT, PD, ERTM_FIMG_EXTRACT>(dprops, dimg, hprops, la, zc, s); return true;
         case ERTM_FIMG_CORRELATE: launch_fused_k<K, NT, PD, ERTM_FIMG_CORRELATE>(dprops, dimg, hprops, la, zc, s); return true;
+        case ERTM_FIMG_RING: launch_fused_k<K, NT, PD, ERTM_FIMG_RING>(dprops, dimg, hprops, la, zc, s); return true;
         default: return false;
     }
 }

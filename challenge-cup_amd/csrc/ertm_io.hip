@@ -180,6 +180,70 @@ __global__ void k_image(const ErtmProp* __restrict__ props, const ErtmImage* __r
     }
 }
 
+// Fold `count` backward steps t = t_hi, t_hi-1, ... (la.t = t_hi per
+// propagator, la.op = count) from the receiver-field ring into the
+// accumulators: per cell the additions happen in the same order, with the
+// same float32 products, as `count` ERTM_IMG_CORRELATE passes.
+__global__ void k_image_flush(const ErtmProp* __restrict__ props,
+                              const ErtmImage* __restrict__ imgs, LaunchArgs la) {
+    const int slot = la.base + blockIdx.y;
+    const ErtmProp& P = props[slot];
+    const ErtmImage& I = imgs[slot];
+    const int64_t ncell = (int64_t)P.nz * P.nx;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncell) return;
+    const int t_hi = la.t[blockIdx.y], count = la.op;
+    const bool src = (I.illum & ERTM_ILLUM_SRC) != 0, rec = (I.illum & ERTM_ILLUM_REC) != 0;
+    const int nrw = I.nrw;
+    double ill = src ? I.ill[q] : 0.0;
+    double n0 = I.num[0][q], n1 = nrw > 1 ? I.num[1][q] : 0.0;
+    double r0 = rec ? I.rill[0][q] : 0.0, r1 = (rec && nrw > 1) ? I.rill[1][q] : 0.0;
+    for (int j = 0; j < count; ++j) {
+        const int t = t_hi - j;
+        const float sv = I.rbuf[(int64_t)(t - I.t0) * ncell + q];
+        const float* ring = I.rring + (int64_t)(t % I.ring_len) * nrw * ncell;
+        if (src) {
+            float ss = sv * sv;
+            ill = ill + (double)ss;
+        }
+        const bool cross = t >= I.start_step;
+        const float rv0 = ring[q];
+        if (rec) {
+            float rr = rv0 * rv0;
+            r0 = r0 + (double)rr;
+        }
+        if (cross) {
+            float sr = sv * rv0;
+            n0 = n0 + (double)sr;
+        }
+        if (nrw > 1) {
+            const float rv1 = ring[ncell + q];
+            if (rec) {
+                float rr = rv1 * rv1;
+                r1 = r1 + (double)rr;
+            }
+            if (cross) {
+                float sr = sv * rv1;
+                n1 = n1 + (double)sr;
+            }
+        }
+    }
+    if (src) I.ill[q] = ill;
+    I.num[0][q] = n0;
+    if (nrw > 1) I.num[1][q] = n1;
+    if (rec) {
+        I.rill[0][q] = r0;
+        if (nrw > 1) I.rill[1][q] = r1;
+    }
+}
+
+void launch_image_flush(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la,
+                        int max_cells, hipStream_t s) {
+    if (max_cells <= 0 || la.op <= 0) return;
+    hipLaunchKernelGGL(k_image_flush, dim3((max_cells + 255) / 256, la.n), dim3(256), 0, s, dprops,
+                       dimg, la);
+}
+
 void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,
                   int max_cells, hipStream_t s) {
     if (max_cells <= 0) return;

@@ -32,6 +32,8 @@ void launch_sample(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchAr
                    int max_samp, hipStream_t s);
 void launch_inject(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,
                    int max_cells, hipStream_t s);
+void launch_image_flush(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la,
+                        int max_cells, hipStream_t s);
 void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,
                   int max_cells, hipStream_t s);
 void launch_absmax(const float* x, int64_t n, float* out, hipStream_t s);

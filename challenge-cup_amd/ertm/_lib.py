@@ -71,10 +71,10 @@ class ErtmImage(C.Structure):
     _fields_ = [
         ("separation", C.c_int32), ("swave", C.c_int32), ("nrw", C.c_int32),
         ("rwave", C.c_int32 * 2), ("start_step", C.c_int32), ("illum", C.c_int32),
-        ("t0", C.c_int32), ("nslots", C.c_int32), ("_pad0", C.c_int32),
+        ("t0", C.c_int32), ("nslots", C.c_int32), ("ring_len", C.c_int32),
         ("rbuf", C.c_void_p), ("num", C.c_void_p * 2), ("ill", C.c_void_p),
         ("rill", C.c_void_p * 2), ("peak_step", C.c_void_p), ("peak_amp", C.c_void_p),
-        ("running", C.c_void_p),
+        ("running", C.c_void_p), ("rring", C.c_void_p),
     ]
 
 

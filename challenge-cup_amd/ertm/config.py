@@ -166,6 +166,9 @@ class GpuConfig:
     kernel_variant (variant, zc) of the step kernels (see
                    ertm_batch_set_variant); None = the tuner's choice.  Every
                    variant gives the same results.
+    image_ring     backward steps whose receiver fields a fused elastic batch
+                   folds into the image accumulators in one pass (0 = fold
+                   every step inside the step kernel).  Same results.
     """
 
     strict_cfl: bool = True
@@ -174,6 +177,7 @@ class GpuConfig:
     replay_budget_gb: float = 16.0
     max_batch: int = 64
     kernel_variant: tuple | None = None
+    image_ring: int = 16
 
     def validate(self) -> None:
         if self.device_mem_gb <= 0:
@@ -184,3 +188,5 @@ class GpuConfig:
             raise ConfigError("gpu.replay_budget_gb must be positive")
         if self.max_batch < 1:
             raise ConfigError("gpu.max_batch must be at least 1")
+        if self.image_ring < 0:
+            raise ConfigError("gpu.image_ring must not be negative")

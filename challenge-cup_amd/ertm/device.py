@@ -384,7 +384,8 @@ class ImageBuffers:
     """Accumulators and replay slots behind one ErtmImage."""
 
     def __init__(self, nz: int, nx: int, separation: str, swave: str, rwaves, start_step: int,
-                 illum: int, excitation: bool = False, rbuf=None, nslots: int = 0):
+                 illum: int, excitation: bool = False, rbuf=None, nslots: int = 0,
+                 ring: int = 0):
         torch = torch_mod()
         self.s = L.ErtmImage()
         s = self.s
@@ -410,6 +411,11 @@ class ImageBuffers:
         self.rbuf = rbuf
         s.rbuf = L.ptr(rbuf)
         s.nslots = int(nslots)
+        self.rring = None
+        if ring > 0 and nrw > 0:
+            self.rring = torch.empty((int(ring), nrw, nz, nx), dtype=torch.float32, device="cuda")
+            s.rring = L.ptr(self.rring)
+            s.ring_len = int(ring)
         if excitation:
             self.peak_step = torch.full((nz, nx), -1, dtype=torch.int32, device="cuda")
             self.peak_amp = z(torch.float64)

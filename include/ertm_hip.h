@@ -191,7 +191,7 @@ typedef struct ErtmImage {
                               rill (receiver) illumination every step */
     int32_t t0;            /* time index of replay slot 0 */
     int32_t nslots;        /* K */
-    int32_t _pad0;
+    int32_t ring_len;      /* R: steps of receiver fields in rring (0 = none) */
     float* rbuf;
     double* num[2];
     double* ill;
@@ -199,6 +199,10 @@ typedef struct ErtmImage {
     int32_t* peak_step;
     double* peak_amp;
     double* running;
+    /* optional (R, nrw, nz, nx) float32 ring: with it, a fused elastic batch
+     * stores the receiver fields of up to R backward steps and folds them into
+     * the accumulators in one pass (same per-cell order: identical results) */
+    float* rring;
 } ErtmImage;
 
 typedef struct ertm_batch_s* ertm_batch_t;

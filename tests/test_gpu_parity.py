@@ -151,19 +151,22 @@ def test_nested_replay_is_bit_identical(tag):
     assert np.array_equal(res.image, c["image"])
 
 
-@pytest.mark.parametrize("kv", [(2, 19), (1, 10), (3, 64), (2, 8)])
+@pytest.mark.parametrize("kv,ring", [((2, 19), 16), ((1, 10), 0), ((3, 64), 7), ((2, 8), 1),
+                                     ((2, 48), 0)])
 @pytest.mark.parametrize("tag", ["el_pp_divcurl", "el_ps_divcurl", "el_pp_stress", "el_ps_stress",
                                  "el_sp_zl", "el_ss_exc", "ac_zero_lag"])
-def test_migrate_with_fused_step_imaging_matches_reference(tag, kv):
-    """Fused step kernels that extract and correlate inside the step (PD = 1
-    variants) give the reference images bit for bit."""
-    _say("migrate fused imaging", tag, kv)
+def test_migrate_with_fused_step_imaging_matches_reference(tag, kv, ring):
+    """Fused step kernels that extract and correlate inside the step, or store
+    the receiver fields in a ring folded every `ring` steps, give the
+    reference images bit for bit."""
+    _say("migrate fused imaging", tag, kv, ring)
     from ertm.config import GpuConfig
     from ertm.imaging import RTMImager
 
     c = migrate_case(tag)
     shot, model, imaging, solver, q = migrate_objects(c)
-    res = RTMImager(imaging, solver, q, GpuConfig(kernel_variant=kv)).migrate(shot, model)
+    res = RTMImager(imaging, solver, q, GpuConfig(kernel_variant=kv, image_ring=ring)).migrate(
+        shot, model)
     assert np.array_equal(res.image, c["image"])
 
 

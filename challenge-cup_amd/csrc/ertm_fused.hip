@@ -73,7 +73,7 @@ struct ElBlock {
     rsrc_t rS, rT, rIR, rLAM, rMU;
     gcfloat* AZ;
     gcfloat* BZ;
-    int nzp, nxp, z0, z1, c, c4, tid, ld4;
+    int nzp, nxp, z0, z1, c, c4, c4own, tid, ld4;
     int fVX, fVZ, fXX, fZZ, fXZ;
     int px0, px1, px2, px3, pz0, pz1, pz2, pz3;
     int wx, wzt, wzb, xrow4, jx4;
@@ -95,64 +95,49 @@ struct ElBlock {
 
 // The row loop.  STRIP = false is the interior version, used by blocks that
 // touch no CPML strip: its body is straight-line (every lane issues the same
-// loads and stores; lanes that must not write store to ERTM_NOWRITE), so the
+// loads and stores; stores that must not happen go out of range), so the
 // number of memory operations per row is fixed and the compiler's wait for
 // the prefetched loads leaves the previous row's stores in flight (vmcnt
 // also counts stores on this architecture).  STRIP = true adds the CPML
 // memory reads/writes, inside per-lane branches.
 //
-// One descriptor covers a whole state buffer (fields + CPML strips); the field
-// slab offset is folded into the per-lane byte offset, so the hardware range
-// check applies to the full offset: reads outside the buffer return 0, reads
-// between slabs return finite values that only feed lanes whose results are
-// discarded.  The launcher only selects this kernel when a state buffer is
-// smaller than 2 GiB.
-// Running per-lane byte offsets (each advances by one row per iteration).
-// At iteration k the loads fetch the inputs of iteration k+PD: stress row
-// k+PD+2K, velocity row k+PD+K, 1/rho row k+PD+K+1, mu row k+PD+1, lambda
-// row k+PD; the stores write velocity row m = k+K and stress row k.
-struct ElOffs {
-    int sxx, szz, sxz, vx, vz, ir, mu, lam;  // loads
-    int svx, svz, txx, tzz, txz;             // stores
-    __device__ __forceinline__ void advance(int d) {
-        sxx += d; szz += d; sxz += d; vx += d; vz += d; ir += d; mu += d; lam += d;
-        svx += d; svz += d; txx += d; tzz += d; txz += d;
-    }
-};
+// Addressing: a lane's byte offset is its column (c4) plus a row part that
+// is uniform across the block (row * ld4 + field slab offset), so only c4 and
+// c4own live in vector registers.  One descriptor covers a whole state
+// buffer (fields + CPML strips): reads of rows above the grid wrap to huge
+// offsets and return 0, reads between slabs return finite values that only
+// feed results that are discarded.  A store that must not happen gets
+// ERTM_BAD added (per lane through c4own, per row through the row part),
+// which puts it beyond every buffer's range (the launcher keeps state
+// buffers below 1 GiB); the hardware drops it.
+#define ERTM_BAD 0x40000000
 
-// PD = prefetch distance in rows: the inputs of the next PD iterations are
-// in flight while a row is computed (more bytes in flight per wave).  PD
-// divides 2K.
-template <int K, int PD, int QXL = 2 * K>
+template <int K, int QXL>
 struct ElState {
-    float q_sxx[2 * K + 1], q_szz[2 * K + 1], q_sxz[2 * K + 1];  // rows k .. k+2K
-    float q_vx[QXL];                                   // rows k+K-QXL+1 .. k+K
-    float q_vz[2 * K + 1];                                      // rows k-K .. k+K
+    float q_sxx[2 * K + 1], q_szz[2 * K + 1], q_sxz[2 * K + 1];  // rows k .. k+2K*D
+    float q_vx[QXL];   // q_vx[i] holds row k + (i - (K - 1 + QXL - 2K)) * D
+    float q_vz[2 * K + 1];                                      // rows k-K*D .. k+K*D
     float ir_c, mu_c;  // own column, carried from the previous row (see el_iter)
-    float n_sxx[PD], n_szz[PD], n_sxz[PD], n_vx[PD], n_vz[PD];  // prefetched inputs of
-    float n_ir[PD], n_mu[PD], n_lam[PD];                        // iterations k .. k+PD-1
-    __device__ __forceinline__ void fetch(const ElBlock& B, const ElOffs& O, int p) {
-        n_sxx[p] = bload(B.rS, O.sxx);
-        n_szz[p] = bload(B.rS, O.szz);
-        n_sxz[p] = bload(B.rS, O.sxz);
-        n_vx[p] = bload(B.rS, O.vx);
-        n_vz[p] = bload(B.rS, O.vz);
-        n_ir[p] = bload(B.rIR, O.ir);
-        n_mu[p] = bload(B.rMU, O.mu);
-        n_lam[p] = bload(B.rLAM, O.lam);
-    }
+    float n_sxx, n_szz, n_sxz, n_vx, n_vz, n_ir, n_mu, n_lam;   // inputs of this row
 };
 
-// One row iteration.  STRESS = false: velocity only (the 2K prologue rows
-// of a chunk, whose stress rows lie above the chunk).  Every lane issues the
-// same memory operations (lanes that must not write store to ERTM_NOWRITE),
-// so the wait for the prefetched loads leaves earlier stores in flight
-// (vmcnt also counts stores on this architecture).  STRIP adds the CPML
-// memory reads/writes in per-lane branches.
-// The prefetched inputs form a ring of PD slots: iteration k reads slot
-// SLOT = k mod PD (rows loaded PD iterations earlier) and refills it with the
-// inputs of iteration k+PD.  SLOT is a compile-time constant (the row loop is
-// unrolled by PD), so no register moves touch loads still in flight.
+// Load the inputs of iteration k: stress row k+2K*D, velocity row m = k+K*D,
+// 1/rho row m+1 (down) / m (up), mu row k+1 (down) / k (up), lambda row k.
+template <int K, int QXL, int D>
+__device__ __forceinline__ void el_fetch(const ElBlock& B, ElState<K, QXL>& X, int k) {
+    const int ld4 = B.ld4;
+    const int rs = (k + 2 * K * D) * ld4, rv = (k + K * D) * ld4;
+    const int ri = (k + K * D + (D > 0 ? 1 : 0)) * ld4, rm = (k + (D > 0 ? 1 : 0)) * ld4;
+    X.n_sxx = bload(B.rS, B.c4 + (rs + B.fXX));
+    X.n_szz = bload(B.rS, B.c4 + (rs + B.fZZ));
+    X.n_sxz = bload(B.rS, B.c4 + (rs + B.fXZ));
+    X.n_vx = bload(B.rS, B.c4 + (rv + B.fVX));
+    X.n_vz = bload(B.rS, B.c4 + (rv + B.fVZ));
+    X.n_ir = bload(B.rIR, B.c4 + ri);
+    X.n_mu = bload(B.rMU, B.c4 + rm);
+    X.n_lam = bload(B.rLAM, B.c4 + k * ld4);
+}
+
 // vx queue length: Dz+vx at row k reads rows k-K+1 .. k+K, i.e. K rows ahead
 // of a downward stream but K rows behind an upward one; streaming down the
 // queue ends at the velocity front m = k+K (2K rows), streaming up it must
@@ -161,6 +146,8 @@ struct ElState {
 template <int K, int IMG, int D>
 constexpr int qx_len() { return (IMG == ERTM_FIMG_NONE && D > 0) ? 2 * K : 2 * K + 1; }
 
+// One row iteration.  STRESS = false: velocity only (the 2K prologue rows
+// of a chunk, whose stress rows lie outside the chunk).
 //
 // D = +1 streams the chunk top to bottom (rows increasing), D = -1 bottom to
 // top.  Register queues are ordered along the stream: q[j] holds row
@@ -169,45 +156,59 @@ constexpr int qx_len() { return (IMG == ERTM_FIMG_NONE && D > 0) ? 2 * K : 2 * K
 // same rounding.  Velocity runs K rows ahead of stress along the stream:
 // m = k + D*K.  1/rho is needed at rows m and m+1 and mu at rows k and k+1:
 // the row reached first along the stream is carried, the other one loaded.
-template <int K, int NT, int PD, int SLOT, bool STRIP, bool STRESS, int IMG, int D>
-__device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD, qx_len<K, IMG, D>()>& X,
-                                        ElOffs& O, int k, float* l_sxx, float* l_sxz,
-                                        float* l_ir, float* l_vx, float* l_vz, float* l_mu0,
-                                        float* l_mu1) {
+// L = this iteration's LDS buffer (7 rows of NT floats).
+template <int K, int NT, bool STRIP, bool STRESS, int IMG, int D>
+__device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, qx_len<K, IMG, D>()>& X,
+                                        int k, float* L) {
     constexpr int QS = 2 * K + 1, QX = qx_len<K, IMG, D>(), QZ = 2 * K + 1;
     constexpr int XO = QX - 2 * K;  // q_vx[i] holds row k + (i - (K - 1 + XO)) * D
     const int tid = B.tid, ld4 = B.ld4, nzp = B.nzp;
     const int m = k + D * K;
-    const int om = O.svx, om_z = O.svz;  // this iteration's store offsets
-    const int otxx = O.txx, otzz = O.tzz, otxz = O.txz;
-    const float vx_old = X.n_vx[SLOT], vz_old = X.n_vz[SLOT], lam_k = X.n_lam[SLOT];
-    const float ir_m = D > 0 ? X.ir_c : X.n_ir[SLOT];   // 1/rho, row m
-    const float ir_m1 = D > 0 ? X.n_ir[SLOT] : X.ir_c;  // row m+1
-    const float mu_k = D > 0 ? X.mu_c : X.n_mu[SLOT];   // mu, row k
-    const float mu_k1 = D > 0 ? X.n_mu[SLOT] : X.mu_c;  // row k+1
+    const float vx_old = X.n_vx, vz_old = X.n_vz, lam_k = X.n_lam;
+    const float ir_m = D > 0 ? X.ir_c : X.n_ir;   // 1/rho, row m
+    const float ir_m1 = D > 0 ? X.n_ir : X.ir_c;  // row m+1
+    const float mu_k = D > 0 ? X.mu_c : X.n_mu;   // mu, row k
+    const float mu_k1 = D > 0 ? X.n_mu : X.mu_c;  // row k+1
 #pragma unroll
     for (int q = 0; q < QS - 1; ++q) {
         X.q_sxx[q] = X.q_sxx[q + 1];
         X.q_szz[q] = X.q_szz[q + 1];
         X.q_sxz[q] = X.q_sxz[q + 1];
     }
-    X.q_sxx[QS - 1] = X.n_sxx[SLOT];
-    X.q_szz[QS - 1] = X.n_szz[SLOT];
-    X.q_sxz[QS - 1] = X.n_sxz[SLOT];
-    // prefetch the inputs of iteration k+PD; in flight during the next PD rows
-    X.fetch(B, O, SLOT);
-    O.advance(D * ld4);
+    X.q_sxx[QS - 1] = X.n_sxx;
+    X.q_szz[QS - 1] = X.n_szz;
+    X.q_sxz[QS - 1] = X.n_sxz;
+    // prefetch the next iteration's inputs; in flight during this row
+    el_fetch<K, QX, D>(B, X, k + D);
 
-    // ---- phase A: x-neighbour rows for the velocity update of row m
+    // ---- LDS rows of this iteration, one barrier: stress row m and 1/rho
+    // row m (velocity x-derivatives); velocity row k, computed K iterations
+    // ago, and mu rows k, k+1 (stress x-derivatives).  The buffers alternate
+    // between iterations, so the next iteration's writes cannot overtake a
+    // wave still reading these.
+    float* l_sxx = L;
+    float* l_sxz = L + NT;
+    float* l_ir = L + 2 * NT;
+    float* l_vx = L + 3 * NT;
+    float* l_vz = L + 4 * NT;
+    float* l_mu0 = L + 5 * NT;
+    float* l_mu1 = L + 6 * NT;
     l_sxx[tid] = X.q_sxx[K];
     l_sxz[tid] = X.q_sxz[K];
     l_ir[tid] = ir_m;
+    if (STRESS) {
+        l_vx[tid] = X.q_vx[K + XO];  // row k (the queue is pushed below)
+        l_vz[tid] = X.q_vz[K + 1];
+        l_mu0[tid] = mu_k;
+        l_mu1[tid] = mu_k1;
+    }
     __syncthreads();
 
     // velocity of row m on every lane (lanes outside the band and rows
     // outside the grid compute values nobody uses)
     const bool inz_m = m >= K && m < nzp - K;
-    const bool own_m = B.own && m >= B.z0 && m < B.z1;
+    const bool row_m = m >= B.z0 && m < B.z1;  // velocity row m is this block's
+    const bool own_m = B.own && row_m;
     float a = dplus<K>(l_sxx, tid, 1, B.cf, B.hx, B.rhx);
     float b = dminus<K>(X.q_sxz, K, D, B.cf, B.hz, B.rhz);
     float cc = dminus<K>(l_sxz, tid, 1, B.cf, B.hx, B.rhx);
@@ -257,8 +258,8 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD, qx_len<
     // step have used the pre-source value (ertm/propagation.py:870-871)
     float vzs = vz_new;
     if (B.src_col && m == B.src_iz && B.mech == ERTM_SRC_FORCE_Z) vzs = vzs + B.amp;
-    bstore(B.rT, own_m ? om : ERTM_NOWRITE, vx_new);
-    bstore(B.rT, own_m ? om_z : ERTM_NOWRITE, vzs);
+    bstore(B.rT, B.c4own + (row_m ? m * ld4 + B.fVX : ERTM_BAD), vx_new);
+    bstore(B.rT, B.c4own + (row_m ? m * ld4 + B.fVZ : ERTM_BAD), vzs);
 #pragma unroll
     for (int q = 0; q < QX - 1; ++q) X.q_vx[q] = X.q_vx[q + 1];
     X.q_vx[QX - 1] = vx_new;
@@ -266,14 +267,7 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD, qx_len<
     for (int q = 0; q < QZ - 1; ++q) X.q_vz[q] = X.q_vz[q + 1];
     X.q_vz[QZ - 1] = vz_new;
 
-    // ---- phase B: x-neighbour rows for the stress update of row k
-    if (STRESS) {
-        l_vx[tid] = X.q_vx[K - 1 + XO];
-        l_vz[tid] = X.q_vz[K];
-        l_mu0[tid] = mu_k;
-        l_mu1[tid] = mu_k1;
-    }
-    __syncthreads();
+    // ---- stress of row k
     if (STRESS) {
         const bool inz_k = k >= K && k < nzp - K;
         const bool own_k = B.own;
@@ -347,9 +341,10 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD, qx_len<
             szz = 0.0f;
             sxz = 0.0f;
         }
-        bstore(B.rT, own_k ? otxx : ERTM_NOWRITE, sxx);
-        bstore(B.rT, own_k ? otzz : ERTM_NOWRITE, szz);
-        bstore(B.rT, own_k ? otxz : ERTM_NOWRITE, sxz);
+        const int rk = k * ld4;
+        bstore(B.rT, B.c4own + (rk + B.fXX), sxx);
+        bstore(B.rT, B.c4own + (rk + B.fZZ), szz);
+        bstore(B.rT, B.c4own + (rk + B.fXZ), sxz);
         if constexpr (IMG != ERTM_FIMG_NONE) {
             // physical cell (k - pad, jcol) of an owned lane
             const int i = k - B.pad;
@@ -414,92 +409,38 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, PD, qx_len<
     X.mu_c = D > 0 ? mu_k1 : mu_k;
 }
 
-// Iterations k .. k+cnt-1 (cnt <= PD, block-uniform) with ring slots 0..cnt-1.
-template <int K, int NT, int PD, bool STRIP, bool STRESS, int IMG, int D, int SLOT = 0>
-__device__ __forceinline__ void el_group(const ElBlock& B, ElState<K, PD, qx_len<K, IMG, D>()>& X,
-                                         ElOffs& O, int k, int cnt, float* l_sxx, float* l_sxz,
-                                         float* l_ir, float* l_vx, float* l_vz, float* l_mu0,
-                                         float* l_mu1) {
-    if constexpr (SLOT < PD) {
-        if (SLOT < cnt) {
-            el_iter<K, NT, PD, SLOT, STRIP, STRESS, IMG, D>(B, X, O, k + D * SLOT, l_sxx, l_sxz,
-                                                            l_ir, l_vx, l_vz, l_mu0, l_mu1);
-            el_group<K, NT, PD, STRIP, STRESS, IMG, D, SLOT + 1>(B, X, O, k, cnt, l_sxx, l_sxz,
-                                                                 l_ir, l_vx, l_vz, l_mu0, l_mu1);
-        }
-    }
-}
-
-// One chunk: 2K prologue rows (velocity only), then zc rows (both updates).
-// One descriptor covers a whole state buffer (fields + CPML strips); the field
-// slab offset is part of the per-lane byte offset, so the hardware range
-// check applies to the full offset: reads outside the buffer return 0, reads
-// between slabs return finite values that only feed lanes whose results are
-// discarded.  The launcher only selects this kernel when a state buffer is
-// smaller than 2 GiB.
-template <int K, int NT, int PD, bool STRIP, int IMG, int D>
-__device__ __forceinline__ void el_loop(const ElBlock& B, float* l_sxx, float* l_sxz, float* l_ir,
-                                        float* l_vx, float* l_vz, float* l_mu0, float* l_mu1) {
+// One chunk: 2K prologue rows (velocity only), then the chunk's rows (both
+// updates), streaming in direction D.
+template <int K, int NT, bool STRIP, int IMG, int D>
+__device__ __forceinline__ void el_loop(const ElBlock& B, float* lds) {
     constexpr int QX = qx_len<K, IMG, D>(), QZ = 2 * K + 1;
     const int ld4 = B.ld4;
-    const int dl = D * ld4;  // byte step of one row along the stream
-    ElState<K, PD, QX> X;
+    ElState<K, QX> X;
 #pragma unroll
     for (int q = 0; q < QX; ++q) X.q_vx[q] = 0.0f;
 #pragma unroll
     for (int q = 0; q < QZ; ++q) X.q_vz[q] = 0.0f;
     // first stress row of the stream (the 2K prologue rows lie outside the chunk)
     const int kbeg = D > 0 ? B.z0 - 2 * K : B.z1 - 1 + 2 * K;
-    const int o0 = kbeg * ld4 + B.c4;  // byte offset of (kbeg, c) in a slab
 #pragma unroll
     for (int q = 0; q < 2 * K; ++q) {  // stress rows kbeg + q*D -> slots 1..2K
-        const int o = o0 + q * dl;
-        X.q_sxx[q + 1] = bload(B.rS, o + B.fXX);
-        X.q_szz[q + 1] = bload(B.rS, o + B.fZZ);
-        X.q_sxz[q + 1] = bload(B.rS, o + B.fXZ);
+        const int r = (kbeg + q * D) * ld4;
+        X.q_sxx[q + 1] = bload(B.rS, B.c4 + (r + B.fXX));
+        X.q_szz[q + 1] = bload(B.rS, B.c4 + (r + B.fZZ));
+        X.q_sxz[q + 1] = bload(B.rS, B.c4 + (r + B.fXZ));
     }
-    X.ir_c = bload(B.rIR, o0 + (D * K + (D > 0 ? 0 : 1)) * ld4);
-    X.mu_c = bload(B.rMU, o0 + (D > 0 ? 0 : 1) * ld4);
-    ElOffs O;  // load offsets of iteration kbeg, store offsets of iteration kbeg
-    O.sxx = o0 + 2 * K * dl + B.fXX;
-    O.szz = o0 + 2 * K * dl + B.fZZ;
-    O.sxz = o0 + 2 * K * dl + B.fXZ;
-    O.vx = o0 + K * dl + B.fVX;
-    O.vz = o0 + K * dl + B.fVZ;
-    O.ir = o0 + (D * K + (D > 0 ? 1 : 0)) * ld4;
-    O.mu = o0 + (D > 0 ? 1 : 0) * ld4;
-    O.lam = o0;
-    O.svx = O.vx;
-    O.svz = O.vz;
-    O.txx = o0 + B.fXX;
-    O.tzz = o0 + B.fZZ;
-    O.txz = o0 + B.fXZ;
-    // inputs of iterations kbeg .. kbeg+(PD-1)D; the load offsets then point
-    // at iteration kbeg+PD*D (the store offsets stay at kbeg)
-#pragma unroll
-    for (int p = 0; p < PD; ++p) {
-        X.fetch(B, O, p);
-        O.sxx += dl; O.szz += dl; O.sxz += dl; O.vx += dl; O.vz += dl;
-        O.ir += dl; O.mu += dl; O.lam += dl;
-    }
-    // 2K prologue iterations (a whole number of PD-groups: PD divides 2K),
-    // then the chunk's rows in PD-groups and a remainder of < PD iterations,
-    // which continues the slot ring from slot 0.
+    X.ir_c = bload(B.rIR, B.c4 + (kbeg + D * K + (D > 0 ? 0 : 1)) * ld4);
+    X.mu_c = bload(B.rMU, B.c4 + (kbeg + (D > 0 ? 0 : 1)) * ld4);
+    el_fetch<K, QX, D>(B, X, kbeg);
     int k = kbeg;
-    for (int p = 0; p < 2 * K; p += PD, k += D * PD)
-        el_group<K, NT, PD, STRIP, false, IMG, D>(B, X, O, k, PD, l_sxx, l_sxz, l_ir, l_vx, l_vz,
-                                                  l_mu0, l_mu1);
-    const int nrows = B.z1 - B.z0;
-    int q = 0;
-    for (; q + PD <= nrows; q += PD, k += D * PD)
-        el_group<K, NT, PD, STRIP, true, IMG, D>(B, X, O, k, PD, l_sxx, l_sxz, l_ir, l_vx, l_vz,
-                                                 l_mu0, l_mu1);
-    if (q < nrows)
-        el_group<K, NT, PD, STRIP, true, IMG, D>(B, X, O, k, nrows - q, l_sxx, l_sxz, l_ir, l_vx,
-                                                 l_vz, l_mu0, l_mu1);
+    int par = 0;
+    for (int p = 0; p < 2 * K; ++p, k += D, par ^= 1)
+        el_iter<K, NT, STRIP, false, IMG, D>(B, X, k, lds + par * 7 * NT);
+    for (int p = B.z0; p < B.z1; ++p, k += D, par ^= 1)
+        el_iter<K, NT, STRIP, true, IMG, D>(B, X, k, lds + par * 7 * NT);
 }
 
-template <int K, int NT, int PD, int IMG>
+template <int K, int NT, int IMG>
 __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict__ props,
                                                       const ErtmImage* __restrict__ imgs,
                                                       LaunchArgs la, int zc) {
@@ -529,6 +470,7 @@ __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict
     B.colv = B.c >= 0 && B.c < nxp;
     B.velc = B.colv && B.tid >= K && B.tid < NT - K;
     B.own = B.colv && B.tid >= 2 * K && B.tid < NT - 2 * K;
+    B.c4own = B.own ? B.c4 : ERTM_BAD;
     B.ld4 = 4 * P.ld;
     const int fs4 = (int)(4 * P.fstride);  // bytes between field slabs
     const int64_t sbytes = 4 * P.state_floats;
@@ -617,7 +559,7 @@ __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict
         }
     }
 
-    __shared__ float l_sxx[NT], l_sxz[NT], l_ir[NT], l_vx[NT], l_vz[NT], l_mu0[NT], l_mu1[NT];
+    __shared__ float lds[2 * 7 * NT];  // two buffers of 7 rows (see el_iter)
     // Does the block (columns x0-2K .. x0+TX+2K, rows z0-2K .. z1+2K) reach a
     // CPML strip?  Uniform per block.
     const int cx0 = x0 - 2 * K, cx1 = x0 + TX + 2 * K;
@@ -630,11 +572,11 @@ __global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict
     // hits in L2.
     const bool up = (chunk & 1) != 0;
     if (strip) {
-        if (up) el_loop<K, NT, PD, true, IMG, -1>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
-        else el_loop<K, NT, PD, true, IMG, 1>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+        if (up) el_loop<K, NT, true, IMG, -1>(B, lds);
+        else el_loop<K, NT, true, IMG, 1>(B, lds);
     } else {
-        if (up) el_loop<K, NT, PD, false, IMG, -1>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
-        else el_loop<K, NT, PD, false, IMG, 1>(B, l_sxx, l_sxz, l_ir, l_vx, l_vz, l_mu0, l_mu1);
+        if (up) el_loop<K, NT, false, IMG, -1>(B, lds);
+        else el_loop<K, NT, false, IMG, 1>(B, lds);
     }
 }
 
@@ -644,47 +586,46 @@ static int blocks_for(const ErtmProp& P, int zc) {
     return ((P.nxp + TX - 1) / TX) * ((P.nzp + zc - 1) / zc);
 }
 
-template <int K, int NT, int PD, int IMG>
+template <int K, int NT, int IMG>
 static void launch_fused_k(const ErtmProp* dprops, const ErtmImage* dimg,
                            const ErtmProp* hprops, const LaunchArgs& la, int zc, hipStream_t s) {
     int nb = 0;
     for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la.base + k], zc));
-    hipLaunchKernelGGL((k_elastic_fused<K, NT, PD, IMG>), dim3(nb, la.n), dim3(NT), 0, s, dprops,
-                       dimg, la, zc);
+    hipLaunchKernelGGL((k_elastic_fused<K, NT, IMG>), dim3(nb, la.n), dim3(NT), 0, s, dprops, dimg,
+                       la, zc);
 }
 
-template <int K, int NT, int PD>
+template <int K, int NT>
 static bool launch_fused_img(const ErtmProp* dprops, const ErtmImage* dimg,
                              const ErtmProp* hprops, const LaunchArgs& la, int zc, int img,
                              hipStream_t s) {
     switch (img) {
-        case ERTM_FIMG_NONE: launch_fused_k<K, NT, PD, ERTM_FIMG_NONE>(dprops, dimg, hprops, la, zc, s); return true;
-        case ERTM_FIMG_EXTRACT: launch_fused_k<K, NT, PD, ERTM_FIMG_EXTRACT>(dprops, dimg, hprops, la, zc, s); return true;
-        case ERTM_FIMG_CORRELATE: launch_fused_k<K, NT, PD, ERTM_FIMG_CORRELATE>(dprops, dimg, hprops, la, zc, s); return true;
-        case ERTM_FIMG_RING: launch_fused_k<K, NT, PD, ERTM_FIMG_RING>(dprops, dimg, hprops, la, zc, s); return true;
+        case ERTM_FIMG_NONE: launch_fused_k<K, NT, ERTM_FIMG_NONE>(dprops, dimg, hprops, la, zc, s); return true;
+        case ERTM_FIMG_EXTRACT: launch_fused_k<K, NT, ERTM_FIMG_EXTRACT>(dprops, dimg, hprops, la, zc, s); return true;
+        case ERTM_FIMG_CORRELATE: launch_fused_k<K, NT, ERTM_FIMG_CORRELATE>(dprops, dimg, hprops, la, zc, s); return true;
+        case ERTM_FIMG_RING: launch_fused_k<K, NT, ERTM_FIMG_RING>(dprops, dimg, hprops, la, zc, s); return true;
         default: return false;
     }
 }
 
-// variant w = 1, 2, 3 -> 256, 128, 64 threads per block, one row of
-// prefetch; zc = rows per chunk.  img (ERTM_FIMG_*) != NONE also images the
-// new state; la.t then holds each propagator's time index.  (The row loop
-// takes a prefetch distance PD dividing 2K; deeper prefetch measured no
-// faster on gfx950 and is not instantiated.)
+// variant w = 1, 2, 3 -> 256, 128, 64 threads per block; zc = rows per chunk.
+// img (ERTM_FIMG_*) != NONE also images the new state; la.t then holds each
+// propagator's time index.
 bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
                        int order, int variant, int zc, const ErtmImage* dimg, int img,
                        hipStream_t s) {
     if (order != 8 || zc < 8) return false;
-    // every byte offset, including prefetches up to 32 rows past the grid,
-    // must fit an int
+    // byte offsets, including rows up to 32 past the grid and ERTM_BAD, must
+    // stay below 2^31; every state buffer below 2^30 bytes keeps ERTM_BAD out
+    // of range
     for (int k = 0; k < la.n; ++k) {
         const ErtmProp& P = hprops[la.base + k];
-        if (4 * P.state_floats + 128 * (int64_t)P.ld >= ((int64_t)1 << 31)) return false;
+        if (4 * P.state_floats + 128 * (int64_t)P.ld >= ((int64_t)1 << 30)) return false;
     }
     switch (variant) {
-        case 1: return launch_fused_img<4, 256, 1>(dprops, dimg, hprops, la, zc, img, s);
-        case 2: return launch_fused_img<4, 128, 1>(dprops, dimg, hprops, la, zc, img, s);
-        case 3: return launch_fused_img<4, 64, 1>(dprops, dimg, hprops, la, zc, img, s);
+        case 1: return launch_fused_img<4, 256>(dprops, dimg, hprops, la, zc, img, s);
+        case 2: return launch_fused_img<4, 128>(dprops, dimg, hprops, la, zc, img, s);
+        case 3: return launch_fused_img<4, 64>(dprops, dimg, hprops, la, zc, img, s);
         default: return false;
     }
 }

@@ -36,14 +36,15 @@ def blocks(variant: int, zc: int, nzp: int, nxp: int, nbatch: int) -> int:
 
 def heuristic(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
               n_cu: int = 256) -> tuple[int, int]:
-    """128-thread fused blocks with the longest chunk that still gives 16
-    blocks per CU: enough row streams keep loads in flight, and long chunks
-    re-read fewer halo rows (measured on 64 shots of 610 x 1210: 48-row
-    chunks); batches too small to fill the chip use the two-launch kernels,
-    whose 2-D tiles need no halo recompute."""
+    """128-thread fused blocks with the longest chunk (up to 24 rows) that
+    still gives 16 blocks per CU: enough row streams keep loads in flight,
+    and alternating chunk directions let L2 absorb most halo re-reads
+    (measured on 64 shots of 610 x 1210: 24-row chunks); batches too small
+    to fill the chip use the two-launch kernels, whose 2-D tiles need no
+    halo recompute."""
     if not fused_supported(engine, order):
         return (0, 64)
-    for zc in (48, 32, 19, 16, 10):
+    for zc in (24, 19, 16, 10):
         if blocks(2, zc, nzp, nxp, nbatch) >= 16 * n_cu:
             return (2, zc)
     if blocks(2, 10, nzp, nxp, nbatch) >= 4 * n_cu:

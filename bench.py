@@ -183,6 +183,96 @@ def serving_leg(n, scale, load, gcells_per_s, seed=2024):
             "errors": sorted({r.error for r in res if not r.ok})[:3]}
 
 
+def c5_model(seed=5):
+    """Large single-shot workload: 2000 x 6000 at 2.5 m, eight layers vp
+    2000-5500 m/s (vs = vp/1.8, Gardner density) and thirty dipping fracture
+    rectangles (60 m x 2.5 m, dip U[50, 80] degrees, vp and vs x0.6, density
+    x0.85) seeded in x 5000-10000 m, z 2000-4000 m."""
+    from ertm.types import ElasticModel, GridSpec
+
+    nz, nx, dx = 2000, 6000, 2.5
+    rng = np.random.default_rng(seed)
+    z = (np.arange(nz) * dx)[:, None]
+    x = (np.arange(nx) * dx)[None, :]
+    vp = np.zeros((nz, nx))
+    for i, v in enumerate(np.linspace(2000.0, 5500.0, 8)):
+        vp[np.broadcast_to(z >= i * nz * dx / 8, (nz, nx))] = v
+    vs, rho = vp / 1.8, 310.0 * vp ** 0.25
+    for _ in range(30):
+        cx, cz = rng.uniform(5000.0, 10000.0), rng.uniform(2000.0, 4000.0)
+        dip = np.deg2rad(rng.uniform(50.0, 80.0))
+        # the rectangle lies within 62.5 m of its centre
+        i0, i1 = max(0, int((cz - 63) / dx)), min(nz, int((cz + 63) / dx) + 1)
+        j0, j1 = max(0, int((cx - 63) / dx)), min(nx, int((cx + 63) / dx) + 1)
+        xs, zs = x[:, j0:j1], z[i0:i1, :]
+        u = (xs - cx) * np.cos(dip) + (zs - cz) * np.sin(dip)
+        w = -(xs - cx) * np.sin(dip) + (zs - cz) * np.cos(dip)
+        m = (np.abs(u) <= 60.0) & (np.abs(w) <= 2.5)
+        for a, f in ((vp, 0.6), (vs, 0.6), (rho, 0.85)):
+            blk = a[i0:i1, j0:j1]
+            blk[m] = blk[m] * f
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    return ElasticModel(grid=GridSpec(nx=nx, nz=nz, dx=dx, dz=dx), vp=f32(vp), vs=f32(vs),
+                        rho=f32(rho), qp=f32(np.full((nz, nx), 100.0)),
+                        qs=f32(np.full((nz, nx), 50.0)))
+
+
+def c5_sweep(budgets_mb, nt):
+    """Excitation vs zero_lag imaging of the large shot at several checkpoint
+    budgets: one JSON line per run with the checkpoint interval, replayed
+    steps / nt (recompute overhead), time and RTM cell-update rate; plus a
+    check that segment replay from sub-checkpoints gives the same image."""
+    import torch
+    from ertm.config import GpuConfig, ImagingConfig, SolverConfig
+    from ertm.imaging import RTMImager
+    from ertm.solver import build_solver
+    from ertm.types import ReceiverSpec, SourceSpec
+
+    model = c5_model()
+    nz, nx = model.shape
+    dt = 2.0e-4
+    src = SourceSpec(x=7500.0, z=5.0, f0=40.0, delay=0.03, mechanism="explosive")
+    recv = ReceiverSpec(x=np.arange(nx) * 2.5, z=np.full(nx, 5.0))
+    solver = SolverConfig(nt=nt, dt=dt, order=8, boundary="free", engine="elastic", npml=20)
+    gpu = GpuConfig(strict_cfl=False)
+    t0 = time.perf_counter()
+    shot = build_solver(solver, gpu_config=gpu).forward(model, src, recv, components=("vx", "vz"))
+    print(f"# c5 record modelled in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    out = []
+    images = {}
+    for cond in ("zero_lag", "excitation"):
+        for mb in budgets_mb:
+            imaging = ImagingConfig(mode="pp", condition=cond, checkpoint_mb=float(mb))
+            imager = RTMImager(imaging, solver, gpu_config=gpu)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = imager.migrate(shot, model)
+            torch.cuda.synchronize()
+            secs = time.perf_counter() - t0
+            meta = res.meta
+            line = {"workload": "elastic 2000x6000 single shot", "condition": cond,
+                    "checkpoint_mb": mb, "interval": meta["checkpoints"]["interval"],
+                    "n_checkpoints": meta["checkpoints"]["n_states"],
+                    "arena_gib": round(meta["checkpoints"]["device_arena_mb"] / 1024, 2),
+                    "replay_overhead": round(meta["replay_overhead"], 4),
+                    "seconds": round(secs, 2),
+                    "rtm_gcell_updates_per_s": round(3 * nt * nz * nx / secs / 1e9, 2),
+                    "finite": bool(np.isfinite(res.image).all())}
+            images[(cond, mb)] = res.image
+            print(json.dumps(line), flush=True)
+            out.append(line)
+    # segment replay (sub-checkpoints, small replay slots) == whole-block replay
+    mb = budgets_mb[0]
+    imaging = ImagingConfig(mode="pp", condition="zero_lag", checkpoint_mb=float(mb))
+    seg = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False, replay_slots=97))
+    res = seg.migrate(shot, model)
+    same = bool(np.array_equal(res.image, images[("zero_lag", mb)]))
+    print(json.dumps({"check": "segment replay (97-step slots) == whole-block replay",
+                      "checkpoint_mb": mb, "bit_exact": same,
+                      "replay_overhead": round(res.meta["replay_overhead"], 4)}), flush=True)
+    return out
+
+
 def read_traffic():
     """HBM bytes per step launch from the committed FETCH_SIZE/WRITE_SIZE
     profile summary (a line 'traffic_bytes_per_launch: N'), else None."""
@@ -237,6 +327,11 @@ def main(argv=None) -> int:
                     help="serving leg: grid sizes and nt scaled by this factor")
     ap.add_argument("--serve-load", type=float, default=0.5,
                     help="serving leg: arrival rate as a fraction of the measured throughput")
+    ap.add_argument("--c5-sweep", action="store_true",
+                    help="instead of the batched benchmark: the large single elastic shot "
+                         "(2000 x 6000, nt 10000), excitation vs zero_lag at checkpoint "
+                         "budgets 1/4/16/64 GB, one JSON line per run")
+    ap.add_argument("--c5-nt", type=int, default=10000)
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="also time this many shots migrated one at a time (the plain per-shot "
                          "path) for the batched/plain ratio; 0 = skip")
@@ -258,6 +353,10 @@ def main(argv=None) -> int:
     from ertm import device as dev
     from ertm.config import GpuConfig, ImagingConfig, SolverConfig
     from ertm.imaging import RTMImager
+
+    if args.c5_sweep:
+        c5_sweep([1024, 4096, 16384, 65536], args.c5_nt)
+        return 0
 
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4

@@ -554,6 +554,59 @@ static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int
 
 extern "C" {
 
+// hipFFT plans for image post-processing, one per shape, shared by every
+// stream (a plan is bound to a stream for the duration of one call).
+namespace {
+std::mutex g_post_m;
+std::map<std::pair<int, int>, hipfftHandle> g_post_plans;
+}  // namespace
+
+int ertm_postprocess(const ErtmPost* p, void* stream) {
+    if (!p || p->nz <= 0 || p->nx <= 0 || !p->num || !p->work || !p->out)
+        return fail(ERTM_EINVAL, "postprocess: bad arguments");
+    if ((p->div_ill && !p->ill) || (p->div_rill && !p->rill))
+        return fail(ERTM_EINVAL, "postprocess: normalisation array missing");
+    hipStream_t s = (hipStream_t)stream;
+    const int nz = p->nz, nx = p->nx;
+    const int64_t n = (int64_t)nz * nx;
+    double* a = p->work;          // n doubles: normalised image
+    double* c = p->work + 2 * n;  // 2n doubles: complex buffer (or the Laplacian)
+    launch_post_norm(p->num, p->ill, p->rill, n, p->eps, p->div_ill, p->div_rill, a, s);
+    LAUNCHCHK("post_norm");
+    const double* img = a;
+    int from_complex = 0;
+    if (p->laplacian) {
+        launch_post_lap(a, nz, nx, p->dx * p->dx, p->dz * p->dz, p->lowpass ? 1 : 0, c, s);
+        LAUNCHCHK("post_lap");
+        img = c;
+    }
+    if (p->lowpass) {
+        if (!p->laplacian) {
+            launch_post_to_complex(a, n, c, s);
+            LAUNCHCHK("post_to_complex");
+        }
+        std::lock_guard<std::mutex> lk(g_post_m);
+        hipfftHandle& plan = g_post_plans[std::make_pair(nz, nx)];
+        if (!plan && hipfftPlan2d(&plan, nz, nx, HIPFFT_Z2Z) != HIPFFT_SUCCESS) {
+            plan = 0;
+            return fail(ERTM_EHIP, "hipfftPlan2d(%d,%d) failed", nz, nx);
+        }
+        if (hipfftSetStream(plan, s) != HIPFFT_SUCCESS) return fail(ERTM_EHIP, "hipfftSetStream failed");
+        auto* z = (hipfftDoubleComplex*)c;
+        if (hipfftExecZ2Z(plan, z, z, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
+            return fail(ERTM_EHIP, "hipfftExecZ2Z forward failed");
+        launch_post_response(c, nz, nx, p->dx, p->dz, p->fraction, s);
+        LAUNCHCHK("post_response");
+        if (hipfftExecZ2Z(plan, z, z, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
+            return fail(ERTM_EHIP, "hipfftExecZ2Z backward failed");
+        img = c;
+        from_complex = 1;
+    }
+    launch_post_finish(img, from_complex, nz, nx, p->npml, p->mask_top, p->out, s);
+    LAUNCHCHK("post_finish");
+    return ERTM_OK;
+}
+
 int ertm_step_timing(int32_t enable) {
     std::lock_guard<std::mutex> lk(g_timer.m);
     if (enable) {

@@ -461,4 +461,113 @@ void launch_lowpass_finish(float* field, int nzp, int nxp, int64_t ld, const voi
                        nxp, ld, (const double2*)cbuf);
 }
 
+// ---------------------------------------------------------------------------
+// image post-processing on the device (ertm/imaging.py:326-331, :476-502):
+// normalisation, 5-point Laplacian with edge padding (ertm/numerics.py:398-404),
+// radial wavenumber low-pass (ertm/numerics.py:424-442), PML mask, float32.
+// All float64; normalisation, Laplacian and mask round exactly like NumPy,
+// the FFT differs from pocketfft in rounding only.
+// ---------------------------------------------------------------------------
+__global__ void k_post_norm(const double* __restrict__ num, const double* __restrict__ ill,
+                            const double* __restrict__ rill, int64_t n, double eps, int div_ill,
+                            int div_rill, double* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double v = num[q];
+    if (div_ill) v = v / (ill[q] + eps);
+    if (div_rill) v = v / (rill[q] + eps);
+    out[q] = v;
+}
+
+// Laplacian of a (nz, nx) float64 image; writes float64 (to_complex = 0) or
+// the real parts of a complex buffer (to_complex = 1)
+__global__ void k_post_lap(const double* __restrict__ a, int nz, int nx, double dxx, double dzz,
+                           int to_complex, double* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nz * nx) return;
+    const int i = (int)(q / nx), j = (int)(q - (int64_t)i * nx);
+    const double c = a[q];
+    const double xl = a[(int64_t)i * nx + (j > 0 ? j - 1 : 0)];
+    const double xr = a[(int64_t)i * nx + (j < nx - 1 ? j + 1 : nx - 1)];
+    const double zu = a[(int64_t)(i > 0 ? i - 1 : 0) * nx + j];
+    const double zd = a[(int64_t)(i < nz - 1 ? i + 1 : nz - 1) * nx + j];
+    const double c2 = 2.0 * c;
+    double hx = xr - c2;
+    hx = hx + xl;
+    double hz = zd - c2;
+    hz = hz + zu;
+    const double v = hx / dxx + hz / dzz;
+    if (to_complex) {
+        out[2 * q] = v;
+        out[2 * q + 1] = 0.0;
+    } else {
+        out[q] = v;
+    }
+}
+
+__global__ void k_post_to_complex(const double* __restrict__ a, int64_t n, double* __restrict__ c) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    c[2 * q] = a[q];
+    c[2 * q + 1] = 0.0;
+}
+
+__global__ void k_post_response(double2* __restrict__ c, int nz, int nx, double dx, double dz,
+                                double frac) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nz * nx) return;
+    const int i = (int)(q / nx), j = (int)(q - (int64_t)i * nx);
+    const double kz = fftfreq(i, nz, dz) / fmax(0.5 / dz, 1e-12);
+    const double kx = fftfreq(j, nx, dx) / fmax(0.5 / dx, 1e-12);
+    const double r = sqrt(kz * kz + kx * kx);
+    double resp = 1.0 - (r - frac) / 0.15;
+    resp = resp < 0.0 ? 0.0 : (resp > 1.0 ? 1.0 : resp);
+    c[q].x = c[q].x * resp;
+    c[q].y = c[q].y * resp;
+}
+
+// real part (complex input, divided by nz*nx) or the float64 image itself,
+// times the PML mask, to float32
+__global__ void k_post_finish(const double* __restrict__ src, int from_complex, int nz, int nx,
+                              int npml, int mask_top, float* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)nz * nx;
+    if (q >= n) return;
+    const int i = (int)(q / nx), j = (int)(q - (int64_t)i * nx);
+    const double v = from_complex ? src[2 * q] / (double)n : src[q];
+    const bool zero = j < npml || j >= nx - npml || i >= nz - npml || (mask_top && i < npml);
+    const double m = zero ? 0.0 : 1.0;
+    out[q] = (float)(v * m);
+}
+
+static dim3 grid_of(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+void launch_post_norm(const double* num, const double* ill, const double* rill, int64_t n,
+                      double eps, int div_ill, int div_rill, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_norm, grid_of(n), dim3(256), 0, s, num, ill, rill, n, eps, div_ill,
+                       div_rill, out);
+}
+
+void launch_post_lap(const double* a, int nz, int nx, double dxx, double dzz, int to_complex,
+                     double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_lap, grid_of((int64_t)nz * nx), dim3(256), 0, s, a, nz, nx, dxx, dzz,
+                       to_complex, out);
+}
+
+void launch_post_to_complex(const double* a, int64_t n, double* c, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_to_complex, grid_of(n), dim3(256), 0, s, a, n, c);
+}
+
+void launch_post_response(double* c, int nz, int nx, double dx, double dz, double frac,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_post_response, grid_of((int64_t)nz * nx), dim3(256), 0, s, (double2*)c,
+                       nz, nx, dx, dz, frac);
+}
+
+void launch_post_finish(const double* src, int from_complex, int nz, int nx, int npml,
+                        int mask_top, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_finish, grid_of((int64_t)nz * nx), dim3(256), 0, s, src,
+                       from_complex, nz, nx, npml, mask_top, out);
+}
+
 }  // namespace ertm

@@ -32,6 +32,15 @@ void launch_sample(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchAr
                    int max_samp, hipStream_t s);
 void launch_inject(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,
                    int max_cells, hipStream_t s);
+void launch_post_norm(const double* num, const double* ill, const double* rill, int64_t n,
+                      double eps, int div_ill, int div_rill, double* out, hipStream_t s);
+void launch_post_lap(const double* a, int nz, int nx, double dxx, double dzz, int to_complex,
+                     double* out, hipStream_t s);
+void launch_post_to_complex(const double* a, int64_t n, double* c, hipStream_t s);
+void launch_post_response(double* c, int nz, int nx, double dx, double dz, double frac,
+                          hipStream_t s);
+void launch_post_finish(const double* src, int from_complex, int nz, int nx, int npml,
+                        int mask_top, float* out, hipStream_t s);
 void launch_image_flush(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la,
                         int max_cells, hipStream_t s);
 void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,

@@ -78,6 +78,16 @@ class ErtmImage(C.Structure):
     ]
 
 
+class ErtmPost(C.Structure):
+    _fields_ = [
+        ("nz", C.c_int32), ("nx", C.c_int32), ("div_ill", C.c_int32), ("div_rill", C.c_int32),
+        ("laplacian", C.c_int32), ("lowpass", C.c_int32), ("npml", C.c_int32),
+        ("mask_top", C.c_int32), ("dx", C.c_double), ("dz", C.c_double), ("eps", C.c_double),
+        ("fraction", C.c_double), ("num", C.c_void_p), ("ill", C.c_void_p), ("rill", C.c_void_p),
+        ("work", C.c_void_p), ("out", C.c_void_p),
+    ]
+
+
 class ErtmRun(C.Structure):
     _fields_ = [
         ("phase", C.c_int32), ("t_begin", C.c_int32), ("t_end", C.c_int32), ("cur0", C.c_int32),
@@ -115,6 +125,7 @@ EXPORTS = [
                                   C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_void_p]),
     ("ertm_absmax", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("ertm_sync", C.c_int, [C.c_void_p]),
+    ("ertm_postprocess", C.c_int, [C.POINTER(ErtmPost), C.c_void_p]),
     ("ertm_step_timing", C.c_int, [C.c_int32]),
     ("ertm_step_timing_read", C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),

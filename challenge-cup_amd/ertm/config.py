@@ -169,6 +169,10 @@ class GpuConfig:
     image_ring     backward steps whose receiver fields a fused elastic batch
                    folds into the image accumulators in one pass (0 = fold
                    every step inside the step kernel).  Same results.
+    device_postprocess  normalise, Laplacian-filter, low-pass and mask the
+                   images on the device (float64; the FFT rounds differently
+                   from NumPy's, so images can differ from the host path in
+                   the last float32 bit) instead of on the host.
     """
 
     strict_cfl: bool = True
@@ -178,6 +182,7 @@ class GpuConfig:
     max_batch: int = 64
     kernel_variant: tuple | None = None
     image_ring: int = 16
+    device_postprocess: bool = False
 
     def validate(self) -> None:
         if self.device_mem_gb <= 0:

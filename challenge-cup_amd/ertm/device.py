@@ -567,6 +567,30 @@ def absmax(t) -> float:
     return float(out.item())
 
 
+def postprocess(num, ill, rill, *, dx: float, dz: float, eps: float, div_ill: bool,
+                div_rill: bool, laplacian: bool, lowpass: bool, fraction: float, npml: int,
+                mask_top: bool):
+    """Normalise, Laplacian, wavenumber low-pass and PML mask of one image on
+    the device (ertm_postprocess); returns a (nz, nx) float32 device tensor."""
+    torch = torch_mod()
+    nz, nx = num.shape
+    work = torch.empty(4 * nz * nx, dtype=torch.float64, device="cuda")
+    out = torch.empty((nz, nx), dtype=torch.float32, device="cuda")
+    p = L.ErtmPost()
+    p.nz, p.nx = int(nz), int(nx)
+    p.div_ill, p.div_rill = int(bool(div_ill)), int(bool(div_rill))
+    p.laplacian, p.lowpass = int(bool(laplacian)), int(bool(lowpass))
+    p.npml, p.mask_top = int(npml), int(bool(mask_top))
+    p.dx, p.dz, p.eps, p.fraction = float(dx), float(dz), float(eps), float(fraction)
+    p.num = L.ptr(num)
+    p.ill = L.ptr(ill) if div_ill else None
+    p.rill = L.ptr(rill) if div_rill else None
+    p.work = L.ptr(work)
+    p.out = L.ptr(out)
+    L.check(L.lib().ertm_postprocess(C.byref(p), L.stream_handle()), "ertm_postprocess")
+    return out
+
+
 class StepTimer:
     """HIP-event timing of every step-kernel launch while active."""
 

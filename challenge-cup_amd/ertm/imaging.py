@@ -395,10 +395,13 @@ class RTMImager:
                 # (NumPy releases the GIL in these array operations; every
                 # image is computed exactly as in a serial loop)
                 work = [(j, i, k) for j, i in enumerate(idx) for k in range(len(job.modes))]
-                with ThreadPoolExecutor(max_workers=_host_workers(len(work))) as pool:
-                    images = list(pool.map(
-                        lambda w: self._finish_image(imgs[w[0]], w[2], condition,
-                                                     job.models[w[1]], job.specs[w[1]]), work))
+                finish = lambda w: self._finish_image(  # noqa: E731
+                    imgs[w[0]], w[2], condition, job.models[w[1]], job.specs[w[1]])
+                if self.gpu_config.device_postprocess:
+                    images = [finish(w) for w in work]  # kernels on this thread's stream
+                else:
+                    with ThreadPoolExecutor(max_workers=_host_workers(len(work))) as pool:
+                        images = list(pool.map(finish, work))
                 for (j, i, k), image in zip(work, images):
                     m = job.modes[k]
                     res = ImageResult(image=image, grid=job.models[i].grid, mode=m,
@@ -581,10 +584,14 @@ class RTMImager:
             num_t, ill, rill = accs[j]
             backs[j].step_index = back_steps
             backs[j].pml.touched = True
-            # raw float64 accumulators on the host; normalised by _finish_image
-            all_outs.append(([x.cpu().numpy() for x in num_t],
-                             ill.cpu().numpy() if use_ill else None,
-                             [r.cpu().numpy() for r in rill] if use_rill else None))
+            if self.gpu_config.device_postprocess:
+                # accumulators stay in HBM; _finish_image runs ertm_postprocess
+                all_outs.append(("device", list(num_t), ill, list(rill)))
+            else:
+                # raw float64 accumulators on the host; normalised by _finish_image
+                all_outs.append(([x.cpu().numpy() for x in num_t],
+                                 ill.cpu().numpy() if use_ill else None,
+                                 [r.cpu().numpy() for r in rill] if use_rill else None))
             nz, nx = models[j].shape
             infos.append(dict(store=stores[j], start_step=starts[j], prop=fwds[j], back=backs[j],
                               replayed_steps=replayed, nt=nt,
@@ -653,6 +660,18 @@ class RTMImager:
                       spec: PropagationSpec) -> np.ndarray:
         """Normalise receiver wave k's accumulator and post-process it
         (ertm/imaging.py:326-349)."""
+        if raw[0] == "device":
+            _tag, nums, ill, rill = raw
+            out = dev.postprocess(
+                nums[k], ill, rill[k], dx=model.grid.dx, dz=model.grid.dz,
+                eps=float(self.config.eps),
+                div_ill=condition == "normalized" and bool(self.config.illumination),
+                div_rill=bool(self.config.receiver_normalization),
+                laplacian=bool(self.config.laplacian),
+                lowpass=self.config.lowpass_fraction < 1.0,
+                fraction=float(self.config.lowpass_fraction), npml=int(spec.npml),
+                mask_top=spec.boundary == "cpml")
+            return out.cpu().numpy()
         nums, ill, rill = raw
         eps = float(self.config.eps)
         image = nums[k]

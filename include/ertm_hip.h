@@ -321,6 +321,27 @@ int ertm_absmax(const float* x, int64_t n, float* out, void* stream);
 /* Block until all work on the stream is done. */
 int ertm_sync(void* stream);
 
+/* ---- image post-processing --------------------------------------------------
+ * RTMImager normalisation and _postprocess (ertm/imaging.py:326-331, :476-502:
+ * laplacian ertm/numerics.py:398-404, lowpass_wavenumber :424-442) on the
+ * device: image = num [/ (ill + eps)] [/ (rill + eps)], 5-point Laplacian
+ * with edge padding, radial wavenumber low-pass (hipFFT, float64), zero
+ * npml cells at the left/right/bottom (and top if mask_top), cast to float32.
+ * work: caller-owned device scratch of 4*nz*nx doubles. */
+typedef struct ErtmPost {
+    int32_t nz, nx;
+    int32_t div_ill, div_rill;   /* normalise by ill / rill */
+    int32_t laplacian, lowpass;  /* apply the Laplacian / the low-pass */
+    int32_t npml, mask_top;
+    double dx, dz, eps, fraction;
+    const double* num;
+    const double* ill;
+    const double* rill;
+    double* work;
+    float* out;                  /* (nz, nx) float32 image */
+} ErtmPost;
+int ertm_postprocess(const ErtmPost* p, void* stream);
+
 /* ---- measurement (no counterpart in the reference) ------------------------ */
 /* Step-kernel timing.  While enabled, each step launch of ertm_step / ertm_run
  * (one launch per group of up to ERTM_MAX_LAUNCH propagators) is bracketed by

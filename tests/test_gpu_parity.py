@@ -324,3 +324,44 @@ def test_fused_variant_large_random_model_matches_two_launch():
         outs.append(p._core.state[p._core.cur].clone())
     for o in outs[1:]:
         assert torch.equal(o, outs[0])
+
+
+# Device post-processing: normalisation, Laplacian and mask round exactly like
+# NumPy; only the float64 FFT of the low-pass differs from pocketfft, by a few
+# units in the last float64 place, so a float32 pixel may differ by one unit
+# in the last place.  Tolerance: |device - host| <= 2**-22 * max|host|.
+POST_TOL = 2.0 ** -22
+
+
+@pytest.mark.parametrize("tag", ["ac_zero_lag", "ac_normalized", "ac_raw_rn", "ac_cpml_zl",
+                                 "el_pp_divcurl", "el_ps_stress", "vi_noq"])
+def test_device_postprocess_matches_host(tag):
+    _say("device postprocess", tag)
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case(tag)
+    shot, model, imaging, solver, q = migrate_objects(c)
+    res = RTMImager(imaging, solver, q, GpuConfig(device_postprocess=True)).migrate(shot, model)
+    ref = c["image"]
+    scale = float(np.abs(ref).max())
+    assert scale > 0
+    diff = np.abs(res.image.astype(np.float64) - ref.astype(np.float64))
+    assert diff.max() <= POST_TOL * scale, (diff.max(), scale)
+    if imaging.lowpass_fraction >= 1.0:
+        assert np.array_equal(res.image, ref)
+
+
+def test_device_postprocess_without_lowpass_is_bit_exact():
+    _say("device postprocess no lowpass")
+    import dataclasses
+
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case("el_pp_divcurl")
+    shot, model, imaging, solver, q = migrate_objects(c)
+    imaging = dataclasses.replace(imaging, lowpass_fraction=1.0)
+    host = RTMImager(imaging, solver, q).migrate(shot, model)
+    devi = RTMImager(imaging, solver, q, GpuConfig(device_postprocess=True)).migrate(shot, model)
+    assert np.array_equal(host.image, devi.image)

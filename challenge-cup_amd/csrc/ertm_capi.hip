@@ -607,6 +607,23 @@ int ertm_postprocess(const ErtmPost* p, void* stream) {
     return ERTM_OK;
 }
 
+int ertm_prepare_record(float* rec, int32_t nt, int32_t nrec, const double* t,
+                        const double* t_mute, double taper, int32_t w_end, void* stream) {
+    if (!rec || nt <= 0 || nrec <= 0 || (t_mute && !t) || w_end < 0)
+        return fail(ERTM_EINVAL, "prepare_record: bad arguments");
+    launch_prep_mute_taper(rec, nt, nrec, t, t_mute, taper, w_end, (hipStream_t)stream);
+    LAUNCHCHK("prep_mute_taper");
+    return ERTM_OK;
+}
+
+int ertm_rotate_components(float* vx, float* vz, int64_t n, double cos_a, double sin_a,
+                           void* stream) {
+    if (!vx || !vz || n <= 0) return fail(ERTM_EINVAL, "rotate_components: bad arguments");
+    launch_prep_rotate(vx, vz, n, cos_a, sin_a, (hipStream_t)stream);
+    LAUNCHCHK("prep_rotate");
+    return ERTM_OK;
+}
+
 int ertm_step_timing(int32_t enable) {
     std::lock_guard<std::mutex> lk(g_timer.m);
     if (enable) {

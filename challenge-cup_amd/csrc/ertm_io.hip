@@ -570,4 +570,52 @@ void launch_post_finish(const double* src, int from_complex, int nz, int nx, int
                        from_complex, nz, nx, npml, mask_top, out);
 }
 
+// ---------------------------------------------------------------------------
+// record preparation on the device (ertm/preprocess.py:36-63, :121-189):
+// float64 arithmetic in the reference's order, float32 after each step
+// ---------------------------------------------------------------------------
+__global__ void k_prep_mute_taper(float* __restrict__ rec, int nt, int nrec,
+                                  const double* __restrict__ t, const double* __restrict__ t_mute,
+                                  double taper, int w_end) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nt * nrec) return;
+    const int it = (int)(q / nrec), ir = (int)(q - (int64_t)it * nrec);
+    double x = (double)rec[q];
+    if (t_mute) {
+        const double start = t_mute[ir] - taper, end = t_mute[ir] + taper;
+        double ramp = (t[it] - start) / (end - start);
+        ramp = ramp < 0.0 ? 0.0 : (ramp > 1.0 ? 1.0 : ramp);
+        const double w = 0.5 - 0.5 * cos(M_PI * ramp);
+        x = (double)(float)(x * w);
+    }
+    if (w_end > 0) {
+        // ramp[i] = 0.5 - 0.5 cos((pi * (i + 0.5)) / w), NumPy's evaluation order
+        if (it < w_end) x = x * (0.5 - 0.5 * cos((M_PI * ((double)it + 0.5)) / w_end));
+        if (it >= nt - w_end) {
+            const int j = nt - 1 - it;
+            x = x * (0.5 - 0.5 * cos((M_PI * ((double)j + 0.5)) / w_end));
+        }
+    }
+    rec[q] = (float)x;
+}
+
+__global__ void k_prep_rotate(float* __restrict__ vx, float* __restrict__ vz, int64_t n, double c,
+                              double s) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double a = (double)vx[q], b = (double)vz[q];
+    vx[q] = (float)(c * a + s * b);
+    vz[q] = (float)(-s * a + c * b);
+}
+
+void launch_prep_mute_taper(float* rec, int nt, int nrec, const double* t, const double* t_mute,
+                            double taper, int w_end, hipStream_t s) {
+    hipLaunchKernelGGL(k_prep_mute_taper, grid_of((int64_t)nt * nrec), dim3(256), 0, s, rec, nt,
+                       nrec, t, t_mute, taper, w_end);
+}
+
+void launch_prep_rotate(float* vx, float* vz, int64_t n, double c, double s_, hipStream_t s) {
+    hipLaunchKernelGGL(k_prep_rotate, grid_of(n), dim3(256), 0, s, vx, vz, n, c, s_);
+}
+
 }  // namespace ertm

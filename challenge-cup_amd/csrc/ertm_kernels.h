@@ -41,6 +41,9 @@ void launch_post_response(double* c, int nz, int nx, double dx, double dz, doubl
                           hipStream_t s);
 void launch_post_finish(const double* src, int from_complex, int nz, int nx, int npml,
                         int mask_top, float* out, hipStream_t s);
+void launch_prep_mute_taper(float* rec, int nt, int nrec, const double* t, const double* t_mute,
+                            double taper, int w_end, hipStream_t s);
+void launch_prep_rotate(float* vx, float* vz, int64_t n, double c, double s_, hipStream_t s);
 void launch_image_flush(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la,
                         int max_cells, hipStream_t s);
 void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,

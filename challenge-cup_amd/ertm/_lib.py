@@ -126,6 +126,10 @@ EXPORTS = [
     ("ertm_absmax", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("ertm_sync", C.c_int, [C.c_void_p]),
     ("ertm_postprocess", C.c_int, [C.POINTER(ErtmPost), C.c_void_p]),
+    ("ertm_prepare_record", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_double, C.c_int32, C.c_void_p]),
+    ("ertm_rotate_components", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
+                                         C.c_double, C.c_void_p]),
     ("ertm_step_timing", C.c_int, [C.c_int32]),
     ("ertm_step_timing_read", C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),

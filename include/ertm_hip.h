@@ -342,6 +342,20 @@ typedef struct ErtmPost {
 } ErtmPost;
 int ertm_postprocess(const ErtmPost* p, void* stream);
 
+/* ---- record preparation ------------------------------------------------------
+ * On a device record (nt, nrec) float32, row-major: the direct-wave top mute
+ * (mute_direct_wave, ertm/preprocess.py:121-169: weight 0.5 - 0.5 cos(pi r),
+ * r rising linearly over t_mute -+ taper; t_mute NULL = no mute), then the
+ * end taper (taper_record_ends, :172-189; w_end = 0: none).  t: nt float64
+ * sample times, t_mute: nrec float64.  Float64 arithmetic, float32 after each
+ * step, as the reference. */
+int ertm_prepare_record(float* rec, int32_t nt, int32_t nrec, const double* t,
+                        const double* t_mute, double taper, int32_t w_end, void* stream);
+/* (vx, vz) rotation by an azimuth, cos/sin given (rotate_components,
+ * ertm/preprocess.py:36-63), in place on n floats each. */
+int ertm_rotate_components(float* vx, float* vz, int64_t n, double cos_a, double sin_a,
+                           void* stream);
+
 /* ---- measurement (no counterpart in the reference) ------------------------ */
 /* Step-kernel timing.  While enabled, each step launch of ertm_step / ertm_run
  * (one launch per group of up to ERTM_MAX_LAUNCH propagators) is bracketed by

@@ -338,6 +338,9 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain float32 copy kernel (out = in * 1) over this many "
                          "MiB, three times: the reference reading for HBM byte counters")
+    ap.add_argument("--kernel-variant", default=None,
+                    help="force the step-kernel launch shape 'variant,zc' (default: the tuner's "
+                         "choice for the shape)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -378,8 +381,10 @@ def main(argv=None) -> int:
 
     imaging = ImagingConfig(mode="pp", condition="normalized", separation="divcurl")
     solver = SolverConfig(nt=args.nt, dt=DT, order=8, boundary="free", engine="elastic", npml=30)
+    kv = tuple(int(v) for v in args.kernel_variant.split(",")) if args.kernel_variant else None
     imager = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False,
-                                                             replay_budget_gb=64.0))
+                                                             replay_budget_gb=64.0,
+                                                             kernel_variant=kv))
     t0 = time.perf_counter()
     job = imager.prepare_batch(shots, model, modes=("pp", "ps"))
     torch.cuda.synchronize()

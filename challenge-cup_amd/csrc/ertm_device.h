@@ -25,22 +25,30 @@ __device__ __forceinline__ gfloat* gp(float* p) { return (gfloat*)p; }
 __device__ __forceinline__ gcfloat* gp(const float* p) { return (gcfloat*)p; }
 
 // Correctly rounded float32 quotient s/h without a division instruction.
-// q0 = s * RN64(1/h) is within 2^-52 (relative) of s/h; one float64 Newton
-// step with the exact FMA residual r = s - q0*h brings q1 within ~2^-104.
-//   * normal results: with 24-bit significands s/h is never a float32
-//     rounding midpoint and stays >= ~2^-48 (relative) away from every
-//     midpoint, so rounding q1 (or even q0) to float32 gives RN32(s/h);
-//   * subnormal results: s/h can sit exactly on a midpoint; that midpoint has
-//     25 significant bits, so it is a float64 number and q1 equals it
-//     exactly, and the final rounding applies ties-to-even like the IEEE
-//     division.
-// Non-finite inputs keep the plain product (the residual would be NaN).
+// q0 = s * RN64(1/h) is within 2^-52 (relative) of s/h.
+//   * normal results: s/h is never a float32 rounding midpoint (a midpoint's
+//     significand is an odd 25-bit integer M, and M * odd(h) has more than 24
+//     bits, so it is no float32 s), and a non-midpoint quotient stays at
+//     least ~2^-50 (relative) away from every midpoint (s - m*h is a nonzero
+//     multiple of 2^(exponent(s) - 48)); so rounding q0 to float32 already
+//     gives RN32(s/h);
+//   * subnormal results (|q0| < 2^-125 covers them): s/h can sit exactly on
+//     a midpoint of the 2^-149 grid.  One float64 Newton step with the exact
+//     FMA residual r = s - q0*h brings q1 within ~2^-104 of s/h; a midpoint
+//     there is a float64 number, q1 equals it, and the final rounding applies
+//     ties-to-even like the IEEE division.  The step runs only when some lane
+//     of the wave needs it (the branch is on the wave's ballot).
+// Non-finite inputs give the plain product's rounding (inf, NaN).
 __device__ __forceinline__ float div_h(float s, float h, double rh) {
-    const double sd = (double)s;
-    const double q0 = sd * rh;
-    const double r = fma(-q0, (double)h, sd);
-    const double q1 = fma(r, rh, q0);
-    return __builtin_isfinite(q1) ? (float)q1 : (float)q0;
+    const double q0 = (double)s * rh;
+    float q = (float)q0;
+    const bool tiny = __builtin_fabs(q0) < 0x1p-125;  // false for NaN
+    if (__builtin_amdgcn_ballot_w64(tiny)) {
+        const double r = fma(-q0, (double)h, (double)s);
+        const double q1 = fma(r, rh, q0);
+        if (tiny) q = (float)q1;
+    }
+    return q;
 }
 
 // Half-stencil K = order/2.  Staggered first derivative, ertm/numerics.py:68-117:

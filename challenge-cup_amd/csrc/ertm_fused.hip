@@ -440,8 +440,11 @@ __device__ __forceinline__ void el_loop(const ElBlock& B, float* lds) {
         el_iter<K, NT, STRIP, true, IMG, D>(B, X, k, lds + par * 7 * NT);
 }
 
+// The plain step fits 96 VGPRs, i.e. 5 waves per SIMD instead of 4; the
+// imaging variants would spill there and keep 4.
 template <int K, int NT, int IMG>
-__global__ void __launch_bounds__(NT) k_elastic_fused(const ErtmProp* __restrict__ props,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(IMG == 0 ? 5 : 4)))
+k_elastic_fused(const ErtmProp* __restrict__ props,
                                                       const ErtmImage* __restrict__ imgs,
                                                       LaunchArgs la, int zc) {
     constexpr int TX = NT - 4 * K;

@@ -21,6 +21,8 @@ the kernels see exactly the reference's float32 inputs.
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
+import weakref
 
 import numpy as np
 
@@ -80,6 +82,36 @@ def padded_rows(a: np.ndarray, ld: int):
     return out
 
 
+try:
+    import xxhash as _xxhash
+except ImportError:  # pragma: no cover - xxhash ships with this image
+    _xxhash = None
+
+# Device copies of medium arrays shared by every propagator built on the same
+# values (the shots of one survey share one model): one HBM copy, and the
+# batched step kernel, seeing equal medium pointers, orders its blocks so the
+# shots' blocks over one tile run together and read the medium through L2.
+_MEDIA: "weakref.WeakValueDictionary" = weakref.WeakValueDictionary()
+
+
+def _digest(a: np.ndarray) -> bytes:
+    buf = np.ascontiguousarray(a).view(np.uint8).reshape(-1)
+    if _xxhash is not None:
+        return _xxhash.xxh3_128_digest(buf)
+    return hashlib.blake2b(buf, digest_size=16).digest()
+
+
+def shared_padded_rows(a: np.ndarray, ld: int):
+    """``padded_rows`` through a cache keyed by the array's bytes, shape and
+    the row pitch; the cached tensor lives while some propagator holds it."""
+    key = (_digest(a), a.shape, a.dtype.str, int(ld))
+    out = _MEDIA.get(key)
+    if out is None:
+        out = padded_rows(a, ld)
+        _MEDIA[key] = out
+    return out
+
+
 class Core:
     """One propagator's device state and its C descriptor."""
 
@@ -128,7 +160,7 @@ class Core:
         self.state_floats = off
         self.state = torch.zeros((2, self.state_floats), dtype=torch.float32, device="cuda")
         self.cur = 0
-        self.media = {k: (padded_rows(v, self.ld) if isinstance(v, np.ndarray) and v.ndim == 2
+        self.media = {k: (shared_padded_rows(v, self.ld) if isinstance(v, np.ndarray) and v.ndim == 2
                           else (to_device(v) if isinstance(v, np.ndarray) else v))
                       for k, v in media.items()}
         self.src = (-1, -1)

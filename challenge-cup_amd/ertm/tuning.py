@@ -36,15 +36,16 @@ def blocks(variant: int, zc: int, nzp: int, nxp: int, nbatch: int) -> int:
 
 def heuristic(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
               n_cu: int = 256) -> tuple[int, int]:
-    """128-thread fused blocks with the longest chunk (up to 24 rows) that
+    """128-thread fused blocks with the longest chunk (up to 48 rows) that
     still gives 16 blocks per CU: enough row streams keep loads in flight,
-    and alternating chunk directions let L2 absorb most halo re-reads
-    (measured on 64 shots of 610 x 1210: 24-row chunks); batches too small
+    and alternating chunk directions let L2 absorb part of the halo
+    re-reads (measured on 64 shots of 610 x 1210 sharing one medium: 48-row
+    chunks 646 us per step, 24-row 731 us, 96-row 734 us); batches too small
     to fill the chip use the two-launch kernels, whose 2-D tiles need no
     halo recompute."""
     if not fused_supported(engine, order):
         return (0, 64)
-    for zc in (24, 19, 16, 10):
+    for zc in (48, 40, 32, 24, 19, 16, 10):
         if blocks(2, zc, nzp, nxp, nbatch) >= 16 * n_cu:
             return (2, zc)
     if blocks(2, 10, nzp, nxp, nbatch) >= 4 * n_cu:

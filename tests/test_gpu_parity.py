@@ -365,3 +365,46 @@ def test_device_postprocess_without_lowpass_is_bit_exact():
     host = RTMImager(imaging, solver, q).migrate(shot, model)
     devi = RTMImager(imaging, solver, q, GpuConfig(device_postprocess=True)).migrate(shot, model)
     assert np.array_equal(host.image, devi.image)
+
+
+@pytest.mark.parametrize("kv", [(0, 64), (2, 19), (1, 64)])
+@pytest.mark.parametrize("boundary", ["cpml", "free"])
+def test_subnormal_quotients_match_oracle(kv, boundary):
+    """Derivative sums in the subnormal range, where s/h can be an exact
+    rounding midpoint (dx = 10: a sum of 5 * 2^-149 gives 2^-150), next to
+    normal-range cells in the same waves: every kernel path divides like
+    NumPy's float32 division (ties to even), checked against the oracle."""
+    _say("subnormal quotients", kv, boundary)
+    from ertm import device as dev
+    from ertm.propagation import PropagationSpec, build_propagator
+    from ertm.types import ElasticModel, GridSpec
+    from oracle import rtm as ortm
+
+    nz, nx, h = 45, 70, 10.0
+    rng = np.random.default_rng(7)
+    vp = rng.uniform(2500.0, 3500.0, (nz, nx)).astype(np.float32)
+    vs = (vp / 1.8).astype(np.float32)
+    rho = rng.uniform(2000.0, 2600.0, (nz, nx)).astype(np.float32)
+    m = ElasticModel(GridSpec(nx=nx, nz=nz, dx=h, dz=h), vp=vp, vs=vs, rho=rho,
+                     qp=np.full_like(vp, 100.0), qs=np.full_like(vp, 50.0))
+    dt = 5e-4
+    spec = PropagationSpec(dt=dt, nt=4, order=8, npml=10, boundary=boundary)
+    p = build_propagator(m, spec, engine="elastic")
+    p._batches["step"] = dev.Batch([p._core], variant=kv)
+    op = ortm.build("elastic", ortm.Model(nz, nx, h, h, vp, vs, rho, m.qp), dt, 4, 8, 10, boundary)
+    shape = op.vx.shape
+    tiny = np.float32(2.0 ** -149)
+    for name in ("vx", "vz", "sxx", "szz", "sxz"):
+        # small multiples of 2^-149 (subnormal sums, many exact midpoints)
+        # in most cells, normal-range values in a band of columns
+        f = rng.integers(-40, 41, shape).astype(np.float32) * tiny
+        f[:, 20:26] = rng.normal(0.0, 1e-3, (shape[0], 6)).astype(np.float32)
+        getattr(op, name)[...] = f
+        p.set_field(name, f)
+    for _ in range(4):
+        op.step(0.0)
+        p.step(0.0)
+    for name in ("vx", "vz", "sxx", "szz", "sxz"):
+        got, want = getattr(p, name), getattr(op, name)
+        assert np.array_equal(got, want), (name, int(np.sum(got != want)))
+    assert np.any((np.abs(op.sxx) > 0) & (np.abs(op.sxx) < 2.0 ** -126))

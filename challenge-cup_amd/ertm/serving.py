@@ -22,7 +22,12 @@ runs a stream of independent migration requests on one device:
   group fails while running, its members are re-run one by one, so a bad
   request fails only itself.  A shot's image does not depend on what else
   is running (each image equals the one ``migrate`` gives for that shot
-  alone).
+  alone);
+* **stacking as shots finish**: requests that name a stack (``stack``,
+  ``stack_index``) are folded into that stack's float64 sum as they finish
+  (``open_stack``), in stack order through a reorder buffer, so the stacked
+  image equals ``stack_images`` over the shots in stack order
+  (ertm/pipeline.py:133-161).
 
 Results carry arrival, start and finish times; ``stats`` reports shots/s and
 the p50/p99 latency.
@@ -44,8 +49,9 @@ import numpy as np
 from . import numerics as num
 from ._lib import EngineUnavailable
 from .config import GpuConfig, ImagingConfig, QConfig, SolverConfig
-from .exceptions import ERTMError, ImagingError
+from .exceptions import ERTMError, ImagingError, PipelineError
 from .imaging import RTMImager, checkpoint_interval
+from .pipeline import StackAccumulator
 from .propagation import PropagationSpec
 from .types import ElasticModel, ImageResult, ShotRecord
 
@@ -63,6 +69,8 @@ class ShotRequest:
     q: QConfig | None = None
     modes: tuple[str, ...] | None = None
     rid: Any = None
+    stack: Any = None       # key of the stack this shot belongs to (``open_stack``)
+    stack_index: int = 0    # its position in that stack's order
 
 
 @dataclass
@@ -197,15 +205,48 @@ class ServingLoop:
         self._open = True
         self._results: list[ShotResult] = []
         self._outstanding = 0
+        self._stacks: dict[Any, dict[str, Any]] = {}
         self._threads = [threading.Thread(target=self._worker, name=f"ertm-serve-{k}",
                                           daemon=True) for k in range(int(streams))]
         for t in self._threads:
             t.start()
 
     # ------------------------------------------------------------ public
+    def open_stack(self, key: Any, count: int, *, normalize: bool = True) -> Future:
+        """Stack the ``count`` requests submitted with ``stack=key`` (indices
+        0 .. count-1).  The Future resolves to {mode: ImageResult}, each equal
+        to ``stack_images`` of that mode's images in index order, or to a
+        PipelineError naming the shots that failed."""
+        if count < 1:
+            raise ValueError("a stack needs at least one shot")
+        fut: Future = Future()
+        with self._lock:
+            if key in self._stacks:
+                raise ValueError(f"stack {key!r} is already open")
+            self._stacks[key] = {"count": int(count), "normalize": bool(normalize), "accs": None,
+                                 "seen": set(), "errors": [], "future": fut}
+        return fut
+
     def submit(self, req: ShotRequest, arrival: float | None = None) -> Future:
         fut: Future = Future()
         arrival = time.perf_counter() if arrival is None else float(arrival)
+        if req.stack is not None:
+            with self._lock:
+                st = self._stacks.get(req.stack)
+                idx = int(req.stack_index)
+                if st is None:
+                    bad = "stack not open"
+                elif not 0 <= idx < st["count"]:
+                    bad = "index outside the stack"
+                elif idx in st["seen"]:
+                    bad = "index given twice"
+                else:
+                    bad = None
+                    st["seen"].add(idx)
+            if bad is not None:
+                self._finish(fut, ShotResult(req.rid, None, f"rejected: stack {req.stack!r}: {bad}",
+                                             arrival, finish=time.perf_counter()))
+                return fut
         try:
             imager = self._imager(req)
             job = imager.prepare_batch([req.shot], req.model, modes=req.modes)
@@ -214,13 +255,13 @@ class ServingLoop:
             raise
         except (ERTMError, ValueError) as exc:
             self._finish(fut, ShotResult(req.rid, None, f"rejected: {exc}", arrival,
-                                         finish=time.perf_counter()))
+                                         finish=time.perf_counter()), req)
             return fut
         if nbytes > self.budget:
             self._finish(fut, ShotResult(req.rid, None,
                                          f"rejected: needs {nbytes / _GIB:.2f} GiB of HBM, "
                                          f"budget {self.budget / _GIB:.2f} GiB", arrival,
-                                         finish=time.perf_counter(), hbm_bytes=nbytes))
+                                         finish=time.perf_counter(), hbm_bytes=nbytes), req)
             return fut
         key = _key(req, imager._batch_key(req.model, job.specs[0]))
         with self._lock:
@@ -286,10 +327,36 @@ class ServingLoop:
             self._groups.append(group)
             self._work.notify_all()
 
-    def _finish(self, fut: Future, res: ShotResult) -> None:
+    def _finish(self, fut: Future, res: ShotResult, req: ShotRequest | None = None) -> None:
         with self._lock:
             self._results.append(res)
+        if req is not None and req.stack is not None:
+            self._fold(req, res)
         fut.set_result(res)
+
+    def _fold(self, req: ShotRequest, res: ShotResult) -> None:
+        """Fold a finished stacked request into its stack; resolve the
+        stack's Future once all its shots are in."""
+        with self._lock:
+            st = self._stacks[req.stack]
+            if res.ok:
+                if st["accs"] is None:
+                    st["accs"] = {m: StackAccumulator(st["count"], normalize=st["normalize"])
+                                  for m in res.images}
+                for m, acc in st["accs"].items():
+                    acc.add(req.stack_index, res.images[m])
+            else:
+                st["errors"].append((req.stack_index, res.error))
+            st["received"] = st.get("received", 0) + 1
+            if st["received"] < st["count"]:
+                return
+            del self._stacks[req.stack]
+        fut = st["future"]
+        if st["errors"]:
+            shots = ", ".join(f"{i}: {e}" for i, e in sorted(st["errors"], key=lambda x: x[0]))
+            fut.set_exception(PipelineError(f"stack {req.stack!r}: failed shots {shots}"))
+        else:
+            fut.set_result({m: acc.result() for m, acc in st["accs"].items()})
 
     def _worker(self) -> None:
         import torch
@@ -307,7 +374,7 @@ class ServingLoop:
             with self._lock:
                 self._adm.release(group)
             for item, res in zip(group, outcome):
-                self._finish(item[5], res)
+                self._finish(item[5], res, item[0])
             with self._lock:
                 self._outstanding -= len(group)
                 self._admit()

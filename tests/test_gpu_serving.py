@@ -104,3 +104,32 @@ def test_admission_keeps_reserved_hbm_within_budget():
         t = r.start
         live = sum(q.hbm_bytes for q in res if q.start <= t < q.finish)
         assert live <= budget_gb * 1024 ** 3
+
+
+def test_stack_as_shots_finish_equals_stack_images():
+    """Shots of one stack finish in whatever order the loop runs them; the
+    stacked image equals stack_images over the shots alone, in stack order
+    (ertm/pipeline.py:133-161)."""
+    _say("serving stack")
+    from ertm.pipeline import stack_images
+    from ertm.serving import ServingLoop
+
+    base = make_request(11, engine="elastic", nz=52, nx=76, nt=400, rid="b")
+    reqs = []
+    for k in range(4):
+        src = dataclasses.replace(base.shot.source, x=base.shot.source.x + 30.0 * k)
+        reqs.append(dataclasses.replace(base, shot=dataclasses.replace(base.shot, source=src),
+                                        rid=f"st{k}", stack="line-1", stack_index=k))
+    others = [make_request(200 + k, rid=f"o{k}") for k in range(2)]
+    with ServingLoop(hbm_budget_gb=16.0, streams=2, max_group=2) as loop:
+        stack = loop.open_stack("line-1", len(reqs))
+        futs = [loop.submit(r) for r in (reqs[3], others[0], reqs[1], reqs[0], others[1], reqs[2])]
+        res = [f.result(timeout=600) for f in futs]
+        stacked = stack.result(timeout=600)
+        bad = loop.submit(dataclasses.replace(reqs[0], stack="nope")).result(timeout=60)
+    assert all(r.ok for r in res), [r.error for r in res if not r.ok]
+    assert not bad.ok and "stack not open" in bad.error
+    mode = base.imaging.mode
+    want = stack_images([_alone(r) for r in reqs])
+    assert np.array_equal(stacked[mode].image, want.image)
+    assert stacked[mode].meta["n_shots"] == 4

@@ -385,9 +385,21 @@ static bool ring_ok(ertm_batch_t b) {
     return true;
 }
 
-static int img_kind(int op) {
-    return op == ERTM_IMG_EXTRACT ? 1 : op == ERTM_IMG_CORRELATE ? 2
-         : op == ERTM_IMG_RING_STORE ? 3 : 0;
+// Fused imaging kind for the kernel (ertm_fused.hip): extraction and ring
+// stores use their no-S variants (fewer registers) unless some propagator of
+// the launch images the divcurl S field.
+static int img_kind(ertm_batch_t b, int op, int base, int n) {
+    bool s = false;
+    for (int k = base; k < base + n; ++k) {
+        const ErtmImage& I = b->img[k];
+        if (I.separation != ERTM_SEP_DIVCURL) continue;
+        if (op == ERTM_IMG_EXTRACT) s = s || I.swave == ERTM_WAVE_S;
+        else
+            for (int w = 0; w < I.nrw; ++w) s = s || I.rwave[w] == ERTM_WAVE_S;
+    }
+    if (op == ERTM_IMG_EXTRACT) return s ? 1 : 4;
+    if (op == ERTM_IMG_RING_STORE) return s ? 3 : 5;
+    return op == ERTM_IMG_CORRELATE ? 2 : 0;
 }
 
 static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t s) {
@@ -433,7 +445,7 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
         }
         bool fused = b->variant != 0 &&
                      launch_step_fused(b->dprops, b->props.data(), la, b->order, b->variant, b->zc,
-                                       b->dimg, fuse_img ? img_kind(img_op) : 0, s);
+                                       b->dimg, fuse_img ? img_kind(b, img_op, base, la.n) : 0, s);
         if (!fused) {
             if (fuse_img) return fail(ERTM_ESTATE, "step: fused imaging launch refused");
             launch_step(b->dprops, la, b->order, b->max_tiles, b->any_force, s);

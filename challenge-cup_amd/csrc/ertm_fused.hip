@@ -67,6 +67,22 @@ __device__ __forceinline__ void bstore64(rsrc_t r, int off, double v) {
 #define ERTM_FIMG_EXTRACT 1
 #define ERTM_FIMG_CORRELATE 2
 #define ERTM_FIMG_RING 3  // store the receiver-side fields into ring slot t mod R
+// Variants of EXTRACT and RING that never need the divcurl S field (Dz-vx -
+// Dx-vz, two derivatives the step does not compute otherwise): P-wave or
+// stress-separation imaging.  Dropping that code path lowers register use.
+#define ERTM_FIMG_EXTRACT_NOS 4
+#define ERTM_FIMG_RING_NOS 5
+
+// The imaging operation of an IMG value, and whether it may need the S field.
+template <int IMG>
+constexpr int img_op() {
+    return IMG == ERTM_FIMG_EXTRACT_NOS ? ERTM_FIMG_EXTRACT
+         : IMG == ERTM_FIMG_RING_NOS ? ERTM_FIMG_RING : IMG;
+}
+template <int IMG>
+constexpr bool img_may_s() {
+    return IMG != ERTM_FIMG_NONE && IMG != ERTM_FIMG_EXTRACT_NOS && IMG != ERTM_FIMG_RING_NOS;
+}
 
 // Per-block constants of the fused elastic step.
 struct ElBlock {
@@ -144,7 +160,7 @@ __device__ __forceinline__ void el_fetch(const ElBlock& B, ElState<K, QXL>& X, i
 // reach back to row k+K (2K+1 rows).  S imaging (Dz-vx, rows k-K .. k+K-1)
 // needs 2K+1 rows either way.
 template <int K, int IMG, int D>
-constexpr int qx_len() { return (IMG == ERTM_FIMG_NONE && D > 0) ? 2 * K : 2 * K + 1; }
+constexpr int qx_len() { return (!img_may_s<IMG>() && D > 0) ? 2 * K : 2 * K + 1; }
 
 // One row iteration.  STRESS = false: velocity only (the 2K prologue rows
 // of a chunk, whose stress rows lie outside the chunk).
@@ -357,15 +373,15 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, qx_len<K, I
                 fs = sxz;
             } else {
                 fp = ex_plain + ez_plain;
-                if (B.need_s) {
+                if (img_may_s<IMG>() && B.need_s) {
                     const float dzvx = dminus<K>(X.q_vx, K + XO - 1, D, B.cf, B.hz, B.rhz);
                     const float dxvz = dminus<K>(l_vz, tid, 1, B.cf, B.hx, B.rhx);
                     fs = dzvx - dxvz;
                 }
             }
-            if constexpr (IMG == ERTM_FIMG_EXTRACT) {
+            if constexpr (img_op<IMG>() == ERTM_FIMG_EXTRACT) {
                 bstore(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.swave == ERTM_WAVE_S ? fs : fp);
-            } else if constexpr (IMG == ERTM_FIMG_RING) {
+            } else if constexpr (img_op<IMG>() == ERTM_FIMG_RING) {
                 bstore(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.rw0 == ERTM_WAVE_S ? fs : fp);
                 if (B.nrw > 1)
                     bstore(B.rR, cell ? B.o_num1 + 4 * q : ERTM_NOWRITE,
@@ -440,10 +456,11 @@ __device__ __forceinline__ void el_loop(const ElBlock& B, float* lds) {
         el_iter<K, NT, STRIP, true, IMG, D>(B, X, k, lds + par * 7 * NT);
 }
 
-// The plain step fits 96 VGPRs, i.e. 5 waves per SIMD instead of 4; the
-// imaging variants would spill there and keep 4.
+// The plain step and the no-S imaging variants fit 96 VGPRs, i.e. 5 waves
+// per SIMD instead of 4; the variants with the S field would spill there and
+// keep 4.
 template <int K, int NT, int IMG>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(IMG == 0 ? 5 : 4)))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(img_may_s<IMG>() ? 4 : 5)))
 k_elastic_fused(const ErtmProp* __restrict__ props,
                                                       const ErtmImage* __restrict__ imgs,
                                                       LaunchArgs la, int zc) {
@@ -531,7 +548,7 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
         B.nxph = P.nx;
         B.jcol = (B.own && B.c >= P.pad && B.c < P.pad + P.nx) ? B.c - P.pad : -1;
         const int64_t ncell = (int64_t)P.nz * P.nx;
-        if (IMG == ERTM_FIMG_RING) {
+        if (img_op<IMG>() == ERTM_FIMG_RING) {
             // ring slot t mod R: nrw fields of ncell floats; o_num1 = second field
             B.rR = make_rsrc(I.rring + (int64_t)(t % I.ring_len) * I.nrw * ncell,
                              4 * ncell * I.nrw);
@@ -546,7 +563,7 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
         B.rw1 = I.rwave[1];
         B.illum = I.illum;
         B.cross = t >= I.start_step;
-        B.need_s = IMG == ERTM_FIMG_EXTRACT
+        B.need_s = img_op<IMG>() == ERTM_FIMG_EXTRACT
                        ? I.swave == ERTM_WAVE_S
                        : (I.rwave[0] == ERTM_WAVE_S || (I.nrw > 1 && I.rwave[1] == ERTM_WAVE_S));
         (void)ncell;
@@ -607,6 +624,8 @@ static bool launch_fused_img(const ErtmProp* dprops, const ErtmImage* dimg,
         case ERTM_FIMG_EXTRACT: launch_fused_k<K, NT, ERTM_FIMG_EXTRACT>(dprops, dimg, hprops, la, zc, s); return true;
         case ERTM_FIMG_CORRELATE: launch_fused_k<K, NT, ERTM_FIMG_CORRELATE>(dprops, dimg, hprops, la, zc, s); return true;
         case ERTM_FIMG_RING: launch_fused_k<K, NT, ERTM_FIMG_RING>(dprops, dimg, hprops, la, zc, s); return true;
+        case ERTM_FIMG_EXTRACT_NOS: launch_fused_k<K, NT, ERTM_FIMG_EXTRACT_NOS>(dprops, dimg, hprops, la, zc, s); return true;
+        case ERTM_FIMG_RING_NOS: launch_fused_k<K, NT, ERTM_FIMG_RING_NOS>(dprops, dimg, hprops, la, zc, s); return true;
         default: return false;
     }
 }

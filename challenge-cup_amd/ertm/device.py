@@ -101,14 +101,38 @@ def _digest(a: np.ndarray) -> bytes:
     return hashlib.blake2b(buf, digest_size=16).digest()
 
 
-def shared_padded_rows(a: np.ndarray, ld: int):
-    """``padded_rows`` through a cache keyed by the array's bytes, shape and
-    the row pitch; the cached tensor lives while some propagator holds it."""
-    key = (_digest(a), a.shape, a.dtype.str, int(ld))
+def shared_padded_rows(a: np.ndarray, ld: int, key=None):
+    """``padded_rows`` through a cache keyed by the array's bytes (or by
+    ``key``, a digest of what the array was computed from), shape and the row
+    pitch; the cached tensor lives while some propagator holds it."""
+    key = (_digest(a) if key is None else key, a.shape, a.dtype.str, int(ld))
     out = _MEDIA.get(key)
     if out is None:
         out = padded_rows(a, ld)
         _MEDIA[key] = out
+    return out
+
+
+def media_digest(*parts) -> tuple:
+    """Key of a medium: the engine, padding and the bytes of the model arrays
+    it is computed from."""
+    return tuple((_digest(p), p.shape, p.dtype.str) if isinstance(p, np.ndarray) else p
+                 for p in parts)
+
+
+_HOST_MEDIA: dict = {}
+_HOST_MEDIA_MAX = 4
+
+
+def host_media(key: tuple, build) -> dict:
+    """Host medium arrays computed once per key (a few recent models kept):
+    every shot of a survey builds its propagators on the same model."""
+    out = _HOST_MEDIA.get(key)
+    if out is None:
+        out = build()
+        if len(_HOST_MEDIA) >= _HOST_MEDIA_MAX:
+            _HOST_MEDIA.pop(next(iter(_HOST_MEDIA)))
+        _HOST_MEDIA[key] = out
     return out
 
 
@@ -117,7 +141,7 @@ class Core:
 
     def __init__(self, engine: str, nz: int, nx: int, dx: float, dz: float, x0: float, z0: float,
                  order: int, dt: float, npml: int, boundary: str, vmax: float, media: dict,
-                 nmech: int = 0):
+                 nmech: int = 0, media_key: tuple | None = None):
         torch = torch_mod()
         L.lib()
         self.engine = engine
@@ -160,7 +184,9 @@ class Core:
         self.state_floats = off
         self.state = torch.zeros((2, self.state_floats), dtype=torch.float32, device="cuda")
         self.cur = 0
-        self.media = {k: (shared_padded_rows(v, self.ld) if isinstance(v, np.ndarray) and v.ndim == 2
+        self.media = {k: (shared_padded_rows(v, self.ld, None if media_key is None
+                                             else (media_key, k))
+                          if isinstance(v, np.ndarray) and v.ndim == 2
                           else (to_device(v) if isinstance(v, np.ndarray) else v))
                       for k, v in media.items()}
         self.src = (-1, -1)

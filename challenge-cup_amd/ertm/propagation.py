@@ -194,11 +194,11 @@ class Propagator2D:
         self._batches: dict = {}
 
     # ------------------------------------------------------------ set-up
-    def _make_core(self, media: dict, nmech: int = 0) -> None:
+    def _make_core(self, media: dict, nmech: int = 0, media_key: tuple | None = None) -> None:
         vmax = float(np.max(self.model.vp))
         self._core = dev.Core(self._engine, self.grid.nz, self.grid.nx, self.dx, self.dz, self.x0,
                               self.z0, self.order, self.dt, self.spec.npml, self.spec.boundary,
-                              vmax, media, nmech)
+                              vmax, media, nmech, media_key)
         self.pml = CPMLState(self._core)
 
     def _staged(self, array: np.ndarray) -> np.ndarray:
@@ -441,11 +441,16 @@ class AcousticPropagator(Propagator2D):
 
     def __init__(self, model: ElasticModel, spec: PropagationSpec) -> None:
         super().__init__(model, spec)
-        rho = self._staged(model.rho)
-        vp = self._staged(model.vp)
-        kappa = (rho * vp * vp).astype(np.float32)
-        irho = (1.0 / rho).astype(np.float32)
-        self._make_core({"kappa": kappa, "irho": irho})
+
+        def build():
+            rho = self._staged(model.rho)
+            vp = self._staged(model.vp)
+            kappa = (rho * vp * vp).astype(np.float32)
+            irho = (1.0 / rho).astype(np.float32)
+            return {"kappa": kappa, "irho": irho}
+
+        key = dev.media_digest("acoustic", self.pad, model.vp, model.rho)
+        self._make_core(dev.host_media(key, build), media_key=key)
 
     def _field_names(self) -> tuple[str, ...]:
         return ("p", "vx", "vz")
@@ -555,13 +560,18 @@ class ElasticPropagator(Propagator2D):
 
     def __init__(self, model: ElasticModel, spec: PropagationSpec) -> None:
         super().__init__(model, spec)
-        rho = self._staged(model.rho)
-        vp = self._staged(model.vp)
-        vs = self._staged(model.vs)
-        mu = (rho * vs * vs).astype(np.float32)
-        lam = (rho * (vp * vp - 2.0 * vs * vs)).astype(np.float32)
-        irho = (1.0 / rho).astype(np.float32)
-        self._make_core({"lam": lam, "mu": mu, "irho": irho})
+
+        def build():
+            rho = self._staged(model.rho)
+            vp = self._staged(model.vp)
+            vs = self._staged(model.vs)
+            mu = (rho * vs * vs).astype(np.float32)
+            lam = (rho * (vp * vp - 2.0 * vs * vs)).astype(np.float32)
+            irho = (1.0 / rho).astype(np.float32)
+            return {"lam": lam, "mu": mu, "irho": irho}
+
+        key = dev.media_digest("elastic", self.pad, model.vp, model.vs, model.rho)
+        self._make_core(dev.host_media(key, build), media_key=key)
 
     def _field_names(self) -> tuple[str, ...]:
         return ("vx", "vz", "sxx", "szz", "sxz")

@@ -101,6 +101,7 @@ struct ertm_batch_s {
     bool any_force = false;
     bool all_visco = false;
     bool all_elastic = false;
+    bool all_acoustic = false;
     int variant = 0;
     int zc = 64;
     float* scratch = nullptr;  // per propagator: peak, scale
@@ -230,6 +231,7 @@ int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmIma
     b->order = props[0].order;
     b->all_visco = true;
     b->all_elastic = true;
+    b->all_acoustic = true;
     for (int k = 0; k < n; ++k) {
         int rc = check_prop(b->props[k], k);
         if (rc == ERTM_OK) rc = check_recv(b->recv[k], b->props[k], k);
@@ -248,6 +250,7 @@ int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmIma
             b->any_force = true;
         if (P.engine != ERTM_VISCO) b->all_visco = false;
         if (P.engine != ERTM_ELASTIC) b->all_elastic = false;
+        if (P.engine != ERTM_ACOUSTIC) b->all_acoustic = false;
     }
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t sp = up(sizeof(ErtmProp) * n), sr = up(sizeof(ErtmRecv) * n);
@@ -297,8 +300,8 @@ int ertm_batch_destroy(ertm_batch_t b) {
 int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc) {
     if (!b) return fail(ERTM_EINVAL, "set_variant: NULL batch");
     if (variant < 0 || variant > 3) return fail(ERTM_EINVAL, "unknown kernel variant %d", variant);
-    if (variant != 0 && (!b->all_elastic || b->order != 8))
-        return fail(ERTM_EINVAL, "fused variants need an all-elastic, order-8 batch");
+    if (variant != 0 && (!(b->all_elastic || b->all_acoustic) || b->order != 8))
+        return fail(ERTM_EINVAL, "fused variants need an all-elastic or all-acoustic, order-8 batch");
     if (zc < 8 || zc > 4096) return fail(ERTM_EINVAL, "z chunk %d out of [8, 4096]", zc);
     b->variant = variant;
     b->zc = zc;

@@ -600,10 +600,281 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused acoustic step (ertm/propagation.py:430-454): the same streaming
+// scheme as the elastic kernel, with one pressure field.
+//   velocity of row m = k + K*D: vx += (dt*bx) * Dx+ p ; vz += (dt*bz) * Dz+ p
+//   pressure of row k:           p += (dt*kappa) * (Dx- vx + Dz- vz)
+// Register queues: pressure rows k .. k+2K*D (Dz+ at row m and the old
+// pressure of row k), new vz rows k-K*D .. k+K*D (Dz- at row k) and new vx
+// rows k .. k+(K-1)*D (vx is only differentiated in x, at row k, K rows
+// after it was computed).  LDS rows per iteration: pressure and 1/rho of row m, new
+// vx of row k.  Per cell 32 B of HBM traffic: p, vx, vz read and written,
+// kappa and 1/rho read.
+// ---------------------------------------------------------------------------
+struct AcBlock {
+    rsrc_t rS, rT, rIR, rKA;
+    gcfloat* AZ;
+    gcfloat* BZ;
+    int nzp, nxp, z0, z1, c, c4, c4own, tid, ld4;
+    int fP, fVX, fVZ;
+    int px0, px1, pz0, pz1;
+    int wx, wzt, wzb, xrow4, jx4;
+    int src_iz, fs_row;
+    bool colv, velc, own, inx, src_col;
+    float dt, hx, hz, amp, axc, bxc;
+    double rhx, rhz;
+    float cf[4];
+};
+
+template <int K>
+struct AcState {
+    float q_p[2 * K + 1];   // pressure rows k + j*D
+    float q_vz[2 * K + 1];  // new vz; after the push q_vz[j] holds row k + (j - K)*D
+    float q_vx[K];          // new vx; before the push q_vx[j] holds row k + j*D
+    float ir_c;             // 1/rho carried from the previous row (see ac_iter)
+    float n_p, n_vx, n_vz, n_ir, n_ka;  // inputs of this row
+};
+
+// Inputs of iteration k: pressure row k+2K*D, old velocities of row
+// m = k+K*D, 1/rho row m+1 (down) / m (up), kappa row k.
+template <int K, int D>
+__device__ __forceinline__ void ac_fetch(const AcBlock& B, AcState<K>& X, int k) {
+    const int ld4 = B.ld4;
+    const int rp = (k + 2 * K * D) * ld4, rv = (k + K * D) * ld4;
+    const int ri = (k + K * D + (D > 0 ? 1 : 0)) * ld4;
+    X.n_p = bload(B.rS, B.c4 + (rp + B.fP));
+    X.n_vx = bload(B.rS, B.c4 + (rv + B.fVX));
+    X.n_vz = bload(B.rS, B.c4 + (rv + B.fVZ));
+    X.n_ir = bload(B.rIR, B.c4 + ri);
+    X.n_ka = bload(B.rKA, B.c4 + k * ld4);
+}
+
+template <int K, int NT, bool STRIP, bool PRES, int D>
+__device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K>& X, int k, float* L) {
+    constexpr int QP = 2 * K + 1;
+    const int tid = B.tid, ld4 = B.ld4, nzp = B.nzp;
+    const int m = k + D * K;
+    const float vx_old = X.n_vx, vz_old = X.n_vz, ka_k = X.n_ka;
+    const float ir_m = D > 0 ? X.ir_c : X.n_ir;
+    const float ir_m1 = D > 0 ? X.n_ir : X.ir_c;
+#pragma unroll
+    for (int q = 0; q < QP - 1; ++q) X.q_p[q] = X.q_p[q + 1];
+    X.q_p[QP - 1] = X.n_p;
+    ac_fetch<K, D>(B, X, k + D);
+
+    float* l_p = L;
+    float* l_ir = L + NT;
+    float* l_vx = L + 2 * NT;
+    l_p[tid] = X.q_p[K];
+    l_ir[tid] = ir_m;
+    if (PRES) l_vx[tid] = X.q_vx[0];  // row k
+    __syncthreads();
+
+    // ---- velocity of row m
+    const bool inz_m = m >= K && m < nzp - K;
+    const bool row_m = m >= B.z0 && m < B.z1;
+    const bool own_m = B.own && row_m;
+    float dx = dplus<K>(l_p, tid, 1, B.cf, B.hx, B.rhx);
+    float dz = dplus<K>(X.q_p, K, D, B.cf, B.hz, B.rhz);
+    dx = B.inx ? dx : 0.0f;
+    dz = inz_m ? dz : 0.0f;
+    if (STRIP) {
+        if (B.velc && m >= 0 && m < nzp) {
+            int pzo = -1;
+            if (m < B.wzt) pzo = m * ld4 + B.c4;
+            else if (B.wzb > 0 && m >= nzp - B.wzb) pzo = (m - nzp + B.wzt + B.wzb) * ld4 + B.c4;
+            const int pxo = B.jx4 >= 0 ? m * B.xrow4 + B.jx4 : -1;
+            float pn;
+            if (pxo >= 0) {
+                dx = cpml_r(dx, bload(B.rS, B.px0 + pxo), B.axc, B.bxc, pn);
+                bstore(B.rT, own_m ? B.px0 + pxo : ERTM_NOWRITE, pn);
+            }
+            if (pzo >= 0) {
+                const float az = B.AZ[m], bz = B.BZ[m];
+                dz = cpml_r(dz, bload(B.rS, B.pz0 + pzo), az, bz, pn);
+                bstore(B.rT, own_m ? B.pz0 + pzo : ERTM_NOWRITE, pn);
+            }
+        }
+    }
+    float irx, irz;
+    {
+        float sx = ir_m + l_ir[tid + 1];
+        irx = (B.c < B.nxp - 1) ? 0.5f * sx : ir_m;
+        float sz = ir_m + ir_m1;
+        irz = (m < nzp - 1) ? 0.5f * sz : ir_m;
+    }
+    const float bxv = B.dt * irx;
+    const float bzv = B.dt * irz;
+    const float ux = bxv * dx;
+    const float uz = bzv * dz;
+    const float vx_new = vx_old + ux;
+    const float vz_new = vz_old + uz;
+    bstore(B.rT, B.c4own + (row_m ? m * ld4 + B.fVX : ERTM_BAD), vx_new);
+    bstore(B.rT, B.c4own + (row_m ? m * ld4 + B.fVZ : ERTM_BAD), vz_new);
+#pragma unroll
+    for (int q = 0; q < K - 1; ++q) X.q_vx[q] = X.q_vx[q + 1];
+    X.q_vx[K - 1] = vx_new;
+#pragma unroll
+    for (int q = 0; q < 2 * K; ++q) X.q_vz[q] = X.q_vz[q + 1];
+    X.q_vz[2 * K] = vz_new;
+
+    // ---- pressure of row k
+    if (PRES) {
+        const bool inz_k = k >= K && k < nzp - K;
+        float dxv = dminus<K>(l_vx, tid, 1, B.cf, B.hx, B.rhx);
+        float dzv = dminus<K>(X.q_vz, K, D, B.cf, B.hz, B.rhz);
+        dxv = B.inx ? dxv : 0.0f;
+        dzv = inz_k ? dzv : 0.0f;
+        if (STRIP) {
+            if (B.own) {
+                int pzo = -1;
+                if (k < B.wzt) pzo = k * ld4 + B.c4;
+                else if (B.wzb > 0 && k >= nzp - B.wzb) pzo = (k - nzp + B.wzt + B.wzb) * ld4 + B.c4;
+                const int pxo = B.jx4 >= 0 ? k * B.xrow4 + B.jx4 : -1;
+                float pn;
+                if (pxo >= 0) {
+                    dxv = cpml_r(dxv, bload(B.rS, B.px1 + pxo), B.axc, B.bxc, pn);
+                    bstore(B.rT, B.px1 + pxo, pn);
+                }
+                if (pzo >= 0) {
+                    const float az = B.AZ[k], bz = B.BZ[k];
+                    dzv = cpml_r(dzv, bload(B.rS, B.pz1 + pzo), az, bz, pn);
+                    bstore(B.rT, B.pz1 + pzo, pn);
+                }
+            }
+        }
+        const float div = dxv + dzv;
+        const float dk = B.dt * ka_k;
+        const float inc = dk * div;
+        float p = X.q_p[0] + inc;
+        if (B.src_col && k == B.src_iz) p = p + B.amp;
+        if (k == B.fs_row) p = 0.0f;
+        bstore(B.rT, B.c4own + (k * ld4 + B.fP), p);
+    }
+    X.ir_c = D > 0 ? ir_m1 : ir_m;
+}
+
+template <int K, int NT, bool STRIP, int D>
+__device__ __forceinline__ void ac_loop(const AcBlock& B, float* lds) {
+    const int ld4 = B.ld4;
+    AcState<K> X;
+#pragma unroll
+    for (int q = 0; q < K; ++q) X.q_vx[q] = 0.0f;
+#pragma unroll
+    for (int q = 0; q <= 2 * K; ++q) X.q_vz[q] = 0.0f;
+    const int kbeg = D > 0 ? B.z0 - 2 * K : B.z1 - 1 + 2 * K;
+#pragma unroll
+    for (int q = 0; q < 2 * K; ++q)  // pressure rows kbeg + q*D -> slots 1..2K
+        X.q_p[q + 1] = bload(B.rS, B.c4 + ((kbeg + q * D) * ld4 + B.fP));
+    X.ir_c = bload(B.rIR, B.c4 + (kbeg + D * K + (D > 0 ? 0 : 1)) * ld4);
+    ac_fetch<K, D>(B, X, kbeg);
+    int k = kbeg;
+    int par = 0;
+    for (int p = 0; p < 2 * K; ++p, k += D, par ^= 1)
+        ac_iter<K, NT, STRIP, false, D>(B, X, k, lds + par * 3 * NT);
+    for (int p = B.z0; p < B.z1; ++p, k += D, par ^= 1)
+        ac_iter<K, NT, STRIP, true, D>(B, X, k, lds + par * 3 * NT);
+}
+
+template <int K, int NT>
+__global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restrict__ props,
+                                                       LaunchArgs la, int zc) {
+    constexpr int TX = NT - 4 * K;
+    const int lin = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int ys = lin / gridDim.x;
+    const int bid = lin - ys * gridDim.x;
+    const ErtmProp& P = props[la.base + ys];
+    const int nzp = P.nzp, nxp = P.nxp;
+    const int nbands = (nxp + TX - 1) / TX;
+    const int nchunks = (nzp + zc - 1) / zc;
+    if (bid >= nbands * nchunks) return;
+    const int band = bid / nchunks, chunk = bid - band * nchunks;
+    AcBlock B;
+    const int x0 = band * TX;
+    B.nzp = nzp;
+    B.nxp = nxp;
+    B.z0 = chunk * zc;
+    B.z1 = min(B.z0 + zc, nzp);
+    B.tid = threadIdx.x;
+    B.c = x0 - 2 * K + B.tid;
+    B.c4 = 4 * B.c;
+    B.colv = B.c >= 0 && B.c < nxp;
+    B.velc = B.colv && B.tid >= K && B.tid < NT - K;
+    B.own = B.colv && B.tid >= 2 * K && B.tid < NT - 2 * K;
+    B.c4own = B.own ? B.c4 : ERTM_BAD;
+    B.ld4 = 4 * P.ld;
+    const int fs4 = (int)(4 * P.fstride);
+    const int64_t sbytes = 4 * P.state_floats;
+    B.rS = make_rsrc(P.state[la.cur], sbytes);
+    B.rT = make_rsrc(P.state[la.cur ^ 1], sbytes);
+    const int64_t mbytes = (int64_t)nzp * P.ld * 4;
+    B.rIR = make_rsrc(P.irho, mbytes);
+    B.rKA = make_rsrc(P.kappa, mbytes);
+    B.fP = ERTM_AC_P * fs4;
+    B.fVX = ERTM_AC_VX * fs4;
+    B.fVZ = ERTM_AC_VZ * fs4;
+    B.AZ = gp(P.az);
+    B.BZ = gp(P.bz);
+    B.px0 = (int)(4 * P.psix[0]);
+    B.px1 = (int)(4 * P.psix[1]);
+    B.pz0 = (int)(4 * P.psiz[0]);
+    B.pz1 = (int)(4 * P.psiz[1]);
+    B.wx = P.wx;
+    B.wzt = P.wzt;
+    B.wzb = P.wzb;
+    B.xrow4 = 8 * P.wx;
+    B.src_iz = P.src_iz;
+    B.fs_row = P.free_surface ? P.pad : -1;
+    B.dt = P.dt;
+    B.hx = P.dx;
+    B.hz = P.dz;
+    B.rhx = P.rdx;
+    B.rhz = P.rdz;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) B.cf[q] = P.coef[q];
+    B.amp = la.amp[ys];
+    B.src_col = B.amp != 0.0f && B.c == P.src_ix;
+    B.inx = B.c >= K && B.c < nxp - K;
+    B.jx4 = -1;
+    if (B.wx > 0 && B.colv) {
+        if (B.c < B.wx) B.jx4 = B.c4;
+        else if (B.c >= nxp - B.wx) B.jx4 = 4 * (B.c - nxp + 2 * B.wx);
+    }
+    B.axc = B.colv ? gp(P.ax)[B.c] : 0.0f;
+    B.bxc = B.colv ? gp(P.bx)[B.c] : 1.0f;
+
+    __shared__ float lds[2 * 3 * NT];
+    const int cx0 = x0 - 2 * K, cx1 = x0 + TX + 2 * K;
+    const int rz0 = B.z0 - 2 * K, rz1 = B.z1 + 2 * K;
+    const bool strip = (B.wx > 0 && (cx0 < B.wx || cx1 > nxp - B.wx)) ||
+                       (B.wzt > 0 && rz0 < B.wzt) || (B.wzb > 0 && rz1 > nzp - B.wzb);
+    const bool up = (chunk & 1) != 0;
+    if (strip) {
+        if (up) ac_loop<K, NT, true, -1>(B, lds);
+        else ac_loop<K, NT, true, 1>(B, lds);
+    } else {
+        if (up) ac_loop<K, NT, false, -1>(B, lds);
+        else ac_loop<K, NT, false, 1>(B, lds);
+    }
+}
+
+template <int K, int NT>
+static void launch_acoustic_k(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
+                              int zc, hipStream_t s);
+
 template <int K, int NT>
 static int blocks_for(const ErtmProp& P, int zc) {
     constexpr int TX = NT - 4 * K;
     return ((P.nxp + TX - 1) / TX) * ((P.nzp + zc - 1) / zc);
+}
+
+template <int K, int NT>
+static void launch_acoustic_k(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
+                              int zc, hipStream_t s) {
+    int nb = 0;
+    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la.base + k], zc));
+    hipLaunchKernelGGL((k_acoustic_fused<K, NT>), dim3(nb, la.n), dim3(NT), 0, s, dprops, la, zc);
 }
 
 template <int K, int NT, int IMG>
@@ -644,6 +915,20 @@ bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const Lau
         const ErtmProp& P = hprops[la.base + k];
         if (4 * P.state_floats + 128 * (int64_t)P.ld >= ((int64_t)1 << 30)) return false;
     }
+    // one engine per launch: all elastic or all acoustic
+    const int engine = hprops[la.base].engine;
+    for (int k = 0; k < la.n; ++k)
+        if (hprops[la.base + k].engine != engine) return false;
+    if (engine == ERTM_ACOUSTIC) {
+        if (img != ERTM_FIMG_NONE) return false;
+        switch (variant) {
+            case 1: launch_acoustic_k<4, 256>(dprops, hprops, la, zc, s); return true;
+            case 2: launch_acoustic_k<4, 128>(dprops, hprops, la, zc, s); return true;
+            case 3: launch_acoustic_k<4, 64>(dprops, hprops, la, zc, s); return true;
+            default: return false;
+        }
+    }
+    if (engine != ERTM_ELASTIC) return false;
     switch (variant) {
         case 1: return launch_fused_img<4, 256>(dprops, dimg, hprops, la, zc, img, s);
         case 2: return launch_fused_img<4, 128>(dprops, dimg, hprops, la, zc, img, s);

@@ -513,7 +513,7 @@ class Batch:
             variant = (tuning.lookup(c0.engine, c0.order, max(c.nzp for c in cores),
                                      max(c.nxp for c in cores), n) if same else (0, 64))
         elif int(variant[0]) != 0 and not fusable:
-            variant = (0, 64)  # fused kernels exist for elastic order 8 only
+            variant = (0, 64)  # fused kernels exist for elastic and acoustic order 8 only
         self.set_variant(*variant)
 
     def set_variant(self, variant: int, zc: int = 64) -> None:

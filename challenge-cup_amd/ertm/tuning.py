@@ -26,7 +26,7 @@ CACHE_FILE = os.environ.get("ERTM_TUNE_CACHE", "")
 
 
 def fused_supported(engine: str, order: int) -> bool:
-    return engine == "elastic" and int(order) == 8
+    return engine in ("elastic", "acoustic") and int(order) == 8
 
 
 def blocks(variant: int, zc: int, nzp: int, nxp: int, nbatch: int) -> int:

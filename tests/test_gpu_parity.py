@@ -270,10 +270,11 @@ def test_acoustic_c1_shape_forward_matches_oracle():
 
 @pytest.mark.parametrize("variant,zc", [(1, 64), (1, 10), (2, 19), (3, 10), (3, 64), (2, 4096),
                                         (2, 8), (1, 9), (3, 13), (2, 20)])
-@pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force"])
+@pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force", "ac_cpml", "ac_free"])
 def test_fused_elastic_variants_bit_identical(tag, variant, zc):
-    """Every fused launch shape reproduces the two-launch kernels (and thus the
-    reference) bit for bit, across chunk seams, CPML strips and all sources."""
+    """Every fused launch shape (elastic and acoustic) reproduces the
+    two-launch kernels (and thus the reference) bit for bit, across chunk
+    seams, CPML strips, free surfaces and all sources."""
     _say("fused variant", tag, variant, zc)
     from ertm import device as dev
     from ertm.propagation import source_wavelet
@@ -294,10 +295,11 @@ def test_fused_elastic_variants_bit_identical(tag, variant, zc):
         assert np.array_equal(psi.get(name, np.zeros_like(ref)), ref), name
 
 
-def test_fused_variant_large_random_model_matches_two_launch():
-    """600 x 1200 elastic (config-2 shape), random medium, CPML on all sides:
-    fused (every variant) == two-launch after 60 steps."""
-    _say("fused vs two-launch C2 shape")
+@pytest.mark.parametrize("engine", ["elastic", "acoustic"])
+def test_fused_variant_large_random_model_matches_two_launch(engine):
+    """600 x 1200 (config-2 shape), random medium, CPML on all sides: fused
+    (every variant) == two-launch after 60 steps, elastic and acoustic."""
+    _say("fused vs two-launch C2 shape", engine)
     import torch
     from ertm import device as dev
     from ertm.propagation import PropagationSpec, build_propagator
@@ -315,9 +317,10 @@ def test_fused_variant_large_random_model_matches_two_launch():
     w = np.sin(np.arange(60) * 0.3).astype(np.float32) * 1e6
     outs = []
     for v in [(0, 64), (1, 64), (1, 10), (2, 28), (3, 127), (2, 19), (2, 8), (3, 33)]:
-        p = build_propagator(m, spec, engine="elastic")
+        p = build_propagator(m, spec, engine=engine)
         p.set_source(src)
         p._batches["step"] = dev.Batch([p._core], variant=v)
+        assert p._batches["step"].variant == v
         for s in range(60):
             p.step(float(w[s]))
         torch.cuda.synchronize()

@@ -344,6 +344,15 @@ int timer_event(hipEvent_t* out) {
 // allocation starting at num[0] (as ertm/device.py allocates them).
 static bool fused_image_ok(ertm_batch_t b, int op) {
     if (op != ERTM_IMG_EXTRACT && op != ERTM_IMG_CORRELATE) return false;
+    // the fused acoustic step extracts (its correlation goes through the ring)
+    if (b->all_acoustic && b->variant >= 1 && b->variant <= 3 && b->order == 8 &&
+        op == ERTM_IMG_EXTRACT) {
+        for (size_t k = 0; k < b->props.size(); ++k) {
+            const ErtmProp& P = b->props[k];
+            if (!b->img[k].rbuf || (int64_t)P.nz * P.nx * 4 >= ((int64_t)1 << 31) - 64) return false;
+        }
+        return true;
+    }
     if (b->variant < 1 || b->variant > 3 || !b->all_elastic || b->order != 8 || b->any_force)
         return false;
     for (size_t k = 0; k < b->props.size(); ++k) {
@@ -375,7 +384,8 @@ static bool fused_image_ok(ertm_batch_t b, int op) {
 #define ERTM_IMG_RING_STORE 100
 
 static bool ring_ok(ertm_batch_t b) {
-    if (b->variant < 1 || b->variant > 3 || !b->all_elastic || b->order != 8 || b->any_force)
+    if (b->variant < 1 || b->variant > 3 || !(b->all_elastic || b->all_acoustic) ||
+        b->order != 8 || b->any_force)
         return false;
     for (size_t k = 0; k < b->props.size(); ++k) {
         const ErtmImage& I = b->img[k];

@@ -625,6 +625,9 @@ struct AcBlock {
     float dt, hx, hz, amp, axc, bxc;
     double rhx, rhz;
     float cf[4];
+    // imaging (IMG != 0): replay slot or ring slot of this propagator
+    rsrc_t rR;
+    int pad, nzph, nxph, jcol, nrw, o_f1;
 };
 
 template <int K>
@@ -650,7 +653,10 @@ __device__ __forceinline__ void ac_fetch(const AcBlock& B, AcState<K>& X, int k)
     X.n_ka = bload(B.rKA, B.c4 + k * ld4);
 }
 
-template <int K, int NT, bool STRIP, bool PRES, int D>
+// IMG: ERTM_FIMG_NONE, ERTM_FIMG_EXTRACT (the new pressure into the replay
+// slot) or ERTM_FIMG_RING (into ring slot t mod R, every receiver field):
+// the acoustic image field is the pressure (k_image's wave_value).
+template <int K, int NT, bool STRIP, bool PRES, int IMG, int D>
 __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K>& X, int k, float* L) {
     constexpr int QP = 2 * K + 1;
     const int tid = B.tid, ld4 = B.ld4, nzp = B.nzp;
@@ -751,11 +757,19 @@ __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K>& X, int k, 
         if (B.src_col && k == B.src_iz) p = p + B.amp;
         if (k == B.fs_row) p = 0.0f;
         bstore(B.rT, B.c4own + (k * ld4 + B.fP), p);
+        if constexpr (IMG != ERTM_FIMG_NONE) {
+            const int i = k - B.pad;
+            const bool cell = B.own && B.jcol >= 0 && i >= 0 && i < B.nzph;
+            const int q4 = cell ? 4 * (i * B.nxph + B.jcol) : ERTM_NOWRITE;
+            bstore(B.rR, q4, p);
+            if (IMG == ERTM_FIMG_RING && B.nrw > 1)
+                bstore(B.rR, cell ? B.o_f1 + q4 : ERTM_NOWRITE, p);
+        }
     }
     X.ir_c = D > 0 ? ir_m1 : ir_m;
 }
 
-template <int K, int NT, bool STRIP, int D>
+template <int K, int NT, bool STRIP, int IMG, int D>
 __device__ __forceinline__ void ac_loop(const AcBlock& B, float* lds) {
     const int ld4 = B.ld4;
     AcState<K> X;
@@ -772,13 +786,14 @@ __device__ __forceinline__ void ac_loop(const AcBlock& B, float* lds) {
     int k = kbeg;
     int par = 0;
     for (int p = 0; p < 2 * K; ++p, k += D, par ^= 1)
-        ac_iter<K, NT, STRIP, false, D>(B, X, k, lds + par * 3 * NT);
+        ac_iter<K, NT, STRIP, false, IMG, D>(B, X, k, lds + par * 3 * NT);
     for (int p = B.z0; p < B.z1; ++p, k += D, par ^= 1)
-        ac_iter<K, NT, STRIP, true, D>(B, X, k, lds + par * 3 * NT);
+        ac_iter<K, NT, STRIP, true, IMG, D>(B, X, k, lds + par * 3 * NT);
 }
 
-template <int K, int NT>
+template <int K, int NT, int IMG>
 __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restrict__ props,
+                                                       const ErtmImage* __restrict__ imgs,
                                                        LaunchArgs la, int zc) {
     constexpr int TX = NT - 4 * K;
     const int lin = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
@@ -843,6 +858,22 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
     }
     B.axc = B.colv ? gp(P.ax)[B.c] : 0.0f;
     B.bxc = B.colv ? gp(P.bx)[B.c] : 1.0f;
+    if constexpr (IMG != ERTM_FIMG_NONE) {
+        const ErtmImage& I = imgs[la.base + ys];
+        const int t = la.t[ys];
+        const int64_t ncell = (int64_t)P.nz * P.nx;
+        B.pad = P.pad;
+        B.nzph = P.nz;
+        B.nxph = P.nx;
+        B.jcol = (B.own && B.c >= P.pad && B.c < P.pad + P.nx) ? B.c - P.pad : -1;
+        B.nrw = I.nrw;
+        B.o_f1 = (int)(4 * ncell);
+        if (IMG == ERTM_FIMG_RING)
+            B.rR = make_rsrc(I.rring + (int64_t)(t % I.ring_len) * I.nrw * ncell,
+                             4 * ncell * I.nrw);
+        else
+            B.rR = make_rsrc(I.rbuf + (int64_t)(t - I.t0) * ncell, 4 * ncell);
+    }
 
     __shared__ float lds[2 * 3 * NT];
     const int cx0 = x0 - 2 * K, cx1 = x0 + TX + 2 * K;
@@ -851,17 +882,18 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
                        (B.wzt > 0 && rz0 < B.wzt) || (B.wzb > 0 && rz1 > nzp - B.wzb);
     const bool up = (chunk & 1) != 0;
     if (strip) {
-        if (up) ac_loop<K, NT, true, -1>(B, lds);
-        else ac_loop<K, NT, true, 1>(B, lds);
+        if (up) ac_loop<K, NT, true, IMG, -1>(B, lds);
+        else ac_loop<K, NT, true, IMG, 1>(B, lds);
     } else {
-        if (up) ac_loop<K, NT, false, -1>(B, lds);
-        else ac_loop<K, NT, false, 1>(B, lds);
+        if (up) ac_loop<K, NT, false, IMG, -1>(B, lds);
+        else ac_loop<K, NT, false, IMG, 1>(B, lds);
     }
 }
 
 template <int K, int NT>
-static void launch_acoustic_k(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
-                              int zc, hipStream_t s);
+static bool launch_acoustic_k(const ErtmProp* dprops, const ErtmImage* dimg,
+                              const ErtmProp* hprops, const LaunchArgs& la, int zc, int img,
+                              hipStream_t s);
 
 template <int K, int NT>
 static int blocks_for(const ErtmProp& P, int zc) {
@@ -869,12 +901,27 @@ static int blocks_for(const ErtmProp& P, int zc) {
     return ((P.nxp + TX - 1) / TX) * ((P.nzp + zc - 1) / zc);
 }
 
+// img: any extraction kind -> pressure into the replay slot, any ring kind ->
+// pressure into the ring; the fused correlation is elastic only.
 template <int K, int NT>
-static void launch_acoustic_k(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
-                              int zc, hipStream_t s) {
+static bool launch_acoustic_k(const ErtmProp* dprops, const ErtmImage* dimg,
+                              const ErtmProp* hprops, const LaunchArgs& la, int zc, int img,
+                              hipStream_t s) {
     int nb = 0;
     for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la.base + k], zc));
-    hipLaunchKernelGGL((k_acoustic_fused<K, NT>), dim3(nb, la.n), dim3(NT), 0, s, dprops, la, zc);
+    const dim3 g(nb, la.n);
+    if (img == ERTM_FIMG_NONE)
+        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_NONE>), g, dim3(NT), 0, s, dprops,
+                           dimg, la, zc);
+    else if (img == ERTM_FIMG_EXTRACT || img == ERTM_FIMG_EXTRACT_NOS)
+        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_EXTRACT>), g, dim3(NT), 0, s, dprops,
+                           dimg, la, zc);
+    else if (img == ERTM_FIMG_RING || img == ERTM_FIMG_RING_NOS)
+        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_RING>), g, dim3(NT), 0, s, dprops,
+                           dimg, la, zc);
+    else
+        return false;
+    return true;
 }
 
 template <int K, int NT, int IMG>
@@ -920,11 +967,10 @@ bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const Lau
     for (int k = 0; k < la.n; ++k)
         if (hprops[la.base + k].engine != engine) return false;
     if (engine == ERTM_ACOUSTIC) {
-        if (img != ERTM_FIMG_NONE) return false;
         switch (variant) {
-            case 1: launch_acoustic_k<4, 256>(dprops, hprops, la, zc, s); return true;
-            case 2: launch_acoustic_k<4, 128>(dprops, hprops, la, zc, s); return true;
-            case 3: launch_acoustic_k<4, 64>(dprops, hprops, la, zc, s); return true;
+            case 1: return launch_acoustic_k<4, 256>(dprops, dimg, hprops, la, zc, img, s);
+            case 2: return launch_acoustic_k<4, 128>(dprops, dimg, hprops, la, zc, img, s);
+            case 3: return launch_acoustic_k<4, 64>(dprops, dimg, hprops, la, zc, img, s);
             default: return false;
         }
     }

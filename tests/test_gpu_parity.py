@@ -154,7 +154,8 @@ def test_nested_replay_is_bit_identical(tag):
 @pytest.mark.parametrize("kv,ring", [((2, 19), 16), ((1, 10), 0), ((3, 64), 7), ((2, 8), 1),
                                      ((2, 48), 0)])
 @pytest.mark.parametrize("tag", ["el_pp_divcurl", "el_ps_divcurl", "el_pp_stress", "el_ps_stress",
-                                 "el_sp_zl", "el_ss_exc", "ac_zero_lag"])
+                                 "el_sp_zl", "el_ss_exc", "ac_zero_lag", "ac_cpml_zl",
+                                 "ac_normalized", "ac_raw_rn", "ac_excitation"])
 def test_migrate_with_fused_step_imaging_matches_reference(tag, kv, ring):
     """Fused step kernels that extract and correlate inside the step, or store
     the receiver fields in a ring folded every `ring` steps, give the

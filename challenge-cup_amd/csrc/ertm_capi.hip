@@ -297,11 +297,14 @@ int ertm_batch_destroy(ertm_batch_t b) {
     return ERTM_OK;
 }
 
+static bool pressure_fused_ok(ertm_batch_t b);
+
 int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc) {
     if (!b) return fail(ERTM_EINVAL, "set_variant: NULL batch");
     if (variant < 0 || variant > 3) return fail(ERTM_EINVAL, "unknown kernel variant %d", variant);
-    if (variant != 0 && (!(b->all_elastic || b->all_acoustic) || b->order != 8))
-        return fail(ERTM_EINVAL, "fused variants need an all-elastic or all-acoustic, order-8 batch");
+    if (variant != 0 && !((b->all_elastic && b->order == 8) || pressure_fused_ok(b)))
+        return fail(ERTM_EINVAL, "fused variants need an order-8 batch of one engine: elastic, "
+                                 "acoustic, or viscoacoustic with 1..4 mechanisms");
     if (zc < 8 || zc > 4096) return fail(ERTM_EINVAL, "z chunk %d out of [8, 4096]", zc);
     b->variant = variant;
     b->zc = zc;
@@ -342,11 +345,22 @@ int timer_event(hipEvent_t* out) {
 // fused PD = 1 variant, no force_z source (the stored vz then differs from the
 // one the stencil sees), and every propagator's accumulators in one
 // allocation starting at num[0] (as ertm/device.py allocates them).
+// Does the fused pressure kernel (ertm_fused.hip: acoustic, or viscoacoustic
+// with 1..4 mechanisms, order 8) cover this batch?
+static bool pressure_fused_ok(ertm_batch_t b) {
+    if (b->order != 8 || b->props.empty()) return false;
+    if (b->all_acoustic) return true;
+    if (!b->all_visco) return false;
+    const int L = b->props[0].nmech;
+    for (const ErtmProp& P : b->props)
+        if (P.nmech != L) return false;
+    return L >= 1 && L <= 4;
+}
+
 static bool fused_image_ok(ertm_batch_t b, int op) {
     if (op != ERTM_IMG_EXTRACT && op != ERTM_IMG_CORRELATE) return false;
-    // the fused acoustic step extracts (its correlation goes through the ring)
-    if (b->all_acoustic && b->variant >= 1 && b->variant <= 3 && b->order == 8 &&
-        op == ERTM_IMG_EXTRACT) {
+    // the fused pressure step extracts (its correlation goes through the ring)
+    if (pressure_fused_ok(b) && b->variant >= 1 && b->variant <= 3 && op == ERTM_IMG_EXTRACT) {
         for (size_t k = 0; k < b->props.size(); ++k) {
             const ErtmProp& P = b->props[k];
             if (!b->img[k].rbuf || (int64_t)P.nz * P.nx * 4 >= ((int64_t)1 << 31) - 64) return false;
@@ -384,7 +398,7 @@ static bool fused_image_ok(ertm_batch_t b, int op) {
 #define ERTM_IMG_RING_STORE 100
 
 static bool ring_ok(ertm_batch_t b) {
-    if (b->variant < 1 || b->variant > 3 || !(b->all_elastic || b->all_acoustic) ||
+    if (b->variant < 1 || b->variant > 3 || !(b->all_elastic || pressure_fused_ok(b)) ||
         b->order != 8 || b->any_force)
         return false;
     for (size_t k = 0; k < b->props.size(); ++k) {

@@ -602,7 +602,10 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
 
 // ---------------------------------------------------------------------------
 // Fused acoustic step (ertm/propagation.py:430-454): the same streaming
-// scheme as the elastic kernel, with one pressure field.
+// scheme as the elastic kernel, with one pressure field.  With VL = L > 0
+// the same kernel is the viscoacoustic GSLS step (ertm/propagation.py:686-723,
+// L relaxation mechanisms): the velocity half is identical, the pressure half
+// updates eps and r_1..r_L of row k and sets p = K_R*eps + sum r_l.
 //   velocity of row m = k + K*D: vx += (dt*bx) * Dx+ p ; vz += (dt*bz) * Dz+ p
 //   pressure of row k:           p += (dt*kappa) * (Dx- vx + Dz- vz)
 // Register queues: pressure rows k .. k+2K*D (Dz+ at row m and the old
@@ -628,21 +631,31 @@ struct AcBlock {
     // imaging (IMG != 0): replay slot or ring slot of this propagator
     rsrc_t rR;
     int pad, nzph, nxph, jcol, nrw, o_f1;
+    // viscoacoustic (VL > 0): eps and r_l slabs, Q class per cell and the
+    // per-class tables (a = exp(-dt/tau) in float32, delta and tau in float64)
+    rsrc_t rQ;
+    int fEPS, fR0, fs4;
+    const __attribute__((address_space(1))) float* qa;
+    const __attribute__((address_space(1))) double* qd;
+    const __attribute__((address_space(1))) double* qt;
+    double dt64;
 };
 
-template <int K>
+template <int K, int VL>
 struct AcState {
     float q_p[2 * K + 1];   // pressure rows k + j*D
     float q_vz[2 * K + 1];  // new vz; after the push q_vz[j] holds row k + (j - K)*D
     float q_vx[K];          // new vx; before the push q_vx[j] holds row k + j*D
     float ir_c;             // 1/rho carried from the previous row (see ac_iter)
     float n_p, n_vx, n_vz, n_ir, n_ka;  // inputs of this row
+    float n_eps, n_r[VL > 0 ? VL : 1];  // viscoacoustic inputs of row k
+    int n_q;
 };
 
 // Inputs of iteration k: pressure row k+2K*D, old velocities of row
 // m = k+K*D, 1/rho row m+1 (down) / m (up), kappa row k.
-template <int K, int D>
-__device__ __forceinline__ void ac_fetch(const AcBlock& B, AcState<K>& X, int k) {
+template <int K, int VL, int D>
+__device__ __forceinline__ void ac_fetch(const AcBlock& B, AcState<K, VL>& X, int k) {
     const int ld4 = B.ld4;
     const int rp = (k + 2 * K * D) * ld4, rv = (k + K * D) * ld4;
     const int ri = (k + K * D + (D > 0 ? 1 : 0)) * ld4;
@@ -651,23 +664,38 @@ __device__ __forceinline__ void ac_fetch(const AcBlock& B, AcState<K>& X, int k)
     X.n_vz = bload(B.rS, B.c4 + (rv + B.fVZ));
     X.n_ir = bload(B.rIR, B.c4 + ri);
     X.n_ka = bload(B.rKA, B.c4 + k * ld4);
+    if constexpr (VL > 0) {
+        const int rk = k * ld4;
+        X.n_eps = bload(B.rS, B.c4 + (rk + B.fEPS));
+#pragma unroll
+        for (int l = 0; l < VL; ++l) X.n_r[l] = bload(B.rS, B.c4 + (rk + B.fR0 + l * B.fs4));
+        X.n_q = __builtin_bit_cast(int, bload(B.rQ, B.c4 + rk));
+    }
 }
 
 // IMG: ERTM_FIMG_NONE, ERTM_FIMG_EXTRACT (the new pressure into the replay
 // slot) or ERTM_FIMG_RING (into ring slot t mod R, every receiver field):
 // the acoustic image field is the pressure (k_image's wave_value).
-template <int K, int NT, bool STRIP, bool PRES, int IMG, int D>
-__device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K>& X, int k, float* L) {
+template <int K, int NT, bool STRIP, bool PRES, int IMG, int VL, int D>
+__device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K, VL>& X, int k, float* L) {
     constexpr int QP = 2 * K + 1;
     const int tid = B.tid, ld4 = B.ld4, nzp = B.nzp;
     const int m = k + D * K;
     const float vx_old = X.n_vx, vz_old = X.n_vz, ka_k = X.n_ka;
+    float eps_k = 0.0f, r_k[VL > 0 ? VL : 1];
+    int q_k = 0;
+    if constexpr (VL > 0) {
+        eps_k = X.n_eps;
+#pragma unroll
+        for (int l = 0; l < VL; ++l) r_k[l] = X.n_r[l];
+        q_k = X.n_q;
+    }
     const float ir_m = D > 0 ? X.ir_c : X.n_ir;
     const float ir_m1 = D > 0 ? X.n_ir : X.ir_c;
 #pragma unroll
     for (int q = 0; q < QP - 1; ++q) X.q_p[q] = X.q_p[q + 1];
     X.q_p[QP - 1] = X.n_p;
-    ac_fetch<K, D>(B, X, k + D);
+    ac_fetch<K, VL, D>(B, X, k + D);
 
     float* l_p = L;
     float* l_ir = L + NT;
@@ -751,11 +779,43 @@ __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K>& X, int k, 
             }
         }
         const float div = dxv + dzv;
-        const float dk = B.dt * ka_k;
-        const float inc = dk * div;
-        float p = X.q_p[0] + inc;
-        if (B.src_col && k == B.src_iz) p = p + B.amp;
-        if (k == B.fs_row) p = 0.0f;
+        float p;
+        if constexpr (VL == 0) {
+            const float dk = B.dt * ka_k;
+            const float inc = dk * div;
+            p = X.q_p[0] + inc;
+            if (B.src_col && k == B.src_iz) p = p + B.amp;
+            if (k == B.fs_row) p = 0.0f;
+        } else {
+            // eps += dt*div ; r_l = r_l*a_l + b_l*inc ; p = K_R*eps + sum r_l, with
+            // b_l = f32(K_R * delta * f32(1 - a) * tau / dt) in float64
+            const float inc = B.dt * div;
+            float eps = eps_k + inc;
+            float tot = 0.0f;
+            const int qb = (q_k < 0 ? 0 : q_k) * VL;  // lanes off the grid read class 0
+#pragma unroll
+            for (int l = 0; l < VL; ++l) {
+                const float a = B.qa[qb + l];
+                const double one_minus_a = (double)(1.0f - a);
+                double bd = (double)ka_k * B.qd[qb + l];
+                bd = bd * one_minus_a;
+                bd = bd * B.qt[qb + l];
+                bd = bd / B.dt64;
+                const float bl = (float)bd;
+                float r = r_k[l] * a;
+                const float bi = bl * inc;
+                r = r + bi;
+                bstore(B.rT, B.c4own + (k * ld4 + B.fR0 + l * B.fs4), r);
+                tot = (l == 0) ? r : (tot + r);
+            }
+            const float ke = ka_k * eps;
+            p = ke + tot;
+            if (B.src_col && k == B.src_iz) {
+                eps = eps + (float)((double)B.amp / (double)ka_k);
+                p = p + B.amp;
+            }
+            bstore(B.rT, B.c4own + (k * ld4 + B.fEPS), eps);
+        }
         bstore(B.rT, B.c4own + (k * ld4 + B.fP), p);
         if constexpr (IMG != ERTM_FIMG_NONE) {
             const int i = k - B.pad;
@@ -769,10 +829,10 @@ __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K>& X, int k, 
     X.ir_c = D > 0 ? ir_m1 : ir_m;
 }
 
-template <int K, int NT, bool STRIP, int IMG, int D>
+template <int K, int NT, bool STRIP, int IMG, int VL, int D>
 __device__ __forceinline__ void ac_loop(const AcBlock& B, float* lds) {
     const int ld4 = B.ld4;
-    AcState<K> X;
+    AcState<K, VL> X;
 #pragma unroll
     for (int q = 0; q < K; ++q) X.q_vx[q] = 0.0f;
 #pragma unroll
@@ -782,16 +842,16 @@ __device__ __forceinline__ void ac_loop(const AcBlock& B, float* lds) {
     for (int q = 0; q < 2 * K; ++q)  // pressure rows kbeg + q*D -> slots 1..2K
         X.q_p[q + 1] = bload(B.rS, B.c4 + ((kbeg + q * D) * ld4 + B.fP));
     X.ir_c = bload(B.rIR, B.c4 + (kbeg + D * K + (D > 0 ? 0 : 1)) * ld4);
-    ac_fetch<K, D>(B, X, kbeg);
+    ac_fetch<K, VL, D>(B, X, kbeg);
     int k = kbeg;
     int par = 0;
     for (int p = 0; p < 2 * K; ++p, k += D, par ^= 1)
-        ac_iter<K, NT, STRIP, false, IMG, D>(B, X, k, lds + par * 3 * NT);
+        ac_iter<K, NT, STRIP, false, IMG, VL, D>(B, X, k, lds + par * 3 * NT);
     for (int p = B.z0; p < B.z1; ++p, k += D, par ^= 1)
-        ac_iter<K, NT, STRIP, true, IMG, D>(B, X, k, lds + par * 3 * NT);
+        ac_iter<K, NT, STRIP, true, IMG, VL, D>(B, X, k, lds + par * 3 * NT);
 }
 
-template <int K, int NT, int IMG>
+template <int K, int NT, int IMG, int VL>
 __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restrict__ props,
                                                        const ErtmImage* __restrict__ imgs,
                                                        LaunchArgs la, int zc) {
@@ -829,6 +889,16 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
     B.fP = ERTM_AC_P * fs4;
     B.fVX = ERTM_AC_VX * fs4;
     B.fVZ = ERTM_AC_VZ * fs4;
+    if constexpr (VL > 0) {
+        B.fs4 = fs4;
+        B.fEPS = ERTM_VI_EPS * fs4;
+        B.fR0 = ERTM_VI_R0 * fs4;
+        B.rQ = make_rsrc(P.qidx, mbytes);
+        B.qa = (const __attribute__((address_space(1))) float*)P.q_a;
+        B.qd = (const __attribute__((address_space(1))) double*)P.q_delta;
+        B.qt = (const __attribute__((address_space(1))) double*)P.q_tau;
+        B.dt64 = P.dt64;
+    }
     B.AZ = gp(P.az);
     B.BZ = gp(P.bz);
     B.px0 = (int)(4 * P.psix[0]);
@@ -882,15 +952,15 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
                        (B.wzt > 0 && rz0 < B.wzt) || (B.wzb > 0 && rz1 > nzp - B.wzb);
     const bool up = (chunk & 1) != 0;
     if (strip) {
-        if (up) ac_loop<K, NT, true, IMG, -1>(B, lds);
-        else ac_loop<K, NT, true, IMG, 1>(B, lds);
+        if (up) ac_loop<K, NT, true, IMG, VL, -1>(B, lds);
+        else ac_loop<K, NT, true, IMG, VL, 1>(B, lds);
     } else {
-        if (up) ac_loop<K, NT, false, IMG, -1>(B, lds);
-        else ac_loop<K, NT, false, IMG, 1>(B, lds);
+        if (up) ac_loop<K, NT, false, IMG, VL, -1>(B, lds);
+        else ac_loop<K, NT, false, IMG, VL, 1>(B, lds);
     }
 }
 
-template <int K, int NT>
+template <int K, int NT, int VL>
 static bool launch_acoustic_k(const ErtmProp* dprops, const ErtmImage* dimg,
                               const ErtmProp* hprops, const LaunchArgs& la, int zc, int img,
                               hipStream_t s);
@@ -903,7 +973,7 @@ static int blocks_for(const ErtmProp& P, int zc) {
 
 // img: any extraction kind -> pressure into the replay slot, any ring kind ->
 // pressure into the ring; the fused correlation is elastic only.
-template <int K, int NT>
+template <int K, int NT, int VL>
 static bool launch_acoustic_k(const ErtmProp* dprops, const ErtmImage* dimg,
                               const ErtmProp* hprops, const LaunchArgs& la, int zc, int img,
                               hipStream_t s) {
@@ -911,17 +981,34 @@ static bool launch_acoustic_k(const ErtmProp* dprops, const ErtmImage* dimg,
     for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la.base + k], zc));
     const dim3 g(nb, la.n);
     if (img == ERTM_FIMG_NONE)
-        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_NONE>), g, dim3(NT), 0, s, dprops,
-                           dimg, la, zc);
+        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_NONE, VL>), g, dim3(NT), 0, s,
+                           dprops, dimg, la, zc);
     else if (img == ERTM_FIMG_EXTRACT || img == ERTM_FIMG_EXTRACT_NOS)
-        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_EXTRACT>), g, dim3(NT), 0, s, dprops,
-                           dimg, la, zc);
+        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_EXTRACT, VL>), g, dim3(NT), 0, s,
+                           dprops, dimg, la, zc);
     else if (img == ERTM_FIMG_RING || img == ERTM_FIMG_RING_NOS)
-        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_RING>), g, dim3(NT), 0, s, dprops,
-                           dimg, la, zc);
+        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_RING, VL>), g, dim3(NT), 0, s,
+                           dprops, dimg, la, zc);
     else
         return false;
     return true;
+}
+
+// Fused launch of a pressure engine: acoustic (VL = 0) or viscoacoustic with
+// L = 1..4 mechanisms (more fall back to the two-launch kernels).
+template <int NT>
+static bool launch_pressure(const ErtmProp* dprops, const ErtmImage* dimg, const ErtmProp* hprops,
+                            const LaunchArgs& la, int zc, int img, int engine, int nmech,
+                            hipStream_t s) {
+    if (engine == ERTM_ACOUSTIC)
+        return launch_acoustic_k<4, NT, 0>(dprops, dimg, hprops, la, zc, img, s);
+    switch (nmech) {
+        case 1: return launch_acoustic_k<4, NT, 1>(dprops, dimg, hprops, la, zc, img, s);
+        case 2: return launch_acoustic_k<4, NT, 2>(dprops, dimg, hprops, la, zc, img, s);
+        case 3: return launch_acoustic_k<4, NT, 3>(dprops, dimg, hprops, la, zc, img, s);
+        case 4: return launch_acoustic_k<4, NT, 4>(dprops, dimg, hprops, la, zc, img, s);
+        default: return false;
+    }
 }
 
 template <int K, int NT, int IMG>
@@ -962,15 +1049,17 @@ bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const Lau
         const ErtmProp& P = hprops[la.base + k];
         if (4 * P.state_floats + 128 * (int64_t)P.ld >= ((int64_t)1 << 30)) return false;
     }
-    // one engine per launch: all elastic or all acoustic
-    const int engine = hprops[la.base].engine;
+    // one engine (and mechanism count) per launch
+    const int engine = hprops[la.base].engine, nmech = hprops[la.base].nmech;
     for (int k = 0; k < la.n; ++k)
-        if (hprops[la.base + k].engine != engine) return false;
-    if (engine == ERTM_ACOUSTIC) {
+        if (hprops[la.base + k].engine != engine ||
+            (engine == ERTM_VISCO && hprops[la.base + k].nmech != nmech))
+            return false;
+    if (engine == ERTM_ACOUSTIC || engine == ERTM_VISCO) {
         switch (variant) {
-            case 1: return launch_acoustic_k<4, 256>(dprops, dimg, hprops, la, zc, img, s);
-            case 2: return launch_acoustic_k<4, 128>(dprops, dimg, hprops, la, zc, img, s);
-            case 3: return launch_acoustic_k<4, 64>(dprops, dimg, hprops, la, zc, img, s);
+            case 1: return launch_pressure<256>(dprops, dimg, hprops, la, zc, img, engine, nmech, s);
+            case 2: return launch_pressure<128>(dprops, dimg, hprops, la, zc, img, engine, nmech, s);
+            case 3: return launch_pressure<64>(dprops, dimg, hprops, la, zc, img, engine, nmech, s);
             default: return false;
         }
     }

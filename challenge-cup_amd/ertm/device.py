@@ -507,13 +507,15 @@ class Batch:
         from . import tuning
 
         c0 = cores[0]
-        same = all((c.engine, c.order) == (c0.engine, c0.order) for c in cores)
-        fusable = same and tuning.fused_supported(c0.engine, c0.order)
+        same = all((c.engine, c.order, c.nmech) == (c0.engine, c0.order, c0.nmech)
+                   for c in cores)
+        fusable = same and tuning.fused_supported(c0.engine, c0.order, c0.nmech)
         if variant is None:
             variant = (tuning.lookup(c0.engine, c0.order, max(c.nzp for c in cores),
-                                     max(c.nxp for c in cores), n) if same else (0, 64))
+                                     max(c.nxp for c in cores), n, c0.nmech)
+                       if fusable else (0, 64))
         elif int(variant[0]) != 0 and not fusable:
-            variant = (0, 64)  # fused kernels exist for elastic and acoustic order 8 only
+            variant = (0, 64)  # no single-launch kernel for this engine/order
         self.set_variant(*variant)
 
     def set_variant(self, variant: int, zc: int = 64) -> None:

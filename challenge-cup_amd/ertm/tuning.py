@@ -25,8 +25,12 @@ _lock = threading.Lock()
 CACHE_FILE = os.environ.get("ERTM_TUNE_CACHE", "")
 
 
-def fused_supported(engine: str, order: int) -> bool:
-    return engine in ("elastic", "acoustic") and int(order) == 8
+def fused_supported(engine: str, order: int, nmech: int = 0) -> bool:
+    """Single-launch kernels: order 8, elastic, acoustic, or viscoacoustic
+    with 1..4 relaxation mechanisms."""
+    if int(order) != 8:
+        return False
+    return engine in ("elastic", "acoustic") or (engine == "viscoacoustic" and 1 <= nmech <= 4)
 
 
 def blocks(variant: int, zc: int, nzp: int, nxp: int, nbatch: int) -> int:
@@ -35,7 +39,7 @@ def blocks(variant: int, zc: int, nzp: int, nxp: int, nbatch: int) -> int:
 
 
 def heuristic(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
-              n_cu: int = 256) -> tuple[int, int]:
+              n_cu: int = 256, nmech: int = 0) -> tuple[int, int]:
     """128-thread fused blocks with the longest chunk (up to 48 rows) that
     still gives 16 blocks per CU: enough row streams keep loads in flight,
     and alternating chunk directions let L2 absorb part of the halo
@@ -43,7 +47,7 @@ def heuristic(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
     chunks 646 us per step, 24-row 731 us, 96-row 734 us); batches too small
     to fill the chip use the two-launch kernels, whose 2-D tiles need no
     halo recompute."""
-    if not fused_supported(engine, order):
+    if not fused_supported(engine, order, nmech):
         return (0, 64)
     for zc in (48, 40, 32, 24, 19, 16, 10):
         if blocks(2, zc, nzp, nxp, nbatch) >= 16 * n_cu:
@@ -57,7 +61,8 @@ def key(engine, order, nzp, nxp, nbatch) -> str:
     return f"{engine}:{order}:{nzp}x{nxp}:b{nbatch}"
 
 
-def lookup(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1) -> tuple[int, int]:
+def lookup(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
+           nmech: int = 0) -> tuple[int, int]:
     k = key(engine, order, nzp, nxp, nbatch)
     with _lock:
         if not _cache and CACHE_FILE and os.path.exists(CACHE_FILE):
@@ -67,7 +72,7 @@ def lookup(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1) -> tupl
                 pass
         if k in _cache:
             return _cache[k]
-    return heuristic(engine, order, nzp, nxp, nbatch)
+    return heuristic(engine, order, nzp, nxp, nbatch, nmech=nmech)
 
 
 def record(engine, order, nzp, nxp, nbatch, choice) -> None:
@@ -80,8 +85,8 @@ def record(engine, order, nzp, nxp, nbatch, choice) -> None:
                 pass
 
 
-def candidates(engine: str, order: int) -> list[tuple[int, int]]:
-    if not fused_supported(engine, order):
+def candidates(engine: str, order: int, nmech: int = 0) -> list[tuple[int, int]]:
+    if not fused_supported(engine, order, nmech):
         return [(0, 64)]
     return [(0, 64)] + [(v, zc) for v in (1, 2, 3)
                         for zc in (16, 24, 32, 48, 64, 96, 128, 160, 208, 320)]
@@ -94,7 +99,7 @@ def autotune(batch, steps: int = 10, cands=None) -> tuple[tuple[int, int], dict]
     import torch
 
     c0 = batch.cores[0]
-    cands = cands or candidates(c0.engine, c0.order)
+    cands = cands or candidates(c0.engine, c0.order, c0.nmech)
     times = {}
     amps = [0.0] * batch.n
     for cand in cands:

@@ -155,7 +155,8 @@ def test_nested_replay_is_bit_identical(tag):
                                      ((2, 48), 0)])
 @pytest.mark.parametrize("tag", ["el_pp_divcurl", "el_ps_divcurl", "el_pp_stress", "el_ps_stress",
                                  "el_sp_zl", "el_ss_exc", "ac_zero_lag", "ac_cpml_zl",
-                                 "ac_normalized", "ac_raw_rn", "ac_excitation"])
+                                 "ac_normalized", "ac_raw_rn", "ac_excitation", "vi_noq",
+                                 "vi_norm"])
 def test_migrate_with_fused_step_imaging_matches_reference(tag, kv, ring):
     """Fused step kernels that extract and correlate inside the step, or store
     the receiver fields in a ring folded every `ring` steps, give the
@@ -271,10 +272,11 @@ def test_acoustic_c1_shape_forward_matches_oracle():
 
 @pytest.mark.parametrize("variant,zc", [(1, 64), (1, 10), (2, 19), (3, 10), (3, 64), (2, 4096),
                                         (2, 8), (1, 9), (3, 13), (2, 20)])
-@pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force", "ac_cpml", "ac_free"])
+@pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force", "ac_cpml", "ac_free",
+                                 "vi_fwd"])
 def test_fused_elastic_variants_bit_identical(tag, variant, zc):
-    """Every fused launch shape (elastic and acoustic) reproduces the
-    two-launch kernels (and thus the reference) bit for bit, across chunk
+    """Every fused launch shape (elastic, acoustic, viscoacoustic) reproduces
+    the two-launch kernels (and thus the reference) bit for bit, across chunk
     seams, CPML strips, free surfaces and all sources."""
     _say("fused variant", tag, variant, zc)
     from ertm import device as dev
@@ -296,7 +298,7 @@ def test_fused_elastic_variants_bit_identical(tag, variant, zc):
         assert np.array_equal(psi.get(name, np.zeros_like(ref)), ref), name
 
 
-@pytest.mark.parametrize("engine", ["elastic", "acoustic"])
+@pytest.mark.parametrize("engine", ["elastic", "acoustic", "viscoacoustic"])
 def test_fused_variant_large_random_model_matches_two_launch(engine):
     """600 x 1200 (config-2 shape), random medium, CPML on all sides: fused
     (every variant) == two-launch after 60 steps, elastic and acoustic."""

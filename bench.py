@@ -338,6 +338,8 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain float32 copy kernel (out = in * 1) over this many "
                          "MiB, three times: the reference reading for HBM byte counters")
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="GpuConfig.graph_steps for the migrations (default: the config's)")
     ap.add_argument("--kernel-variant", default=None,
                     help="force the step-kernel launch shape 'variant,zc' (default: the tuner's "
                          "choice for the shape)")
@@ -382,9 +384,10 @@ def main(argv=None) -> int:
     imaging = ImagingConfig(mode="pp", condition="normalized", separation="divcurl")
     solver = SolverConfig(nt=args.nt, dt=DT, order=8, boundary="free", engine="elastic", npml=30)
     kv = tuple(int(v) for v in args.kernel_variant.split(",")) if args.kernel_variant else None
+    gkw = {} if args.graph_steps is None else {"graph_steps": args.graph_steps}
     imager = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False,
                                                              replay_budget_gb=64.0,
-                                                             kernel_variant=kv))
+                                                             kernel_variant=kv, **gkw))
     t0 = time.perf_counter()
     job = imager.prepare_batch(shots, model, modes=("pp", "ps"))
     torch.cuda.synchronize()
@@ -417,7 +420,9 @@ def main(argv=None) -> int:
     gcells = 3.0 * args.nt * NZ * NX * nshots / elapsed / 1e9
     avg_ms = kt["total_ms"] / max(kt["launches"], 1)
     cells_per_launch = kt["cell_updates"] / max(kt["launches"], 1)
-    achieved = BYTES_PER_CELL * cells_per_launch / (avg_ms * 1e-3) / 1e9
+    # step launches replayed from captured graphs are not timed one by one
+    achieved = (BYTES_PER_CELL * cells_per_launch / (avg_ms * 1e-3) / 1e9
+                if kt["launches"] > 0 else None)
     traffic = read_traffic()
     line = {
         "metric": "RTM shots/s & Gcell-updates/s (% HBM roofline) on 1 GPU; p50/p99 shot latency",
@@ -443,10 +448,10 @@ def main(argv=None) -> int:
         "roofline": {
             "kernel": "k_elastic_fused" if kt["fused"] == 1 else "k_velocity+k_stress",
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": None if achieved is None else round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "launches": kt["launches"],
             "avg_launch_us": round(avg_ms * 1e3, 2),

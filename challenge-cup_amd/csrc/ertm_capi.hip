@@ -109,12 +109,26 @@ struct ertm_batch_s {
     size_t block_bytes = 0;
     std::vector<hipStream_t> streams;  // streams this batch has enqueued work on
     std::map<std::pair<int, int>, FftCache> fft;
+    // captured time loops (ertm_run with graph_steps > 0): device clock
+    // {step, t0, dir}, amplitude table, checkpoint arena pointers, and the
+    // instantiated graphs keyed by everything their launches bake in
+    int32_t* dclk = nullptr;
+    float* damp = nullptr;
+    size_t damp_bytes = 0;
+    float** dark = nullptr;
+    std::map<std::vector<int64_t>, hipGraphExec_t> graphs;
+    hipStream_t cap = nullptr;  // private stream the loops are captured on
     void note(hipStream_t s) {
         for (hipStream_t t : streams)
             if (t == s) return;
         streams.push_back(s);
     }
     ~ertm_batch_s() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        if (cap) (void)hipStreamDestroy(cap);
+        pool_put(dclk, 64);
+        pool_put(damp, damp_bytes);
+        pool_put(dark, sizeof(float*) * props.size());
         pool_put(block, block_bytes);
         for (auto& kv : fft) {
             if (kv.second.plan) hipfftDestroy(kv.second.plan);
@@ -429,7 +443,27 @@ static int img_kind(ertm_batch_t b, int op, int base, int n) {
     return op == ERTM_IMG_CORRELATE ? 2 : 0;
 }
 
-static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t s) {
+// Launches recorded into a captured time loop read time and amplitude from
+// the batch's device clock (LaunchArgs::clk); qoff shifts the step they see.
+// clk[0] holds the first step of the chunk being replayed; the launches of
+// the chunk's step j see step clk[0] + j (q is j while recording).
+struct DevClock {
+    const int32_t* clk;
+    const float* ta;
+    int ntab;
+    int q;
+};
+
+static void use_clock(LaunchArgs& la, const DevClock* dc, int qoff = 0) {
+    if (!dc) return;
+    la.clk = dc->clk;
+    la.ta = dc->ta;
+    la.ntab = dc->ntab;
+    la.qoff = dc->q + qoff;
+}
+
+static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t s,
+                    const DevClock* dc = nullptr) {
     const int n = (int)b->props.size();
     for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
@@ -437,6 +471,7 @@ static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t 
         la.n = std::min(ERTM_MAX_LAUNCH, n - base);
         la.op = count;
         for (int k = 0; k < la.n; ++k) la.t[k] = t_hi[base + k];
+        use_clock(la, dc, -(count - 1));  // the group's first (latest) step
         launch_image_flush(b->dprops, b->dimg, la, b->max_img, s);
         LAUNCHCHK("image_flush");
     }
@@ -446,14 +481,16 @@ static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t 
 // One step; with img_op >= 0 and *imaged set on return, the step launches
 // also did that imaging op (t = time index per propagator).
 static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
-                   int img_op = -1, const int32_t* t = nullptr, bool* imaged = nullptr) {
+                   int img_op = -1, const int32_t* t = nullptr, bool* imaged = nullptr,
+                   const DevClock* dc = nullptr) {
     const int n = (int)b->props.size();
     std::unique_lock<std::mutex> lk(g_timer.m, std::defer_lock);
     const bool fuse_img = img_op >= 0 && t &&
                           (img_op == ERTM_IMG_RING_STORE ? ring_ok(b) : fused_image_ok(b, img_op));
     if (imaged) *imaged = fuse_img;
-    // only plain step launches are timed (the 52 B/cell algorithmic figure)
-    const bool timed = g_timer.on && !fuse_img;
+    // only plain step launches are timed (the 52 B/cell algorithmic figure);
+    // nothing is timed inside a captured loop
+    const bool timed = g_timer.on && !fuse_img && !dc;
     if (timed) lk.lock();
     for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
@@ -463,6 +500,7 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
         for (int k = 0; k < la.n; ++k) la.amp[k] = amp ? amp[base + k] : 0.0f;
         if (fuse_img)
             for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        use_clock(la, dc);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timed) {
             int rc = timer_event(&e0);
@@ -489,7 +527,8 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
     return ERTM_OK;
 }
 
-static int do_sample(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s) {
+static int do_sample(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s,
+                     const DevClock* dc = nullptr) {
     const int n = (int)b->props.size();
     for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
@@ -497,13 +536,15 @@ static int do_sample(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s) {
         la.n = std::min(ERTM_MAX_LAUNCH, n - base);
         la.cur = cur;
         for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        use_clock(la, dc);
         launch_sample(b->dprops, b->drecv, la, b->max_samp, s);
         LAUNCHCHK("sample");
     }
     return ERTM_OK;
 }
 
-static int do_inject(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s) {
+static int do_inject(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s,
+                     const DevClock* dc = nullptr) {
     const int n = (int)b->props.size();
     for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
@@ -511,13 +552,15 @@ static int do_inject(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s) {
         la.n = std::min(ERTM_MAX_LAUNCH, n - base);
         la.cur = cur;
         for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        use_clock(la, dc);
         launch_inject(b->dprops, b->drecv, la, b->max_cells, s);
         LAUNCHCHK("inject");
     }
     return ERTM_OK;
 }
 
-static int do_image(ertm_batch_t b, int cur, const int32_t* t, int op, hipStream_t s) {
+static int do_image(ertm_batch_t b, int cur, const int32_t* t, int op, hipStream_t s,
+                    const DevClock* dc = nullptr) {
     const int n = (int)b->props.size();
     for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
@@ -526,6 +569,7 @@ static int do_image(ertm_batch_t b, int cur, const int32_t* t, int op, hipStream
         la.cur = cur;
         la.op = op;
         for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        use_clock(la, dc);
         launch_image(b->dprops, b->dimg, la, b->order, b->max_img, s);
         LAUNCHCHK("image");
     }
@@ -760,6 +804,7 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
                         (long long)r->save_stride, (long long)sf, k);
     }
     if (r->op_after > ERTM_IMG_EXC_APPLY) return fail(ERTM_EINVAL, "run: bad image op");
+    if (r->graph_steps < 0) return fail(ERTM_EINVAL, "run: graph_steps must be >= 0");
     std::vector<float> amp(n, 0.0f);
     std::vector<int32_t> tt(n, 0);
     int cur = r->cur0 & 1;
@@ -778,40 +823,61 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
     const bool use_ring = r->phase == 2 && r->op_after == ERTM_IMG_CORRELATE && !r->stabilize &&
                           ring_ok(b);
     const int ring_len = use_ring ? b->img[0].ring_len : 0;
+    const int t0 = r->phase == 2 ? r->t_end - 1 : r->t_begin, dir = r->phase == 2 ? -1 : 1;
+    const int fop = use_ring ? ERTM_IMG_RING_STORE
+                             : ((!r->stabilize && r->op_after >= 0) ? r->op_after : -1);
     std::vector<int32_t> t_hi(n, 0);
     int pending = 0;
-    for (int q = 0; q < nsteps; ++q) {
-        const int t = r->phase == 2 ? r->t_end - 1 - q : r->t_begin + q;
+    int64_t max_sf = 0;
+    for (int k = 0; k < n; ++k) max_sf = std::max<int64_t>(max_sf, b->props[k].state_floats);
+
+    // One time step.  dc == nullptr: eager launches with host time and
+    // amplitude; dc set: launches recorded into a captured loop, which read
+    // both from the device clock, save checkpoints through k_checkpoint and
+    // end by advancing the clock.
+    auto one_step = [&](int q, const DevClock* dc, hipStream_t s) -> int {
+        const int t = t0 + dir * q;
         for (int k = 0; k < n; ++k) {
             tt[k] = t;
             amp[k] = r->phase == 2 ? 0.0f : r->wavelets[k][t];
         }
-        if (r->phase == 2) {
-            if ((rc = do_inject(b, cur, tt.data(), s)) != ERTM_OK) return rc;
-        }
+        int rc2;
+        if (r->phase == 2 && (rc2 = do_inject(b, cur, tt.data(), s, dc)) != ERTM_OK) return rc2;
         // imaging of the new state inside the step when nothing runs between
         // them that changes the state (the viscoacoustic stabiliser)
         bool imaged = false;
-        const int fop = use_ring ? ERTM_IMG_RING_STORE
-                                 : ((!r->stabilize && r->op_after >= 0) ? r->op_after : -1);
-        if ((rc = do_step(b, cur, amp.data(), s, fop, tt.data(), &imaged)) != ERTM_OK) return rc;
+        if ((rc2 = do_step(b, cur, amp.data(), s, fop, tt.data(), &imaged, dc)) != ERTM_OK)
+            return rc2;
         if (use_ring && !imaged) return fail(ERTM_ESTATE, "run: ring imaging launch refused");
         cur ^= 1;
         if (use_ring) {
             if (pending == 0) t_hi.assign(tt.begin(), tt.end());
-            if (++pending == ring_len || q == nsteps - 1) {
-                if ((rc = do_flush(b, t_hi.data(), pending, s)) != ERTM_OK) return rc;
+            if (++pending == ring_len || (!dc && q == nsteps - 1)) {
+                if ((rc2 = do_flush(b, t_hi.data(), pending, s, dc)) != ERTM_OK) return rc2;
                 pending = 0;
             }
         }
         if (r->stabilize) {
-            if ((rc = do_stabilize(b, cur, step_index, r->cap, r->lp_interval, r->ctl, s)) != ERTM_OK)
-                return rc;
+            if ((rc2 = do_stabilize(b, cur, step_index, r->cap, r->lp_interval, r->ctl, s)) !=
+                ERTM_OK)
+                return rc2;
         }
         ++step_index;
         if (r->phase != 2) {
-            if (r->do_sample && (rc = do_sample(b, cur, tt.data(), s)) != ERTM_OK) return rc;
-            if (r->ck_interval > 0 && t >= r->ck_origin && (t - r->ck_origin) % r->ck_interval == 0) {
+            if (r->do_sample && (rc2 = do_sample(b, cur, tt.data(), s, dc)) != ERTM_OK) return rc2;
+            if (r->ck_interval > 0 && dc) {
+                for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+                    LaunchArgs la{};
+                    la.base = base;
+                    la.n = std::min(ERTM_MAX_LAUNCH, n - base);
+                    la.cur = cur;
+                    use_clock(la, dc);
+                    launch_checkpoint(b->dprops, la, b->dark, r->save_stride, r->ck_origin,
+                                      r->ck_interval, max_sf, s);
+                    LAUNCHCHK("checkpoint");
+                }
+            } else if (r->ck_interval > 0 && t >= r->ck_origin &&
+                       (t - r->ck_origin) % r->ck_interval == 0) {
                 const int64_t slot = (t - r->ck_origin) / r->ck_interval;
                 for (int k = 0; k < n; ++k) {
                     const ErtmProp& P = b->props[k];
@@ -822,9 +888,109 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
             }
         }
         if (r->op_after >= 0 && !imaged) {
-            if ((rc = do_image(b, cur, tt.data(), r->op_after, s)) != ERTM_OK) return rc;
+            if ((rc2 = do_image(b, cur, tt.data(), r->op_after, s, dc)) != ERTM_OK) return rc2;
         }
+        return ERTM_OK;
+    };
+
+    // Captured loop: chunks of S steps replayed as one graph.  S is even (the
+    // chunk ends in the parity it began with) and a multiple of the ring
+    // length (every chunk flushes its own ring); the rest runs eagerly.
+    const int S = r->graph_steps;
+    bool graph = S > 0 && S % 2 == 0 && nsteps >= S && !r->stabilize &&
+                 (!use_ring || S % ring_len == 0);
+    if (graph && r->phase != 2 && r->ck_interval > 0) {
+        graph = r->save_stride % 4 == 0;  // k_checkpoint copies float4
+        for (int k = 0; k < n && graph; ++k)
+            graph = b->props[k].state_floats % 4 == 0 &&
+                    ((uintptr_t)b->props[k].state[0] | (uintptr_t)b->props[k].state[1] |
+                     (uintptr_t)r->save_arenas[k]) % 16 == 0;
     }
+    int q = 0;
+    if (graph) {
+        const int nchunks = nsteps / S;
+        if (!b->dclk && !(b->dclk = (int32_t*)pool_get(64)))
+            return fail(ERTM_EHIP, "run: device clock allocation failed");
+        if (!b->dark && !(b->dark = (float**)pool_get(sizeof(float*) * n)))
+            return fail(ERTM_EHIP, "run: arena table allocation failed");
+        const int32_t clk_h[4] = {0, t0, dir, 0};
+        HIPCHK(hipMemcpyAsync(b->dclk, clk_h, sizeof(clk_h), hipMemcpyHostToDevice, s));
+        std::vector<float> ta;
+        if (r->phase != 2) {
+            const size_t bytes = sizeof(float) * (size_t)nchunks * S * n;
+            if (bytes > b->damp_bytes) {
+                // the amplitude table is baked into the graphs: drop them
+                for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+                b->graphs.clear();
+                pool_put(b->damp, b->damp_bytes);
+                b->damp = (float*)pool_get(bytes);
+                b->damp_bytes = b->damp ? pool_class(bytes) : 0;
+                if (!b->damp) return fail(ERTM_EHIP, "run: amplitude table allocation failed");
+            }
+            ta.resize((size_t)nchunks * S * n);
+            for (int qq = 0; qq < nchunks * S; ++qq)
+                for (int k = 0; k < n; ++k) ta[(size_t)qq * n + k] = r->wavelets[k][t0 + dir * qq];
+            HIPCHK(hipMemcpyAsync(b->damp, ta.data(), sizeof(float) * ta.size(),
+                                  hipMemcpyHostToDevice, s));
+        }
+        if (r->phase != 2 && r->ck_interval > 0)
+            HIPCHK(hipMemcpyAsync(b->dark, r->save_arenas, sizeof(float*) * n,
+                                  hipMemcpyHostToDevice, s));
+        const DevClock dc{b->dclk, r->phase == 2 ? nullptr : b->damp, n, 0};
+        std::vector<int64_t> key = {r->phase, fop, use_ring, ring_len, r->do_sample, r->op_after,
+                                    r->ck_interval, r->ck_origin, r->save_stride, S, cur,
+                                    b->variant, b->zc, n};
+        for (const ErtmImage& I : b->img)
+            key.push_back((int64_t)I.separation | (int64_t)I.swave << 2 | (int64_t)I.nrw << 4 |
+                          (int64_t)I.rwave[0] << 8 | (int64_t)I.rwave[1] << 10 |
+                          (int64_t)I.illum << 12 | (int64_t)(I.rring != nullptr) << 16 |
+                          (int64_t)(I.rbuf != nullptr) << 17 | (int64_t)I.ring_len << 20);
+        auto it = b->graphs.find(key);
+        if (it == b->graphs.end()) {
+            if (b->graphs.size() >= 32) {  // keep the cache small
+                for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+                b->graphs.clear();
+            }
+            // recorded on a private stream (the caller's may be the null
+            // stream, which cannot be captured), replayed on the caller's
+            if (!b->cap) HIPCHK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+            // one capture at a time in the process
+            static std::mutex capture_m;
+            std::lock_guard<std::mutex> cap_lk(capture_m);
+            const int cur_save = cur, si_save = step_index;
+            HIPCHK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
+            for (int qq = 0; qq < S && rc == ERTM_OK; ++qq) {
+                DevClock dq = dc;
+                dq.q = qq;
+                rc = one_step(qq, &dq, b->cap);
+            }
+            if (rc == ERTM_OK) {  // the next replay starts S steps later
+                launch_tick(b->dclk, S, b->cap);
+                if (hipGetLastError() != hipSuccess) rc = fail(ERTM_EHIP, "run: tick launch failed");
+            }
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(b->cap, &g);
+            if (rc != ERTM_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ec != hipSuccess) return fail(ERTM_EHIP, "run: capture failed: %s", hipGetErrorString(ec));
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess)
+                return fail(ERTM_EHIP, "run: graph instantiation failed: %s", hipGetErrorString(ei));
+            it = b->graphs.emplace(key, ex).first;
+            cur = cur_save;  // capture recorded the steps; nothing has run yet
+            step_index = si_save;
+            pending = 0;
+        }
+        for (int c = 0; c < nchunks; ++c) HIPCHK(hipGraphLaunch(it->second, s));
+        q = nchunks * S;
+        step_index += q;  // S is even: the parity is back where it started
+    }
+    for (; q < nsteps; ++q)
+        if ((rc = one_step(q, nullptr, s)) != ERTM_OK) return rc;
     if (cur_out) *cur_out = cur;
     return ERTM_OK;
 }

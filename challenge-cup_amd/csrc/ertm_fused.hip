@@ -529,7 +529,7 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
     B.rhz = P.rdz;
 #pragma unroll
     for (int q = 0; q < 4; ++q) B.cf[q] = P.coef[q];
-    B.amp = la.amp[ys];
+    B.amp = la_amp(la, ys);
     B.src_col = B.amp != 0.0f && B.c == P.src_ix;
     B.inx = B.c >= K && B.c < nxp - K;
     B.jx4 = -1;
@@ -542,7 +542,7 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
 
     if constexpr (IMG != ERTM_FIMG_NONE) {
         const ErtmImage& I = imgs[slot];
-        const int t = la.t[ys];
+        const int t = la_time(la, ys);
         B.pad = P.pad;
         B.nzph = P.nz;
         B.nxph = P.nx;
@@ -918,7 +918,7 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
     B.rhz = P.rdz;
 #pragma unroll
     for (int q = 0; q < 4; ++q) B.cf[q] = P.coef[q];
-    B.amp = la.amp[ys];
+    B.amp = la_amp(la, ys);
     B.src_col = B.amp != 0.0f && B.c == P.src_ix;
     B.inx = B.c >= K && B.c < nxp - K;
     B.jx4 = -1;
@@ -930,7 +930,7 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
     B.bxc = B.colv ? gp(P.bx)[B.c] : 1.0f;
     if constexpr (IMG != ERTM_FIMG_NONE) {
         const ErtmImage& I = imgs[la.base + ys];
-        const int t = la.t[ys];
+        const int t = la_time(la, ys);
         const int64_t ncell = (int64_t)P.nz * P.nx;
         B.pad = P.pad;
         B.nzph = P.nz;

@@ -26,7 +26,7 @@ __global__ void k_sample(const ErtmProp* __restrict__ props, const ErtmRecv* __r
     if (e >= R.nsamp) return;
     const ErtmProp& P = props[slot];
     const float* S = P.state[la.cur];
-    const int t = la.t[blockIdx.y];
+    const int t = la_time(la, blockIdx.y);
     const int field = R.s_field[e];
     double v = 0.0;
     if (field != -1 && R.s_valid[e]) {
@@ -70,7 +70,7 @@ __global__ void k_inject(const ErtmProp* __restrict__ props, const ErtmRecv* __r
     if (u >= R.ncells) return;
     const ErtmProp& P = props[slot];
     float* S = P.state[la.cur];
-    const int t = la.t[blockIdx.y];
+    const int t = la_time(la, blockIdx.y);
     float* f = S + (int64_t)R.cell_field[u] * P.fstride + R.cell_off[u];
     float x = *f;
     const int e1 = R.cell_ptr[u + 1];
@@ -128,7 +128,7 @@ __global__ void k_image(const ErtmProp* __restrict__ props, const ErtmImage* __r
     const int i = (int)(q / P.nx), j = (int)(q - (int64_t)i * P.nx);
     const int64_t o = (int64_t)(i + P.pad) * P.ld + (j + P.pad);
     const float* S = P.state[la.cur];
-    const int t = la.t[blockIdx.y];
+    const int t = la_time(la, blockIdx.y);
     switch (la.op) {
         case ERTM_IMG_EXTRACT: {
             I.rbuf[(int64_t)(t - I.t0) * ncell + q] = wave_value<K>(P, S, I.separation, I.swave, o);
@@ -192,7 +192,7 @@ __global__ void k_image_flush(const ErtmProp* __restrict__ props,
     const int64_t ncell = (int64_t)P.nz * P.nx;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ncell) return;
-    const int t_hi = la.t[blockIdx.y], count = la.op;
+    const int t_hi = la_time(la, blockIdx.y), count = la.op;
     const bool src = (I.illum & ERTM_ILLUM_SRC) != 0, rec = (I.illum & ERTM_ILLUM_REC) != 0;
     const int nrw = I.nrw;
     double ill = src ? I.ill[q] : 0.0;
@@ -616,6 +616,46 @@ void launch_prep_mute_taper(float* rec, int nt, int nrec, const double* t, const
 
 void launch_prep_rotate(float* vx, float* vz, int64_t n, double c, double s_, hipStream_t s) {
     hipLaunchKernelGGL(k_prep_rotate, grid_of(n), dim3(256), 0, s, vx, vz, n, c, s_);
+}
+
+// ---------------------------------------------------------------------------
+// device clock of a captured time loop
+// ---------------------------------------------------------------------------
+__global__ void k_tick(int32_t* clk, int steps) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = clk[0] + steps;
+}
+
+void launch_tick(int32_t* clk, int steps, hipStream_t s) {
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, clk, steps);
+}
+
+// Checkpoint save whose slot depends on the device clock; blocks of
+// propagators whose time is no checkpoint time return at once.  States are
+// whole multiples of 64 floats and 256-byte aligned: float4 copies.
+__global__ void k_checkpoint(const ErtmProp* __restrict__ props, LaunchArgs la,
+                             float* const* __restrict__ arenas, int64_t stride, int origin,
+                             int interval) {
+    const int k = blockIdx.y;
+    const int t = la_time(la, k);
+    if (t < origin || (t - origin) % interval != 0) return;
+    const ErtmProp& P = props[la.base + k];
+    const int64_t slot = (t - origin) / interval;
+    const float4* src = (const float4*)P.state[la.cur];
+    float4* dst = (float4*)(arenas[la.base + k] + slot * stride);
+    const int64_t n4 = P.state_floats / 4;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_checkpoint(const ErtmProp* dprops, const LaunchArgs& la, float* const* darenas,
+                       int64_t stride, int origin, int interval, int64_t max_floats,
+                       hipStream_t s) {
+    // 64 blocks per propagator: enough to stream a due checkpoint, few to
+    // retire on the (far more common) steps where nothing is due
+    const int64_t blocks = std::min<int64_t>(64, (max_floats / 4 + 255) / 256);
+    hipLaunchKernelGGL(k_checkpoint, dim3((unsigned)std::max<int64_t>(blocks, 1), la.n), dim3(256),
+                       0, s, dprops, la, darenas, stride, origin, interval);
 }
 
 }  // namespace ertm

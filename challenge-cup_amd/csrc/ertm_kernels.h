@@ -8,6 +8,14 @@ namespace ertm {
 // Per-launch arguments: the launch covers propagators [base, base + n) of the
 // batch table; propagator base+k is at time index t[k] with source amplitude
 // amp[k]; states are read from state[cur] and written to state[cur ^ 1].
+//
+// Device clock (launches inside a captured time loop, replayed as a graph):
+// with clk != nullptr the time index and amplitude come from device memory,
+// so the same launch serves every step it is replayed for.  With
+// q = clk[0] + qoff (clk[0] is the chunk's first step, advanced by k_tick
+// after each replay; qoff the launch's step within the chunk), every
+// propagator is at time clk[1] + clk[2] * q and its amplitude is
+// ta[q * ntab + base + k] (0 without a table).
 struct LaunchArgs {
     int32_t base;
     int32_t n;
@@ -15,7 +23,21 @@ struct LaunchArgs {
     int32_t op;
     int32_t t[ERTM_MAX_LAUNCH];
     float amp[ERTM_MAX_LAUNCH];
+    const int32_t* clk;
+    const float* ta;
+    int32_t ntab;
+    int32_t qoff;
 };
+
+__device__ __forceinline__ int la_time(const LaunchArgs& la, int k) {
+    if (!la.clk) return la.t[k];
+    return la.clk[1] + la.clk[2] * (la.clk[0] + la.qoff);
+}
+
+__device__ __forceinline__ float la_amp(const LaunchArgs& la, int k) {
+    if (!la.clk) return la.amp[k];
+    return la.ta ? la.ta[(int64_t)(la.clk[0] + la.qoff) * la.ntab + la.base + k] : 0.0f;
+}
 
 struct DerivCoef {
     float c[4];
@@ -49,6 +71,14 @@ void launch_image_flush(const ErtmProp* dprops, const ErtmImage* dimg, const Lau
 void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,
                   int max_cells, hipStream_t s);
 void launch_absmax(const float* x, int64_t n, float* out, hipStream_t s);
+// device clock: advance clk[0] by `steps` (after a replayed chunk)
+void launch_tick(int32_t* clk, int steps, hipStream_t s);
+// checkpoint save inside a captured loop: state[cur] of every propagator of
+// the launch goes to arenas[k] + slot * stride when its time t satisfies
+// t >= origin and (t - origin) % interval == 0 (slot = (t - origin) / interval)
+void launch_checkpoint(const ErtmProp* dprops, const LaunchArgs& la, float* const* darenas,
+                       int64_t stride, int origin, int interval, int64_t max_floats,
+                       hipStream_t s);
 void launch_derivative(const void* f, void* out, int n0, int n1, int axis, int plus, int order,
                        const DerivCoef& c, double h, int is_f64, hipStream_t s);
 // Viscoacoustic stabiliser pieces (ertm/propagation.py:726-749).

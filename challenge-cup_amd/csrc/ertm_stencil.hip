@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(256) k_stress(const ErtmProp* __restrict__ pro
     const bool inz = (i >= K) && (i < P.nzp - K);
     const int64_t px = psix_index(P, i, j);
     const int64_t pz = psiz_index(P, i, j);
-    const float amp = la.amp[blockIdx.y];
+    const float amp = la_amp(la, blockIdx.y);
     const bool at_src = (amp != 0.0f) && (i == P.src_iz) && (j == P.src_ix);
 
     if (P.engine == ERTM_ELASTIC) {
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(256) k_stress(const ErtmProp* __restrict__ pro
 __global__ void k_point_source(const ErtmProp* __restrict__ props, LaunchArgs la) {
     const int slot = la.base + blockIdx.x;
     const ErtmProp& P = props[slot];
-    const float amp = la.amp[blockIdx.x];
+    const float amp = la_amp(la, blockIdx.x);
     if (threadIdx.x != 0 || amp == 0.0f || P.engine != ERTM_ELASTIC ||
         P.mechanism != ERTM_SRC_FORCE_Z || P.src_iz < 0)
         return;

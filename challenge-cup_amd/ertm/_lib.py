@@ -16,7 +16,7 @@ from .exceptions import ERTMError, SolverError
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libertm_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_LAUNCH = 64
 MAX_MECH = 8
 
@@ -97,7 +97,7 @@ class ErtmRun(C.Structure):
         ("restore_stride", C.c_int64), ("restore_arenas", C.POINTER(C.c_void_p)),
         ("wavelets", C.POINTER(C.c_void_p)),
         ("stabilize", C.c_int32), ("step_index0", C.c_int32), ("lp_interval", C.c_int32),
-        ("_pad1", C.c_int32), ("cap", C.c_double), ("ctl", C.c_void_p),
+        ("graph_steps", C.c_int32), ("cap", C.c_double), ("ctl", C.c_void_p),
     ]
 
 

@@ -490,7 +490,7 @@ class Batch:
     """A set of propagators advanced together (one ertm_batch_t)."""
 
     def __init__(self, cores: list, recvs: list | None = None, imgs: list | None = None,
-                 variant: tuple[int, int] | None = None):
+                 variant: tuple[int, int] | None = None, graph_steps: int = 0):
         lib = L.lib()
         n = len(cores)
         self.cores = list(cores)
@@ -504,6 +504,7 @@ class Batch:
                 "ertm_batch_create")
         self.h = h
         self.n = n
+        self.graph_steps = int(graph_steps)  # ertm_run: chunks replayed as one graph
         from . import tuning
 
         c0 = cores[0]
@@ -594,6 +595,7 @@ class Batch:
         r.stabilize = int(stabilize)
         r.step_index0 = int(step_index0)
         r.lp_interval = int(lp_interval)
+        r.graph_steps = self.graph_steps
         r.cap = float(cap)
         r.ctl = L.ptr(ctl)
         cur = C.c_int32(0)

@@ -153,7 +153,8 @@ def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
 
 
 def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarray], *,
-                             budget_mb: float, variant=None) -> list[CheckpointStore]:
+                             budget_mb: float, variant=None,
+                             graph_steps: int = 0) -> list[CheckpointStore]:
     """``record_source_wavefield`` for several shots advanced in one launch per step.
 
     All shots must share nt and the checkpoint interval (the interval follows
@@ -181,7 +182,7 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     curs = {c.cur for c in cores}
     if len(step0) != 1 or len(curs) != 1:
         raise ImagingError("batched propagators must be at the same step")
-    b = dev.Batch(cores, variant=variant)
+    b = dev.Batch(cores, variant=variant, graph_steps=graph_steps)
     ws = [np.asarray(w, dtype=np.float32) for w in wavelets]
     kw = batch_run_kwargs(props)
     cur = b.run(0, 0, nt, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
@@ -465,8 +466,9 @@ class RTMImager:
             fwd.set_receivers(shot.receivers)
             fwds.append(fwd)
         kv = self.gpu_config.kernel_variant
+        gsteps = int(self.gpu_config.graph_steps)
         stores = record_source_wavefields(fwds, waves, budget_mb=float(self.config.checkpoint_mb),
-                                          variant=kv)
+                                          variant=kv, graph_steps=gsteps)
         for model, spec, shot, record in zip(models, specs, shots, records):
             back = build_propagator(model, spec, engine=engine,
                                     q_options=self.q_options(compensation=True))
@@ -507,7 +509,7 @@ class RTMImager:
         if condition == "excitation":
             ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, "P", rwaves[:1], st,
                                     rec_illum, excitation=True) for m, st in zip(models, starts)]
-            fb = dev.Batch(fcores, imgs=ibs, variant=kv)
+            fb = dev.Batch(fcores, imgs=ibs, variant=kv, graph_steps=gsteps)
             for blk in range(nblocks):
                 s0, s1 = store0.bounds(blk, nt)
                 fcur = fb.run(1, s0, s1, fcur, wavelets=ws, op_after=L.IMG_EXC_PEAK,
@@ -515,7 +517,7 @@ class RTMImager:
                               restore_slot=store0.states[blk][1])
                 replayed += s1 - s0
             fb.close()
-            bb = dev.Batch(bcores, recvs=tabs, imgs=ibs, variant=kv)
+            bb = dev.Batch(bcores, recvs=tabs, imgs=ibs, variant=kv, graph_steps=gsteps)
             for blk in range(nblocks - 1, -1, -1):
                 s0, s1 = store0.bounds(blk, nt)
                 bcur = bb.run(2, s0, s1, bcur, op_after=L.IMG_EXC_APPLY,
@@ -534,8 +536,8 @@ class RTMImager:
                    for m, st, rb in zip(models, starts, rbufs)]
             fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, 0,
                                       rbuf=rb, nslots=slots) for m, rb in zip(models, rbufs)]
-            fb = dev.Batch(fcores, imgs=fimgs, variant=kv)
-            bb = dev.Batch(bcores, recvs=tabs, imgs=ibs, variant=kv)
+            fb = dev.Batch(fcores, imgs=fimgs, variant=kv, graph_steps=gsteps)
+            bb = dev.Batch(bcores, recvs=tabs, imgs=ibs, variant=kv, graph_steps=gsteps)
             subs = None
             if slots < store0.interval:
                 nsub = int(math.ceil(store0.interval / slots))

@@ -316,7 +316,10 @@ class ServingLoop:
 
     # ------------------------------------------------------------ internals
     def _imager(self, req: ShotRequest) -> RTMImager:
-        return RTMImager(req.imaging, req.solver, req.q, self.gpu_config)
+        # several worker threads drive the GPU at once: no captured time loops
+        # (another thread's work can invalidate a capture in progress)
+        gpu = dataclasses.replace(self.gpu_config, graph_steps=0)
+        return RTMImager(req.imaging, req.solver, req.q, gpu)
 
     def _admit(self) -> None:
         """Start every group that can start now (called with the lock held)."""

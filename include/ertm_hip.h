@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define ERTM_ABI_VERSION 2
+#define ERTM_ABI_VERSION 3
 
 enum { ERTM_OK = 0, ERTM_EINVAL = 1, ERTM_EHIP = 2, ERTM_ENOMEM = 3, ERTM_ESTATE = 4 };
 
@@ -305,7 +305,10 @@ typedef struct ErtmRun {
     int32_t stabilize;         /* viscoacoustic compensation stabiliser */
     int32_t step_index0;
     int32_t lp_interval;
-    int32_t _pad1;
+    int32_t graph_steps;       /* > 0: capture chunks of this many steps as one
+                                  hipGraph and replay it (even; a multiple of the
+                                  ring length when the ring images; not with the
+                                  stabiliser); the rest of the range runs eagerly */
     double cap;
     double* ctl;               /* device, 4 doubles per propagator */
 } ErtmRun;

@@ -414,3 +414,27 @@ def test_subnormal_quotients_match_oracle(kv, boundary):
         got, want = getattr(p, name), getattr(op, name)
         assert np.array_equal(got, want), (name, int(np.sum(got != want)))
     assert np.any((np.abs(op.sxx) > 0) & (np.abs(op.sxx) < 2.0 ** -126))
+
+
+@pytest.mark.parametrize("kv,ring,gsteps", [((2, 19), 16, 16), ((0, 64), 0, 4), ((1, 10), 4, 8),
+                                            ((3, 64), 0, 2)])
+@pytest.mark.parametrize("tag", ["el_pp_divcurl", "el_ps_divcurl", "el_sp_zl", "ac_zero_lag",
+                                 "ac_normalized", "ac_cpml_zl", "vi_noq"])
+def test_migrate_with_captured_time_loop_matches_reference(tag, kv, ring, gsteps):
+    """The time loops captured as hipGraphs (chunks of gsteps steps replayed
+    many times per pass, with eager tail steps, checkpoint restores and
+    segment boundaries between the replays) give the reference images bit
+    for bit, for fused and two-launch kernels, ring and per-step imaging."""
+    _say("captured loop", tag, kv, ring, gsteps)
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case(tag)
+    shot, model, imaging, solver, q = migrate_objects(c)
+    gpu = GpuConfig(kernel_variant=kv, image_ring=ring, graph_steps=gsteps)
+    imager = RTMImager(imaging, solver, q, gpu)
+    first = imager.migrate(shot, model)
+    assert np.array_equal(first.image, c["image"])
+    # the cached graphs are replayed again by a second migration
+    again = imager.migrate(shot, model)
+    assert np.array_equal(again.image, c["image"])

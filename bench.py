@@ -338,6 +338,8 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain float32 copy kernel (out = in * 1) over this many "
                          "MiB, three times: the reference reading for HBM byte counters")
+    ap.add_argument("--image-ring", type=int, default=None,
+                    help="GpuConfig.image_ring for the migrations (default: the config's)")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="GpuConfig.graph_steps for the migrations (default: the config's)")
     ap.add_argument("--kernel-variant", default=None,
@@ -385,6 +387,8 @@ def main(argv=None) -> int:
     solver = SolverConfig(nt=args.nt, dt=DT, order=8, boundary="free", engine="elastic", npml=30)
     kv = tuple(int(v) for v in args.kernel_variant.split(",")) if args.kernel_variant else None
     gkw = {} if args.graph_steps is None else {"graph_steps": args.graph_steps}
+    if args.image_ring is not None:
+        gkw["image_ring"] = args.image_ring
     imager = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False,
                                                              replay_budget_gb=64.0,
                                                              kernel_variant=kv, **gkw))

@@ -125,10 +125,12 @@ def request_bytes(req: ShotRequest, gpu: GpuConfig | None = None) -> int:
     sub = math.ceil(interval / slots) * state if slots < interval else 0
     nmodes = len(req.modes or (req.imaging.mode,))
     accum = (2 * nmodes + 1 + 3) * cells * 2                      # float64 -> 2 floats
+    # receiver-field ring of the backward pass (elastic and acoustic engines)
+    ring = int(gpu.image_ring) * nmodes * cells if engine in ("elastic", "acoustic") else 0
     nrec = int(req.shot.receivers.nrec)
     ncomp = max(1, len(req.shot.components))
     record = nt * nrec * ncomp + 16 * nrec * ncomp
-    floats = props + arena + slots * cells + sub + accum + record
+    floats = props + arena + slots * cells + sub + accum + ring + record
     return int(floats * 4 * 1.05) + (8 << 20)
 
 

@@ -358,6 +358,7 @@ def main(argv=None) -> int:
         print("bench.py needs a GPU (HIP device)", file=sys.stderr)
         return 1
     from ertm import device as dev
+    from ertm import tuning
     from ertm.config import GpuConfig, ImagingConfig, SolverConfig
     from ertm.imaging import RTMImager
 
@@ -460,6 +461,9 @@ def main(argv=None) -> int:
             "launches": kt["launches"],
             "avg_launch_us": round(avg_ms * 1e3, 2),
             "bytes_per_launch": BYTES_PER_CELL * cells_per_launch,
+            # (variant, rows per chunk): 4 = 192-lane, 2 = 128-lane blocks
+            "launch_shape": list(kv) if kv else list(tuning.lookup("elastic", 8, NZ + 10,
+                                                                    NX + 10, args.shots)),
         },
         "cpu_baseline": None,
     }

@@ -315,7 +315,7 @@ static bool pressure_fused_ok(ertm_batch_t b);
 
 int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc) {
     if (!b) return fail(ERTM_EINVAL, "set_variant: NULL batch");
-    if (variant < 0 || variant > 3) return fail(ERTM_EINVAL, "unknown kernel variant %d", variant);
+    if (variant < 0 || variant > 4) return fail(ERTM_EINVAL, "unknown kernel variant %d", variant);
     if (variant != 0 && !((b->all_elastic && b->order == 8) || pressure_fused_ok(b)))
         return fail(ERTM_EINVAL, "fused variants need an order-8 batch of one engine: elastic, "
                                  "acoustic, or viscoacoustic with 1..4 mechanisms");
@@ -374,14 +374,14 @@ static bool pressure_fused_ok(ertm_batch_t b) {
 static bool fused_image_ok(ertm_batch_t b, int op) {
     if (op != ERTM_IMG_EXTRACT && op != ERTM_IMG_CORRELATE) return false;
     // the fused pressure step extracts (its correlation goes through the ring)
-    if (pressure_fused_ok(b) && b->variant >= 1 && b->variant <= 3 && op == ERTM_IMG_EXTRACT) {
+    if (pressure_fused_ok(b) && b->variant >= 1 && b->variant <= 4 && op == ERTM_IMG_EXTRACT) {
         for (size_t k = 0; k < b->props.size(); ++k) {
             const ErtmProp& P = b->props[k];
             if (!b->img[k].rbuf || (int64_t)P.nz * P.nx * 4 >= ((int64_t)1 << 31) - 64) return false;
         }
         return true;
     }
-    if (b->variant < 1 || b->variant > 3 || !b->all_elastic || b->order != 8 || b->any_force)
+    if (b->variant < 1 || b->variant > 4 || !b->all_elastic || b->order != 8 || b->any_force)
         return false;
     for (size_t k = 0; k < b->props.size(); ++k) {
         const ErtmImage& I = b->img[k];
@@ -412,7 +412,7 @@ static bool fused_image_ok(ertm_batch_t b, int op) {
 #define ERTM_IMG_RING_STORE 100
 
 static bool ring_ok(ertm_batch_t b) {
-    if (b->variant < 1 || b->variant > 3 || !(b->all_elastic || pressure_fused_ok(b)) ||
+    if (b->variant < 1 || b->variant > 4 || !(b->all_elastic || pressure_fused_ok(b)) ||
         b->order != 8 || b->any_force)
         return false;
     for (size_t k = 0; k < b->props.size(); ++k) {

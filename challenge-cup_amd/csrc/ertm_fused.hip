@@ -1035,7 +1035,7 @@ static bool launch_fused_img(const ErtmProp* dprops, const ErtmImage* dimg,
     }
 }
 
-// variant w = 1, 2, 3 -> 256, 128, 64 threads per block; zc = rows per chunk.
+// variant w = 1, 2, 3, 4 -> 256, 128, 64, 192 threads per block; zc = rows per chunk.
 // img (ERTM_FIMG_*) != NONE also images the new state; la.t then holds each
 // propagator's time index.
 bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
@@ -1060,6 +1060,7 @@ bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const Lau
             case 1: return launch_pressure<256>(dprops, dimg, hprops, la, zc, img, engine, nmech, s);
             case 2: return launch_pressure<128>(dprops, dimg, hprops, la, zc, img, engine, nmech, s);
             case 3: return launch_pressure<64>(dprops, dimg, hprops, la, zc, img, engine, nmech, s);
+            case 4: return launch_pressure<192>(dprops, dimg, hprops, la, zc, img, engine, nmech, s);
             default: return false;
         }
     }
@@ -1068,6 +1069,7 @@ bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const Lau
         case 1: return launch_fused_img<4, 256>(dprops, dimg, hprops, la, zc, img, s);
         case 2: return launch_fused_img<4, 128>(dprops, dimg, hprops, la, zc, img, s);
         case 3: return launch_fused_img<4, 64>(dprops, dimg, hprops, la, zc, img, s);
+        case 4: return launch_fused_img<4, 192>(dprops, dimg, hprops, la, zc, img, s);
         default: return false;
     }
 }

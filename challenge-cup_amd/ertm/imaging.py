@@ -183,6 +183,14 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     if len(step0) != 1 or len(curs) != 1:
         raise ImagingError("batched propagators must be at the same step")
     b = dev.Batch(cores, variant=variant, graph_steps=graph_steps)
+    if variant is None:
+        from . import tuning
+
+        # one tuning pass per shape and process; later batches look it up
+        if tuning.maybe_autotune(b, 3.0 * nt * sum(c.nz * c.nx for c in cores)) is not None:
+            b.set_variant(*tuning.lookup(cores[0].engine, cores[0].order,
+                                         max(c.nzp for c in cores), max(c.nxp for c in cores),
+                                         len(cores), cores[0].nmech))
     ws = [np.asarray(w, dtype=np.float32) for w in wavelets]
     kw = batch_run_kwargs(props)
     cur = b.run(0, 0, nt, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,

@@ -1,7 +1,7 @@
 """Stencil-kernel variant selection and autotuning.
 
 Variants (see ertm_batch_set_variant): 0 = two launches per step (any
-engine/order); 1/2/3 = fused single launch with 256/128/64 threads per
+engine/order); 1/2/3/4 = fused single launch with 256/128/64/192 threads per
 block, each block streaming a chunk of ``zc`` rows.  Every variant gives bit-identical
 results, so the choice is purely a performance decision: the heuristic picks
 the widest block that still yields enough blocks to keep all 256 CUs busy,
@@ -15,7 +15,7 @@ import math
 import os
 import threading
 
-FUSED_THREADS = {1: 256, 2: 128, 3: 64}
+FUSED_THREADS = {1: 256, 2: 128, 3: 64, 4: 192}
 
 
 def threads(variant: int) -> int:
@@ -49,6 +49,10 @@ def heuristic(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
     halo recompute."""
     if not fused_supported(engine, order, nmech):
         return (0, 64)
+    # 192-lane blocks (176 owned columns) re-read fewer halo columns than
+    # 128-lane ones: 670 us against 714 us per step on 64 shots of 610 x 1210
+    if blocks(4, 48, nzp, nxp, nbatch) >= 16 * n_cu:
+        return (4, 48)
     for zc in (48, 40, 32, 24, 19, 16, 10):
         if blocks(2, zc, nzp, nxp, nbatch) >= 16 * n_cu:
             return (2, zc)
@@ -88,7 +92,7 @@ def record(engine, order, nzp, nxp, nbatch, choice) -> None:
 def candidates(engine: str, order: int, nmech: int = 0) -> list[tuple[int, int]]:
     if not fused_supported(engine, order, nmech):
         return [(0, 64)]
-    return [(0, 64)] + [(v, zc) for v in (1, 2, 3)
+    return [(0, 64)] + [(v, zc) for v in (1, 2, 3, 4)
                         for zc in (16, 24, 32, 48, 64, 96, 128, 160, 208, 320)]
 
 
@@ -122,3 +126,40 @@ def autotune(batch, steps: int = 10, cands=None) -> tuple[tuple[int, int], dict]
     batch.set_variant(*best)
     record(c0.engine, c0.order, c0.nzp, c0.nxp, batch.n, best)
     return best, times
+
+
+# A run of at least this many cell-updates (cells x steps x shots) is worth a
+# short tuning pass on its first batch: about 60 ms of steps against seconds.
+TUNE_MIN_WORK = 2.0e10
+
+
+def maybe_autotune(batch, work: float, n_cu: int = 256) -> tuple[int, int] | None:
+    """Tune a fused batch's launch shape once per (engine, order, grid, batch
+    size), when the run ahead is long enough to amortise it: 192- and 128-lane
+    blocks with 32/48/64-row chunks, 6 steps each.  The batch's states are
+    saved and restored around the timed steps, so the run is unaffected.  The
+    fastest shape differs between MI355X boards (e.g. (2, 48) 646 us against
+    (4, 48) 658 us on one, 714 us against 670 us on another)."""
+    import torch
+
+    c0 = batch.cores[0]
+    if work < TUNE_MIN_WORK or not fused_supported(c0.engine, c0.order, c0.nmech):
+        return None
+    nzp, nxp = max(c.nzp for c in batch.cores), max(c.nxp for c in batch.cores)
+    k = key(c0.engine, c0.order, nzp, nxp, batch.n)
+    with _lock:
+        if k in _cache:
+            return None
+    cands = [(v, zc) for v in (4, 2) for zc in (48, 32, 64)
+             if blocks(v, zc, nzp, nxp, batch.n) >= 4 * n_cu]
+    if len(cands) < 2:
+        return None
+    saved = [(c.state.clone(), c.cur) for c in batch.cores]
+    best, _times = autotune(batch, steps=6, cands=cands)
+    for c, (st, cur) in zip(batch.cores, saved):
+        c.state.copy_(st)
+        c.cur = cur
+    torch.cuda.synchronize()
+    del saved
+    return best
+

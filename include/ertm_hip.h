@@ -227,9 +227,10 @@ int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img, void* stream);
 /* Waits for the work enqueued with this batch, then releases it. */
 int ertm_batch_destroy(ertm_batch_t b);
 /* Stencil kernel variant: 0 = two launches per step (velocity, then
- * pressure/stress; every engine and order); 1, 2, 3 = single fused launch
- * per step (all-elastic or all-acoustic batches, order 8) with 256, 128, 64
- * threads per block streaming
+ * pressure/stress; every engine and order); 1, 2, 3, 4 = single fused launch
+ * per step (batches of one engine at order 8: elastic, acoustic, or
+ * viscoacoustic with 1..4 mechanisms) with 256, 128, 64, 192 threads per
+ * block streaming
  * chunks of zc rows (8 <= zc <= 4096), alternate chunks in opposite
  * directions.  Results are bit-identical across variants. */
 int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc);

@@ -152,7 +152,7 @@ def test_nested_replay_is_bit_identical(tag):
 
 
 @pytest.mark.parametrize("kv,ring", [((2, 19), 16), ((1, 10), 0), ((3, 64), 7), ((2, 8), 1),
-                                     ((2, 48), 0)])
+                                     ((2, 48), 0), ((4, 11), 32)])
 @pytest.mark.parametrize("tag", ["el_pp_divcurl", "el_ps_divcurl", "el_pp_stress", "el_ps_stress",
                                  "el_sp_zl", "el_ss_exc", "ac_zero_lag", "ac_cpml_zl",
                                  "ac_normalized", "ac_raw_rn", "ac_excitation", "vi_noq",
@@ -271,7 +271,7 @@ def test_acoustic_c1_shape_forward_matches_oracle():
 
 
 @pytest.mark.parametrize("variant,zc", [(1, 64), (1, 10), (2, 19), (3, 10), (3, 64), (2, 4096),
-                                        (2, 8), (1, 9), (3, 13), (2, 20)])
+                                        (2, 8), (1, 9), (3, 13), (2, 20), (4, 48), (4, 13)])
 @pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force", "ac_cpml", "ac_free",
                                  "vi_fwd"])
 def test_fused_elastic_variants_bit_identical(tag, variant, zc):
@@ -319,7 +319,8 @@ def test_fused_variant_large_random_model_matches_two_launch(engine):
     src = SourceSpec(x=3000.0, z=1500.0, f0=30.0, delay=0.01)
     w = np.sin(np.arange(60) * 0.3).astype(np.float32) * 1e6
     outs = []
-    for v in [(0, 64), (1, 64), (1, 10), (2, 28), (3, 127), (2, 19), (2, 8), (3, 33)]:
+    for v in [(0, 64), (1, 64), (1, 10), (2, 28), (3, 127), (2, 19), (2, 8), (3, 33), (4, 48),
+              (4, 9)]:
         p = build_propagator(m, spec, engine=engine)
         p.set_source(src)
         p._batches["step"] = dev.Batch([p._core], variant=v)

@@ -178,7 +178,7 @@ def start_step(delay, f0, start_periods, dt, nt):
 
 
 def migrate_raw(engine, m: Model, dt, nt, order, boundary, cfg: Imaging, rec, src, rx, rz,
-                w=None, q: QSpec | None = None, q_enabled=False):
+                w=None, q: QSpec | None = None, q_enabled=False, dtype="float32"):
     """ertm/imaging.py:205-331 up to (and including) normalisation.
 
     ``src`` = dict(x, z, f0, delay, amplitude, kind, mechanism); ``rec`` maps
@@ -204,11 +204,11 @@ def migrate_raw(engine, m: Model, dt, nt, order, boundary, cfg: Imaging, rec, sr
             pad[: d.shape[0]] = d
             d = pad
         aligned[k] = d
-    fwd = build(engine, m, dt, nt, order, npml, boundary, q=qf)
+    fwd = build(engine, m, dt, nt, order, npml, boundary, dtype=dtype, q=qf)
     fwd.set_source(src["x"], src["z"], src.get("mechanism", "explosive"))
     fwd.set_receivers(rx, rz)
     st = record(fwd, w, cfg.checkpoint_mb)
-    back = build(engine, m, dt, nt, order, npml, boundary, q=qb)
+    back = build(engine, m, dt, nt, order, npml, boundary, dtype=dtype, q=qb)
     back.set_source(src["x"], src["z"], src.get("mechanism", "explosive"))
     back.set_receivers(rx, rz)
     sw, rw = MODES[cfg.mode]
@@ -288,10 +288,12 @@ def postprocess(img, m: Model, npml, boundary, cfg: Imaging):
 
 
 def migrate(engine, m, dt, nt, order, boundary, cfg, rec, src, rx, rz, w=None, q=None,
-            q_enabled=False):
-    """Full single-shot migration; returns the float32 image (ImageResult.image)."""
+            q_enabled=False, dtype="float32"):
+    """Full single-shot migration; returns the float32 image (ImageResult.image).
+    dtype="float64" runs the propagation in float64 (the reference's
+    PropagationSpec.dtype), the accuracy reference for the float32 engine."""
     img, info = migrate_raw(engine, m, dt, nt, order, boundary, cfg, rec, src, rx, rz, w, q,
-                            q_enabled)
+                            q_enabled, dtype)
     out = postprocess(img, m, info["npml"], boundary, cfg)
     return np.asarray(out, np.float32), info
 

@@ -1,0 +1,131 @@
+"""Accuracy against a float64 run: the float32 HIP engine's shot records and
+images against the oracle run in float64 (the reference's
+PropagationSpec.dtype = "float64"), within a relative L2 tolerance stated per
+engine.  The float32 results equal the reference's float32 ones bit for bit
+(test_gpu_parity.py), so these tolerances bound float32 rounding itself.
+
+Measured float32-vs-float64 relative L2 on these cases (oracle on the CPU):
+records 1e-7 .. 6e-6, images 4e-7 .. 4.4e-6; a 200 x 300 acoustic
+migration (nt 700) 5.8e-7.  The tolerances leave a margin of
+about 4x over the worst case of each engine.  The compensated
+viscoacoustic path also carries the hipFFT stabiliser's difference from
+pocketfft (up to 1e-4 against the float32 reference)."""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from golden_cases import forward_case, migrate_case, tags
+from product_cases import migrate_objects, propagator_of, rel_l2, source_of
+from test_oracle_golden import migrate_oracle
+
+pytestmark = pytest.mark.gpu
+
+# (records, images) relative L2 tolerance against float64, per engine
+TOL = {"acoustic": (1e-5, 2e-5), "elastic": (1e-5, 2e-5), "viscoacoustic": (3e-5, 3e-5)}
+VISCO_COMPENSATED_IMAGE_TOL = 3e-4
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_forward64(c):
+    from oracle import rtm as ortm
+
+    m = ortm.Model(c["nz"], c["nx"], c["dx"], c["dz"], c["vp"], c["vs"], c["rho"], c["qp"],
+                   c["x0"], c["z0"])
+    q = None
+    if c["engine"] == "viscoacoustic":
+        qq = c["q"]
+        q = ortm.QSpec(f_ref=qq["f_ref"], n=qq["n"], band_ratio=qq["band_ratio"],
+                       compensation=False)
+    p = ortm.build(c["engine"], m, c["dt"], c["nt"], c["order"], c["npml"], c["boundary"],
+                   dtype="float64", q=q)
+    p.set_source(c["src"]["x"], c["src"]["z"], c["src"].get("mechanism", "explosive"))
+    p.set_receivers(c["rx"], c["rz"])
+    s = c["src"]
+    w = ortm.wavelet("ricker", s["f0"], s["delay"], s["amplitude"], c["dt"], c["nt"])
+    return ortm.forward(p, w, c["components"])
+
+
+@pytest.mark.parametrize("tag", [t for t in tags("forward") if t != "vi_comp"])
+def test_records_within_float64_tolerance(tag):
+    _say("records vs float64", tag)
+    from ertm.propagation import source_wavelet
+
+    c = forward_case(tag)
+    p, spec = propagator_of(c)
+    rec = p.run_steps(source_wavelet(source_of(c), spec.dt, spec.nt), c["components"])
+    ref = _oracle_forward64(c)
+    tol = TOL[c["engine"]][0]
+    for name in c["components"]:
+        if np.linalg.norm(ref[name]) == 0:
+            assert not np.any(rec[name]), name
+            continue
+        err = rel_l2(rec[name], ref[name])
+        assert err < tol, (name, err)
+
+
+@pytest.mark.parametrize("tag", tags("migrate"))
+def test_images_within_float64_tolerance(tag):
+    _say("image vs float64", tag)
+    from ertm.imaging import RTMImager
+
+    c = migrate_case(tag)
+    shot, model, imaging, solver, q = migrate_objects(c)
+    img = RTMImager(imaging, solver, q).migrate(shot, model).image
+    ref, _info = migrate_oracle(c, "float64")
+    tol = TOL[c["engine"]][1]
+    if c["engine"] == "viscoacoustic" and c["q"]["enabled"]:
+        tol = VISCO_COMPENSATED_IMAGE_TOL
+    err = rel_l2(img, ref)
+    assert err < tol, err
+
+
+def test_layered_acoustic_migration_within_float64_tolerance():
+    """Plain acoustic RTM shape (200 x 300 at 5 m, four flat layers
+    2000..3500 m/s, CPML 20, 300 surface receivers), zero_lag + Laplacian:
+    the GPU image against the oracle's float64 image.  dt is 0.35 ms: the
+    0.5 ms of that workload's description exceeds the order-8 CFL limit for
+    3500 m/s at 5 m (0.43 ms), which the imager refuses."""
+    _say("layered 200x300 acoustic image vs float64")
+    from ertm.config import ImagingConfig, SolverConfig
+    from ertm.imaging import RTMImager
+    from ertm.types import ElasticModel, GridSpec, ReceiverSpec, ShotRecord, SourceSpec
+    from oracle import rtm as ortm
+
+    nz, nx, nt, dt = 200, 300, 700, 3.5e-4
+    z = np.arange(nz)[:, None] * 5.0
+    vp = (np.select([z < 250, z < 500, z < 750], [2000.0, 2500.0, 3000.0], 3500.0)
+          * np.ones((1, nx))).astype(np.float32)
+    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    vs = (vp / 1.8).astype(np.float32)
+    om = ortm.Model(nz, nx, 5.0, 5.0, vp, vs, rho, np.full_like(vp, 100.0))
+    src = dict(x=750.0, z=10.0, f0=25.0, delay=0.048, amplitude=1.0, kind="ricker",
+               mechanism="explosive")
+    rx, rz = np.arange(nx) * 5.0, np.full(nx, 10.0)
+    op = ortm.build("acoustic", om, dt, nt, 8, 20, "cpml")
+    op.set_source(src["x"], src["z"])
+    op.set_receivers(rx, rz)
+    rec = ortm.forward(op, ortm.wavelet("ricker", 25.0, 0.048, 1.0, dt, nt), ("p",))
+    cfg = ortm.Imaging(mode="pp", condition="zero_lag", laplacian=True, checkpoint_mb=2.0)
+    cfg64 = ortm.Imaging(**{**cfg.__dict__, "checkpoint_mb": 4.0})
+    ref, _ = ortm.migrate("acoustic", om, dt, nt, 8, "cpml", cfg64, rec, src, rx, rz,
+                          dtype="float64")
+
+    model = ElasticModel(GridSpec(nx=nx, nz=nz, dx=5.0, dz=5.0), vp=vp, vs=vs, rho=rho,
+                         qp=np.full_like(vp, 100.0), qs=np.full_like(vp, 50.0))
+    shot = ShotRecord(data=rec["p"], dt=dt, t=np.arange(nt) * dt,
+                      receivers=ReceiverSpec(x=rx, z=rz, component="p"),
+                      source=SourceSpec(x=750.0, z=10.0, f0=25.0, delay=0.048),
+                      components={"p": rec["p"]})
+    imaging = ImagingConfig(mode="pp", condition="zero_lag", laplacian=True, checkpoint_mb=2.0)
+    solver = SolverConfig(nt=nt, dt=dt, order=8, boundary="cpml", engine="acoustic", npml=20)
+    img = RTMImager(imaging, solver).migrate(shot, model).image
+    err = rel_l2(img, ref)
+    assert err < TOL["acoustic"][1], err

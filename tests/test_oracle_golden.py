@@ -8,6 +8,8 @@ reordered product shows up as a mismatch.  The fixtures were written by
 
 from __future__ import annotations
 
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -127,8 +129,13 @@ def migrate_oracle(c, dtype="float32"):
                        lowpass_interval=qq["lowpass_interval"], f_max=qq["f_max"],
                        gain_budget_db=qq["gain_budget_db"])
         q_enabled = qq["enabled"]
+    if dtype == "float64":
+        # same checkpoint interval as the float32 run: twice the budget for
+        # states twice the size
+        cfg = dataclasses.replace(cfg, checkpoint_mb=2.0 * cfg.checkpoint_mb)
     return ortm.migrate(c["engine"], m, c["dt"], c["nt"], c["order"], c["boundary"], cfg,
-                        c["shot"], c["src"], c["rx"], c["rz"], q=q, q_enabled=q_enabled)
+                        c["shot"], c["src"], c["rx"], c["rz"], q=q, q_enabled=q_enabled,
+                        dtype=dtype)
 
 
 @pytest.mark.parametrize("tag", tags("migrate"))
@@ -153,3 +160,38 @@ def test_forward_shot_records_feed_migrate_cases():
     rec = ortm.forward(p, w, c["components"])
     for comp in c["components"]:
         assert np.array_equal(rec[comp], c["shot"][comp])
+
+
+@pytest.mark.parametrize("tag", ["ac_cpml", "el_force", "vi_fwd"])
+def test_float64_mode_runs_in_float64(tag):
+    """The oracle's float64 mode (the reference's PropagationSpec.dtype,
+    ertm/propagation.py:52,207) keeps the wavefields in float64 while the CPML
+    profiles and the records stay float32 (ertm/numerics.py:217-219,
+    ertm/propagation.py:1045).  Its records must differ from the float32 run's
+    (else the float64 path is not taken) by float32 rounding only."""
+    c = forward_case(tag)
+    m = ortm.Model(c["nz"], c["nx"], c["dx"], c["dz"], c["vp"], c["vs"], c["rho"], c["qp"],
+                   c["x0"], c["z0"])
+    q = None
+    if c["engine"] == "viscoacoustic":
+        qq = c["q"]
+        q = ortm.QSpec(f_ref=qq["f_ref"], n=qq["n"], band_ratio=qq["band_ratio"],
+                       compensation=False)
+    s = c["src"]
+    w = ortm.wavelet("ricker", s["f0"], s["delay"], s["amplitude"], c["dt"], c["nt"])
+    out = {}
+    for dtype in ("float32", "float64"):
+        p = ortm.build(c["engine"], m, c["dt"], c["nt"], c["order"], c["npml"], c["boundary"],
+                       dtype=dtype, q=q)
+        assert p.vz.dtype == np.dtype(dtype)
+        p.set_source(s["x"], s["z"], s.get("mechanism", "explosive"))
+        p.set_receivers(c["rx"], c["rz"])
+        out[dtype] = ortm.forward(p, w, c["components"])
+    assert all(np.array_equal(out["float32"][k], c["rec"][k]) for k in c["components"])
+    for k in c["components"]:
+        a = out["float32"][k].astype(np.float64)
+        b = out["float64"][k].astype(np.float64)
+        if not np.any(b):
+            continue
+        err = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert 0.0 < err < 1e-5, (k, err)

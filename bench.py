@@ -273,13 +273,15 @@ def c5_sweep(budgets_mb, nt):
     return out
 
 
-def read_traffic():
-    """HBM bytes per step launch from the committed FETCH_SIZE/WRITE_SIZE
-    profile summary (a line 'traffic_bytes_per_launch: N'), else None."""
+def read_traffic(shape):
+    """HBM bytes per step launch of launch shape ``shape`` = (variant, rows
+    per chunk) from the committed FETCH_SIZE/WRITE_SIZE profile summary (a
+    line 'traffic_bytes_per_launch[V,ZC]: N'), else None."""
+    tag = "traffic_bytes_per_launch[%d,%d]:" % tuple(int(v) for v in shape)
     try:
         with open(TRAFFIC_FILE) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(tag):
                     return float(line.split(":", 1)[1])
     except OSError:
         return None
@@ -428,7 +430,8 @@ def main(argv=None) -> int:
     # step launches replayed from captured graphs are not timed one by one
     achieved = (BYTES_PER_CELL * cells_per_launch / (avg_ms * 1e-3) / 1e9
                 if kt["launches"] > 0 else None)
-    traffic = read_traffic()
+    shape = list(kv) if kv else list(tuning.lookup("elastic", 8, NZ + 10, NX + 10, args.shots))
+    traffic = read_traffic(shape)
     line = {
         "metric": "RTM shots/s & Gcell-updates/s (% HBM roofline) on 1 GPU; p50/p99 shot latency",
         "value": round(shots_s, 4),
@@ -462,8 +465,8 @@ def main(argv=None) -> int:
             "avg_launch_us": round(avg_ms * 1e3, 2),
             "bytes_per_launch": BYTES_PER_CELL * cells_per_launch,
             # (variant, rows per chunk): 4 = 192-lane, 2 = 128-lane blocks
-            "launch_shape": list(kv) if kv else list(tuning.lookup("elastic", 8, NZ + 10,
-                                                                    NX + 10, args.shots)),
+            "launch_shape": shape,
+            "image_ring": int(imager.gpu_config.image_ring),
         },
         "cpu_baseline": None,
     }

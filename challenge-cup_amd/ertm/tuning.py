@@ -40,13 +40,13 @@ def blocks(variant: int, zc: int, nzp: int, nxp: int, nbatch: int) -> int:
 
 def heuristic(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
               n_cu: int = 256, nmech: int = 0) -> tuple[int, int]:
-    """128-thread fused blocks with the longest chunk (up to 48 rows) that
-    still gives 16 blocks per CU: enough row streams keep loads in flight,
-    and alternating chunk directions let L2 absorb part of the halo
-    re-reads (measured on 64 shots of 610 x 1210 sharing one medium: 48-row
-    chunks 646 us per step, 24-row 731 us, 96-row 734 us); batches too small
-    to fill the chip use the two-launch kernels, whose 2-D tiles need no
-    halo recompute."""
+    """192-lane fused blocks with 48-row chunks when they give 16 blocks per
+    CU, else 128-lane blocks with the longest chunk (up to 48 rows) that
+    does: enough row streams keep loads in flight, and alternating chunk
+    directions let L2 absorb part of the halo re-reads (measured on 64 shots
+    of 610 x 1210 sharing one medium, 128 lanes: 48-row chunks 646 us per
+    step, 24-row 731 us, 96-row 734 us); batches too small to fill the chip
+    use the two-launch kernels, whose 2-D tiles need no halo recompute."""
     if not fused_supported(engine, order, nmech):
         return (0, 64)
     # 192-lane blocks (176 owned columns) re-read fewer halo columns than

@@ -6,7 +6,7 @@ engine.  The float32 results equal the reference's float32 ones bit for bit
 
 Measured float32-vs-float64 relative L2 on these cases (oracle on the CPU):
 records 1e-7 .. 6e-6, images 4e-7 .. 4.4e-6; a 200 x 300 acoustic
-migration (nt 700) 5.8e-7.  The tolerances leave a margin of
+migration (nt 700) 3.7e-6.  The tolerances leave a margin of
 about 4x over the worst case of each engine.  The compensated
 viscoacoustic path also carries the hipFFT stabiliser's difference from
 pocketfft (up to 1e-4 against the float32 reference)."""
@@ -90,16 +90,17 @@ def test_images_within_float64_tolerance(tag):
 def test_layered_acoustic_migration_within_float64_tolerance():
     """Plain acoustic RTM shape (200 x 300 at 5 m, four flat layers
     2000..3500 m/s, CPML 20, 300 surface receivers), zero_lag + Laplacian:
-    the GPU image against the oracle's float64 image.  dt is 0.35 ms: the
-    0.5 ms of that workload's description exceeds the order-8 CFL limit for
-    3500 m/s at 5 m (0.43 ms), which the imager refuses."""
+    the GPU image against the oracle's float64 image, at that workload's
+    dt of 0.5 ms.  That step exceeds the reference's guard of 0.55 x the CFL
+    limit (0.43 ms at 3500 m/s and 5 m) but is 0.64 of the limit itself, so
+    the imager runs with strict_cfl off, as for that workload."""
     _say("layered 200x300 acoustic image vs float64")
-    from ertm.config import ImagingConfig, SolverConfig
+    from ertm.config import GpuConfig, ImagingConfig, SolverConfig
     from ertm.imaging import RTMImager
     from ertm.types import ElasticModel, GridSpec, ReceiverSpec, ShotRecord, SourceSpec
     from oracle import rtm as ortm
 
-    nz, nx, nt, dt = 200, 300, 700, 3.5e-4
+    nz, nx, nt, dt = 200, 300, 700, 5e-4
     z = np.arange(nz)[:, None] * 5.0
     vp = (np.select([z < 250, z < 500, z < 750], [2000.0, 2500.0, 3000.0], 3500.0)
           * np.ones((1, nx))).astype(np.float32)
@@ -126,6 +127,6 @@ def test_layered_acoustic_migration_within_float64_tolerance():
                       components={"p": rec["p"]})
     imaging = ImagingConfig(mode="pp", condition="zero_lag", laplacian=True, checkpoint_mb=2.0)
     solver = SolverConfig(nt=nt, dt=dt, order=8, boundary="cpml", engine="acoustic", npml=20)
-    img = RTMImager(imaging, solver).migrate(shot, model).image
+    img = RTMImager(imaging, solver, None, GpuConfig(strict_cfl=False)).migrate(shot, model).image
     err = rel_l2(img, ref)
     assert err < TOL["acoustic"][1], err

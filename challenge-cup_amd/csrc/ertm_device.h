@@ -37,13 +37,18 @@ __device__ __forceinline__ gcfloat* gp(const float* p) { return (gcfloat*)p; }
 //     FMA residual r = s - q0*h brings q1 within ~2^-104 of s/h; a midpoint
 //     there is a float64 number, q1 equals it, and the final rounding applies
 //     ties-to-even like the IEEE division.  The step runs only when some lane
-//     of the wave needs it (the branch is on the wave's ballot).
+//     of the wave needs it (the branch is on the wave's ballot).  A zero
+//     quotient (s = 0: q0 = s * rh cannot underflow in float64) is exact
+//     and skips it; zeros fill every region the wave has not reached, and
+//     the empty asm keeps the step a real branch instead of code that every
+//     wave executes.
 // Non-finite inputs give the plain product's rounding (inf, NaN).
 __device__ __forceinline__ float div_h(float s, float h, double rh) {
     const double q0 = (double)s * rh;
     float q = (float)q0;
-    const bool tiny = __builtin_fabs(q0) < 0x1p-125;  // false for NaN
+    const bool tiny = q0 != 0.0 && __builtin_fabs(q0) < 0x1p-125;  // false for NaN
     if (__builtin_amdgcn_ballot_w64(tiny)) {
+        asm volatile("");
         const double r = fma(-q0, (double)h, (double)s);
         const double q1 = fma(r, rh, q0);
         if (tiny) q = (float)q1;

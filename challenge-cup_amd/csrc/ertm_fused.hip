@@ -42,11 +42,16 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void* p, int64_t bytes) {
                                              0x00020000);
 }
 // The b32 builtins move raw 32-bit patterns: bit-cast, never convert.
-// Stores carry the non-temporal hint (cache-policy bit 1, "nt"): what a step
-// writes is read back only by the next launch, after far more data than the
-// caches hold has passed, so its lines should not displace the medium and
-// the halo rows that blocks of the same launch re-read (measured: 630 us
-// against 649 us per 64-shot elastic step).
+// State stores carry the non-temporal hint (cache-policy bit 1, "nt"): what a
+// step writes is read back only by the next launch, after far more data than
+// the caches hold has passed, so its lines should not displace the medium
+// and the halo rows that blocks of the same launch re-read (measured: 630 us
+// against 649 us per 64-shot elastic step).  Image stores (replay slots,
+// receiver ring) keep the default policy: their rows are not aligned to
+// cache lines, and a line that two blocks write in part is merged in L2
+// before it goes to HBM (non-temporal, the replay and backward passes of a
+// 64-shot migration took 6.50 s instead of 6.31 s).  The CPML memories'
+// x-strip rows are 2 x 80-byte pieces, so they keep the default policy too.
 #define ERTM_CPOL_NT 2
 __device__ __forceinline__ float bload(rsrc_t r, int off) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
@@ -54,6 +59,9 @@ __device__ __forceinline__ float bload(rsrc_t r, int off) {
 __device__ __forceinline__ void bstore(rsrc_t r, int off, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0,
                                           ERTM_CPOL_NT);
+}
+__device__ __forceinline__ void bstore_l2(rsrc_t r, int off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0, 0);
 }
 
 __device__ __forceinline__ double bload64(rsrc_t r, int off) {
@@ -249,16 +257,16 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, qx_len<K, I
             float pn;
             if (pxo >= 0) {
                 a = cpml_r(a, bload(B.rS, B.px0 + pxo), B.axc, B.bxc, pn);
-                bstore(B.rT, own_m ? B.px0 + pxo : ERTM_NOWRITE, pn);
+                bstore_l2(B.rT, own_m ? B.px0 + pxo : ERTM_NOWRITE, pn);
                 cc = cpml_r(cc, bload(B.rS, B.px1 + pxo), B.axc, B.bxc, pn);
-                bstore(B.rT, own_m ? B.px1 + pxo : ERTM_NOWRITE, pn);
+                bstore_l2(B.rT, own_m ? B.px1 + pxo : ERTM_NOWRITE, pn);
             }
             if (pzo >= 0) {
                 const float az = B.AZ[m], bz = B.BZ[m];
                 b = cpml_r(b, bload(B.rS, B.pz0 + pzo), az, bz, pn);
-                bstore(B.rT, own_m ? B.pz0 + pzo : ERTM_NOWRITE, pn);
+                bstore_l2(B.rT, own_m ? B.pz0 + pzo : ERTM_NOWRITE, pn);
                 d = cpml_r(d, bload(B.rS, B.pz1 + pzo), az, bz, pn);
-                bstore(B.rT, own_m ? B.pz1 + pzo : ERTM_NOWRITE, pn);
+                bstore_l2(B.rT, own_m ? B.pz1 + pzo : ERTM_NOWRITE, pn);
             }
         }
     }
@@ -312,16 +320,16 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, qx_len<K, I
                 float pn;
                 if (pxo >= 0) {
                     ex = cpml_r(ex, bload(B.rS, B.px2 + pxo), B.axc, B.bxc, pn);
-                    bstore(B.rT, B.px2 + pxo, pn);
+                    bstore_l2(B.rT, B.px2 + pxo, pn);
                     f = cpml_r(f, bload(B.rS, B.px3 + pxo), B.axc, B.bxc, pn);
-                    bstore(B.rT, B.px3 + pxo, pn);
+                    bstore_l2(B.rT, B.px3 + pxo, pn);
                 }
                 if (pzo >= 0) {
                     const float az = B.AZ[k], bz = B.BZ[k];
                     ez = cpml_r(ez, bload(B.rS, B.pz2 + pzo), az, bz, pn);
-                    bstore(B.rT, B.pz2 + pzo, pn);
+                    bstore_l2(B.rT, B.pz2 + pzo, pn);
                     e = cpml_r(e, bload(B.rS, B.pz3 + pzo), az, bz, pn);
-                    bstore(B.rT, B.pz3 + pzo, pn);
+                    bstore_l2(B.rT, B.pz3 + pzo, pn);
                 }
             }
         }
@@ -387,11 +395,11 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, qx_len<K, I
                 }
             }
             if constexpr (img_op<IMG>() == ERTM_FIMG_EXTRACT) {
-                bstore(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.swave == ERTM_WAVE_S ? fs : fp);
+                bstore_l2(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.swave == ERTM_WAVE_S ? fs : fp);
             } else if constexpr (img_op<IMG>() == ERTM_FIMG_RING) {
-                bstore(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.rw0 == ERTM_WAVE_S ? fs : fp);
+                bstore_l2(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.rw0 == ERTM_WAVE_S ? fs : fp);
                 if (B.nrw > 1)
-                    bstore(B.rR, cell ? B.o_num1 + 4 * q : ERTM_NOWRITE,
+                    bstore_l2(B.rR, cell ? B.o_num1 + 4 * q : ERTM_NOWRITE,
                            B.rw1 == ERTM_WAVE_S ? fs : fp);
             } else {
                 const int q4 = cell ? 4 * q : ERTM_NOWRITE, q8 = 8 * q;
@@ -729,12 +737,12 @@ __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K, VL>& X, int
             float pn;
             if (pxo >= 0) {
                 dx = cpml_r(dx, bload(B.rS, B.px0 + pxo), B.axc, B.bxc, pn);
-                bstore(B.rT, own_m ? B.px0 + pxo : ERTM_NOWRITE, pn);
+                bstore_l2(B.rT, own_m ? B.px0 + pxo : ERTM_NOWRITE, pn);
             }
             if (pzo >= 0) {
                 const float az = B.AZ[m], bz = B.BZ[m];
                 dz = cpml_r(dz, bload(B.rS, B.pz0 + pzo), az, bz, pn);
-                bstore(B.rT, own_m ? B.pz0 + pzo : ERTM_NOWRITE, pn);
+                bstore_l2(B.rT, own_m ? B.pz0 + pzo : ERTM_NOWRITE, pn);
             }
         }
     }
@@ -776,12 +784,12 @@ __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K, VL>& X, int
                 float pn;
                 if (pxo >= 0) {
                     dxv = cpml_r(dxv, bload(B.rS, B.px1 + pxo), B.axc, B.bxc, pn);
-                    bstore(B.rT, B.px1 + pxo, pn);
+                    bstore_l2(B.rT, B.px1 + pxo, pn);
                 }
                 if (pzo >= 0) {
                     const float az = B.AZ[k], bz = B.BZ[k];
                     dzv = cpml_r(dzv, bload(B.rS, B.pz1 + pzo), az, bz, pn);
-                    bstore(B.rT, B.pz1 + pzo, pn);
+                    bstore_l2(B.rT, B.pz1 + pzo, pn);
                 }
             }
         }
@@ -828,9 +836,9 @@ __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K, VL>& X, int
             const int i = k - B.pad;
             const bool cell = B.own && B.jcol >= 0 && i >= 0 && i < B.nzph;
             const int q4 = cell ? 4 * (i * B.nxph + B.jcol) : ERTM_NOWRITE;
-            bstore(B.rR, q4, p);
+            bstore_l2(B.rR, q4, p);
             if (IMG == ERTM_FIMG_RING && B.nrw > 1)
-                bstore(B.rR, cell ? B.o_f1 + q4 : ERTM_NOWRITE, p);
+                bstore_l2(B.rR, cell ? B.o_f1 + q4 : ERTM_NOWRITE, p);
         }
     }
     X.ir_c = D > 0 ? ir_m1 : ir_m;

@@ -420,7 +420,7 @@ static bool ring_ok(ertm_batch_t b) {
         const ErtmProp& P = b->props[k];
         if (!I.rbuf || !I.rring || I.ring_len < 1 || I.nrw < 1 || I.nrw > 2 || !I.num[0])
             return false;
-        if ((int64_t)P.nz * P.nx * 4 * I.nrw >= ((int64_t)1 << 31) - 64) return false;
+        if ((int64_t)P.nz * P.ld * 4 * I.nrw >= ((int64_t)1 << 31) - 64) return false;
         if (I.ring_len != b->img[0].ring_len) return false;
     }
     return true;

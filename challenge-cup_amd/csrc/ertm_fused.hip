@@ -397,10 +397,11 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, qx_len<K, I
             if constexpr (img_op<IMG>() == ERTM_FIMG_EXTRACT) {
                 bstore_l2(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.swave == ERTM_WAVE_S ? fs : fp);
             } else if constexpr (img_op<IMG>() == ERTM_FIMG_RING) {
-                bstore_l2(B.rR, cell ? 4 * q : ERTM_NOWRITE, B.rw0 == ERTM_WAVE_S ? fs : fp);
+                const int r4 = cell ? i * ld4 + B.c4 : ERTM_NOWRITE;
+                bstore_l2(B.rR, r4, B.rw0 == ERTM_WAVE_S ? fs : fp);
                 if (B.nrw > 1)
-                    bstore_l2(B.rR, cell ? B.o_num1 + 4 * q : ERTM_NOWRITE,
-                           B.rw1 == ERTM_WAVE_S ? fs : fp);
+                    bstore_l2(B.rR, cell ? B.o_num1 + r4 : ERTM_NOWRITE,
+                              B.rw1 == ERTM_WAVE_S ? fs : fp);
             } else {
                 const int q4 = cell ? 4 * q : ERTM_NOWRITE, q8 = 8 * q;
                 const float sv = bload(B.rR, q4);
@@ -564,10 +565,12 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
         B.jcol = (B.own && B.c >= P.pad && B.c < P.pad + P.nx) ? B.c - P.pad : -1;
         const int64_t ncell = (int64_t)P.nz * P.nx;
         if (img_op<IMG>() == ERTM_FIMG_RING) {
-            // ring slot t mod R: nrw fields of ncell floats; o_num1 = second field
-            B.rR = make_rsrc(I.rring + (int64_t)(t % I.ring_len) * I.nrw * ncell,
-                             4 * ncell * I.nrw);
-            B.o_num1 = (int)(4 * ncell);
+            // ring slot t mod R: nrw fields of nz rows on the state's pitch,
+            // lane column c; o_num1 = second field
+            const int64_t nring = (int64_t)P.nz * P.ld;
+            B.rR = make_rsrc(I.rring + (int64_t)(t % I.ring_len) * I.nrw * nring,
+                             4 * nring * I.nrw);
+            B.o_num1 = (int)(4 * nring);
         } else {
             B.rR = make_rsrc(I.rbuf + (int64_t)(t - I.t0) * ncell, 4 * ncell);
         }
@@ -835,7 +838,8 @@ __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K, VL>& X, int
         if constexpr (IMG != ERTM_FIMG_NONE) {
             const int i = k - B.pad;
             const bool cell = B.own && B.jcol >= 0 && i >= 0 && i < B.nzph;
-            const int q4 = cell ? 4 * (i * B.nxph + B.jcol) : ERTM_NOWRITE;
+            const int q4 = !cell ? ERTM_NOWRITE
+                           : IMG == ERTM_FIMG_RING ? i * B.ld4 + B.c4 : 4 * (i * B.nxph + B.jcol);
             bstore_l2(B.rR, q4, p);
             if (IMG == ERTM_FIMG_RING && B.nrw > 1)
                 bstore_l2(B.rR, cell ? B.o_f1 + q4 : ERTM_NOWRITE, p);
@@ -952,10 +956,12 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
         B.nxph = P.nx;
         B.jcol = (B.own && B.c >= P.pad && B.c < P.pad + P.nx) ? B.c - P.pad : -1;
         B.nrw = I.nrw;
-        B.o_f1 = (int)(4 * ncell);
+        // ring rows on the state's pitch (lane column c), replay slots (nz, nx)
+        const int64_t nring = (int64_t)P.nz * P.ld;
+        B.o_f1 = (int)(4 * nring);
         if (IMG == ERTM_FIMG_RING)
-            B.rR = make_rsrc(I.rring + (int64_t)(t % I.ring_len) * I.nrw * ncell,
-                             4 * ncell * I.nrw);
+            B.rR = make_rsrc(I.rring + (int64_t)(t % I.ring_len) * I.nrw * nring,
+                             4 * nring * I.nrw);
         else
             B.rR = make_rsrc(I.rbuf + (int64_t)(t - I.t0) * ncell, 4 * ncell);
     }

@@ -198,16 +198,20 @@ __global__ void k_image_flush(const ErtmProp* __restrict__ props,
     double ill = src ? I.ill[q] : 0.0;
     double n0 = I.num[0][q], n1 = nrw > 1 ? I.num[1][q] : 0.0;
     double r0 = rec ? I.rill[0][q] : 0.0, r1 = (rec && nrw > 1) ? I.rill[1][q] : 0.0;
+    // ring rows are on the state's pitch: cell (i, j) at row i, column j + pad
+    const int64_t nring = (int64_t)P.nz * P.ld;
+    const int64_t qi = q / P.nx;
+    const int64_t qr = qi * P.ld + (q - qi * P.nx) + P.pad;
     for (int j = 0; j < count; ++j) {
         const int t = t_hi - j;
         const float sv = I.rbuf[(int64_t)(t - I.t0) * ncell + q];
-        const float* ring = I.rring + (int64_t)(t % I.ring_len) * nrw * ncell;
+        const float* ring = I.rring + (int64_t)(t % I.ring_len) * nrw * nring;
         if (src) {
             float ss = sv * sv;
             ill = ill + (double)ss;
         }
         const bool cross = t >= I.start_step;
-        const float rv0 = ring[q];
+        const float rv0 = ring[qr];
         if (rec) {
             float rr = rv0 * rv0;
             r0 = r0 + (double)rr;
@@ -217,7 +221,7 @@ __global__ void k_image_flush(const ErtmProp* __restrict__ props,
             n0 = n0 + (double)sr;
         }
         if (nrw > 1) {
-            const float rv1 = ring[ncell + q];
+            const float rv1 = ring[nring + qr];
             if (rec) {
                 float rr = rv1 * rv1;
                 r1 = r1 + (double)rr;

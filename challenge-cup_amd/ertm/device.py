@@ -443,7 +443,7 @@ class ImageBuffers:
 
     def __init__(self, nz: int, nx: int, separation: str, swave: str, rwaves, start_step: int,
                  illum: int, excitation: bool = False, rbuf=None, nslots: int = 0,
-                 ring: int = 0):
+                 ring: int = 0, ring_ld: int = 0):
         torch = torch_mod()
         self.s = L.ErtmImage()
         s = self.s
@@ -471,7 +471,12 @@ class ImageBuffers:
         s.nslots = int(nslots)
         self.rring = None
         if ring > 0 and nrw > 0:
-            self.rring = torch.empty((int(ring), nrw, nz, nx), dtype=torch.float32, device="cuda")
+            # rows on the propagator's pitch (ring_ld >= nx + 2 pad), cell (i, j)
+            # at column j + pad
+            if ring_ld < nx:
+                raise ValueError("ring_ld must be the propagator's row pitch")
+            self.rring = torch.empty((int(ring), nrw, nz, int(ring_ld)), dtype=torch.float32,
+                                     device="cuda")
             s.rring = L.ptr(self.rring)
             s.ring_len = int(ring)
         if excitation:

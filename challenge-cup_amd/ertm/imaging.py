@@ -540,8 +540,8 @@ class RTMImager:
                                  device="cuda") for m in models]
             ring = int(self.gpu_config.image_ring) if engine in ("elastic", "acoustic") else 0
             ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, rwaves, st, illum,
-                                    rbuf=rb, nslots=slots, ring=ring)
-                   for m, st, rb in zip(models, starts, rbufs)]
+                                    rbuf=rb, nslots=slots, ring=ring, ring_ld=bc.ld)
+                   for m, st, rb, bc in zip(models, starts, rbufs, bcores)]
             fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, 0,
                                       rbuf=rb, nslots=slots) for m, rb in zip(models, rbufs)]
             fb = dev.Batch(fcores, imgs=fimgs, variant=kv, graph_steps=gsteps)

@@ -199,9 +199,12 @@ typedef struct ErtmImage {
     int32_t* peak_step;
     double* peak_amp;
     double* running;
-    /* optional (R, nrw, nz, nx) float32 ring: with it, a fused elastic batch
-     * stores the receiver fields of up to R backward steps and folds them into
-     * the accumulators in one pass (same per-cell order: identical results) */
+    /* optional (R, nrw, nz, ld) float32 ring, ld = the propagator's row
+     * pitch (ErtmProp.ld), physical cell (i, j) at column j + pad: with it, a
+     * fused batch stores the receiver fields of up to R backward steps and
+     * folds them into the accumulators in one pass (same per-cell order:
+     * identical results).  Rows on the state's pitch let each block write
+     * the same cache-line spans as its state stores. */
     float* rring;
 } ErtmImage;
 

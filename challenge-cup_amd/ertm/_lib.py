@@ -16,7 +16,7 @@ from .exceptions import ERTMError, SolverError
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libertm_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_LAUNCH = 64
 MAX_MECH = 8
 

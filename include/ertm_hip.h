@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define ERTM_ABI_VERSION 3
+#define ERTM_ABI_VERSION 4
 
 enum { ERTM_OK = 0, ERTM_EINVAL = 1, ERTM_EHIP = 2, ERTM_ENOMEM = 3, ERTM_ESTATE = 4 };
 

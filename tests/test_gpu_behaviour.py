@@ -1,8 +1,11 @@
-"""The reference's behavioural tests (tests/test_rtm.py:177-345) run on the
-HIP engine: absorbing boundary, checkpoint replay, moveout, P/S separation,
-layer depth, steep dip, elastic multi-component imaging.  The thresholds are
-the reference's own; the physics, not a second implementation, is the check.
-Also the device record preparation against the host one."""
+"""The reference's behavioural tests (tests/test_rtm.py:177-345 and
+391-576) run on the HIP engine: absorbing boundary, checkpoint replay,
+moveout, P/S separation, layer depth, steep dip, elastic multi-component
+imaging, and for the viscoacoustic engine the measured Q, the round-trip
+compensation gain, the band limit and gain budget, and deep-amplitude
+recovery.  The thresholds are the reference's own; the physics, not a second
+implementation, is the check.  Also the device record preparation against
+the host one."""
 
 from __future__ import annotations
 
@@ -218,3 +221,139 @@ def test_device_record_preparation_matches_host(rotation):
         ulp = np.spacing(np.abs(want).astype(np.float32))
         assert np.all(np.abs(got - want) <= ulp), name
         assert np.mean(got == want) > 0.99, name
+
+
+# ---------------------------------------------------------------------------
+# Viscoacoustic engine (tests/test_rtm.py:391-558): measured Q, round-trip
+# gain of attenuation then compensation, the band limit and gain budget of
+# the compensation, and deep-amplitude recovery in migration.  The reference
+# runs the first two with PropagationSpec.dtype = "float64"; this engine
+# computes in float32, and the thresholds hold all the same.
+
+def homogeneous_q(qp, nx=121, nz=101):
+    grid = GridSpec(nx=nx, nz=nz, dx=10.0, dz=10.0)
+    shape = grid.shape
+    return ElasticModel(grid=grid, vp=np.full(shape, 3000.0, np.float32),
+                        vs=np.full(shape, 1700.0, np.float32),
+                        rho=np.full(shape, 2400.0, np.float32),
+                        qp=np.full(shape, qp, np.float32), qs=np.full(shape, 50.0, np.float32))
+
+
+def visco_traces(model, spec, source, receivers, q_options):
+    """Pressure at the receivers after every step, stepping one at a time."""
+    from ertm.propagation import ViscoacousticPropagator
+
+    prop = ViscoacousticPropagator(model, spec, q_options=q_options)
+    prop.set_source(source)
+    prop.set_receivers(receivers)
+    wave = source_wavelet(source, spec.dt, spec.nt)
+    out = np.zeros((spec.nt, receivers.nrec))
+    for k in range(spec.nt):
+        prop.step(float(wave[k]))
+        out[k] = prop.sample(("p",))["p"]
+    return out
+
+
+def test_viscoacoustic_measures_target_q():
+    """Spectral-ratio Q between receivers 200 m and 400 m from the source
+    within 20 % of the model's Q = 40 (tests/test_rtm.py:391-429)."""
+    from ertm.propagation import QOptions
+
+    model = homogeneous_q(40.0, nx=141, nz=121)
+    spec = PropagationSpec(dt=4e-4, nt=900, order=8, npml=12, dtype="float64", boundary="cpml")
+    source = SourceSpec(x=600.0, z=600.0, f0=25.0, delay=0.03)
+    receivers = ReceiverSpec(x=np.array([800.0, 1000.0]), z=np.array([600.0, 600.0]))
+    data = visco_traces(model, spec, source, receivers, QOptions(enabled=True, f_ref=25.0))
+
+    def spectrum(trace, offset):
+        start = max(int((source.delay + offset / 3000.0) / spec.dt) - 100, 0)
+        return np.abs(np.fft.rfft(trace[start:start + 400] * np.hanning(400)))
+
+    freqs = np.fft.rfftfreq(400, spec.dt)
+    band = (freqs > 5) & (freqs < 60)
+    # geometric spreading in 2-D: amplitude ~ r^-1/2, i.e. sqrt(2) between 200 m and 400 m
+    ratio = np.log(spectrum(data[:, 0], 200.0)[band] / spectrum(data[:, 1], 400.0)[band])
+    slope = np.polyfit(freqs[band], ratio + 0.5 * np.log(2.0), 1)[0]
+    measured = float(np.pi * 200.0 / 3000.0 / max(slope, 1e-12))
+    assert abs(measured / 40.0 - 1.0) < 0.2, measured
+
+
+def test_round_trip_gain_matches_theory_and_is_frequency_dependent():
+    """Compensated over attenuated spectrum along 400 m of Q = 25 within
+    15 % of exp(2 pi f t / Q) at 10, 25 and 40 Hz, and at 40 Hz more than
+    twice the 10 Hz gain (tests/test_rtm.py:492-539)."""
+    from ertm.propagation import QOptions
+
+    spec = PropagationSpec(dt=4e-4, nt=800, order=8, npml=12, dtype="float64", boundary="cpml")
+    source = SourceSpec(x=500.0, z=500.0, f0=25.0, delay=0.03)
+    receivers = ReceiverSpec(x=np.array([900.0]), z=np.array([500.0]))
+
+    def trace(compensation):
+        qo = QOptions(enabled=True, f_ref=25.0, compensation=compensation, alpha_c=1.0,
+                      amplitude_cap=1e6, lowpass_interval=100000, gain_budget_db=200.0)
+        return visco_traces(homogeneous_q(25.0), spec, source, receivers, qo)[:, 0]
+
+    att, amp = trace(False), trace(True)
+    freqs = np.fft.rfftfreq(spec.nt, spec.dt)
+    ratio = np.abs(np.fft.rfft(amp)) / np.maximum(np.abs(np.fft.rfft(att)), 1e-30)
+    travel = 400.0 / 3000.0
+    got = [float(ratio[int(np.argmin(np.abs(freqs - f)))]) for f in (10.0, 25.0, 40.0)]
+    for g, f in zip(got, (10.0, 25.0, 40.0)):
+        expected = float(np.exp(2.0 * np.pi * f * travel / 25.0))
+        assert abs(g / expected - 1.0) < 0.15, (f, g, expected)
+    assert got[2] > 2.0 * got[0]
+
+
+def test_band_limit_and_gain_budget_regularization():
+    """Mechanisms peaking above f_max get no compensation, the others some;
+    a 6 dB budget keeps every mechanism's strength below 1
+    (tests/test_rtm.py:541-576)."""
+    from ertm.propagation import QOptions, ViscoacousticPropagator
+
+    model = homogeneous_q(25.0)
+    spec = PropagationSpec(dt=4e-4, nt=2000, order=8, npml=12, dtype="float32", boundary="cpml")
+    limited = ViscoacousticPropagator(model, spec, q_options=QOptions(
+        enabled=True, f_ref=30.0, compensation=True, alpha_c=1.0, f_max=60.0,
+        gain_budget_db=40.0)).quality_summary
+    peaks, alphas = limited["f_peak_mechanisms_hz"], limited["alpha_mechanisms"]
+    assert len(peaks) == len(alphas)
+    for peak, alpha in zip(peaks, alphas):
+        assert (alpha == 0.0) if peak > 60.0 else (alpha > 0.0), (peak, alpha)
+    tight = ViscoacousticPropagator(model, spec, q_options=QOptions(
+        enabled=True, f_ref=30.0, compensation=True, alpha_c=1.0, f_max=None,
+        gain_budget_db=6.0)).quality_summary
+    assert all(a < 1.0 for a in tight["alpha_mechanisms"])
+
+
+def test_compensation_recovers_deep_amplitude():
+    """Compensated migration lifts the energy around the 600 m interface
+    (under Q = 15) by more than 15 % over the uncompensated one
+    (tests/test_rtm.py:431-473)."""
+    from ertm.config import QConfig
+    from ertm.imaging import build_imager
+
+    grid = GridSpec(nx=121, nz=101, dx=10.0, dz=10.0)
+    z = np.arange(grid.nz)[:, None] * grid.dz + 0.0 * np.arange(grid.nx)[None, :]
+    model = ElasticModel(grid=grid, vp=np.where(z < 600.0, 3000.0, 4200.0).astype(np.float32),
+                         vs=np.full(grid.shape, 1700.0, np.float32),
+                         rho=np.full(grid.shape, 2400.0, np.float32),
+                         qp=np.where(z < 600.0, 60.0, 15.0).astype(np.float32),
+                         qs=np.full(grid.shape, 50.0, np.float32))
+    solver = SolverConfig(nt=1500, dt=4e-4, order=8, boundary="free", dtype="float32",
+                          engine="viscoacoustic", npml=12)
+    spec = PropagationSpec.from_config(solver, model)
+    source = SourceSpec(x=600.0, z=20.0, f0=20.0, delay=0.06)
+    x = np.linspace(0.0, 1200.0, 81)
+    shot = forward_shot(model, source, ReceiverSpec(x=x, z=np.full_like(x, 20.0), component="p"),
+                        spec, engine="viscoacoustic")
+    shot = prepare_shot(shot, model=model, mute=True, v_reference=3000.0)
+
+    def band_energy(alpha_c):
+        q = QConfig(enabled=alpha_c > 0.0, model="gsls", alpha_c=max(alpha_c, 0.0))
+        image = build_imager(ImagingConfig(condition="normalized", laplacian=True,
+                                           lowpass_fraction=0.85, checkpoint_mb=64.0),
+                             solver, q).migrate(shot, model).image
+        return float(np.sqrt((np.abs(image[55:65, :]) ** 2).mean()))
+
+    baseline, compensated = band_energy(0.0), band_energy(1.0)
+    assert compensated > 1.15 * baseline, (baseline, compensated)

@@ -14,12 +14,15 @@ log-spaced frequencies, solved with SciPy's bounded trust-region solver
 
 from __future__ import annotations
 
+import logging
 from dataclasses import dataclass
 from typing import Iterable
 
 import numpy as np
 
 from .exceptions import ERTMError
+
+logger = logging.getLogger("ertm.viscoelastic")
 
 
 def normalized_modulus(tau_sigma: Iterable[float], delta: Iterable[float],
@@ -96,7 +99,8 @@ def fit_gsls(q: float, f_ref: float, *, n_mechanisms: int = 3, band_ratio: float
         return _params(q, f_ref, [tau], [dl], (lo, hi), 0.0)
     try:
         from scipy.optimize import least_squares
-    except Exception:  # noqa: BLE001 - scipy is a dependency; keep the closed form as last resort
+    except Exception:  # noqa: BLE001 - scipy is a dependency; the closed form as last resort
+        logger.warning("scipy unavailable: GSLS falls back to the single-mechanism closed form")
         tau, dl = single_mechanism(q, f_ref)
         return _params(q, f_ref, [tau], [dl], (lo, hi), 0.0)
     n = int(n_mechanisms)
@@ -116,7 +120,8 @@ def fit_gsls(q: float, f_ref: float, *, n_mechanisms: int = 3, band_ratio: float
     try:
         sol = least_squares(residual, x0, bounds=(x0 - 3.0, x0 + 3.0), max_nfev=400,
                             xtol=1e-10, ftol=1e-10)
-    except Exception:  # noqa: BLE001 - degenerate fit: closed form
+    except Exception as exc:  # noqa: BLE001 - degenerate fit: closed form
+        logger.warning("GSLS fit failed (%s): single-mechanism closed form", exc)
         tau, dl = single_mechanism(q, f_ref)
         return _params(q, f_ref, [tau], [dl], (lo, hi), 0.0)
     tau, dl = np.exp(sol.x[:n]), np.exp(sol.x[n:])

@@ -378,19 +378,33 @@ class RTMImager:
     def run_batch(self, job: "PreparedBatch") -> list[dict[str, ImageResult]]:
         """Forward, replay, backward and imaging of prepared shots; records are
         already in HBM.  Returns one {mode: ImageResult} per shot."""
+        return self.run_batches([job])[0]
+
+    def run_batches(self, jobs: list["PreparedBatch"]) -> list[list[dict[str, ImageResult]]]:
+        """``run_batch`` over several prepared batches, back to back: the host
+        post-processing of one launch group runs in a background thread while
+        the GPU advances the next, so a survey split into batches keeps the
+        GPU busy.  Each result equals what ``run_batch`` returns for its job."""
         t_start = time.perf_counter()
         engine = self.solver_config.engine
         condition = str(self.config.condition).lower()
-        groups: dict[tuple[int, int], list[int]] = {}
-        for i, (model, spec) in enumerate(zip(job.models, job.specs)):
-            groups.setdefault(self._batch_key(model, spec), []).append(i)
-        results: list[dict[str, ImageResult] | None] = [None] * len(job.shots)
         timing = {"forward_s": 0.0, "replay_backward_s": 0.0, "postprocess_s": 0.0,
                   "cell_updates": 0.0, "shots": 0.0, "launch_groups": 0.0}
         cap = int(self.gpu_config.max_batch)
-        for members in groups.values():
-            for c0 in range(0, len(members), cap):
-                idx = members[c0:c0 + cap]
+        outs: list[list[dict[str, ImageResult] | None]] = [[None] * len(j.shots) for j in jobs]
+        units = []
+        for jn, job in enumerate(jobs):
+            groups: dict[tuple[int, int], list[int]] = {}
+            for i, (model, spec) in enumerate(zip(job.models, job.specs)):
+                groups.setdefault(self._batch_key(model, spec), []).append(i)
+            for members in groups.values():
+                units.extend((jn, members[c0:c0 + cap]) for c0 in range(0, len(members), cap))
+        # device post-processing launches on this thread's stream: no overlap
+        overlap = not self.gpu_config.device_postprocess
+        pending = []
+        with ThreadPoolExecutor(max_workers=1) as poster:
+            for jn, idx in units:
+                job = jobs[jn]
                 imgs, infos = self._run_shots(
                     [job.models[i] for i in idx], [job.specs[i] for i in idx], engine,
                     [job.shots[i] for i in idx], [job.records[i] for i in idx],
@@ -399,31 +413,43 @@ class RTMImager:
                 for key in ("forward_s", "replay_backward_s", "cell_updates", "shots"):
                     timing[key] += self.last_timing.get(key, 0.0)
                 timing["launch_groups"] += 1.0
-                t_post = time.perf_counter()
-                # normalisation and post-processing per image, in host threads
-                # (NumPy releases the GIL in these array operations; every
-                # image is computed exactly as in a serial loop)
-                work = [(j, i, k) for j, i in enumerate(idx) for k in range(len(job.modes))]
-                finish = lambda w: self._finish_image(  # noqa: E731
-                    imgs[w[0]], w[2], condition, job.models[w[1]], job.specs[w[1]])
-                if self.gpu_config.device_postprocess:
-                    images = [finish(w) for w in work]  # kernels on this thread's stream
+                # metadata now, so that no device buffer of this group outlives it
+                metas = [[self._meta(job.specs[i], engine, job.shots[i], infos[j], condition, m)
+                          for m in job.modes] for j, i in enumerate(idx)]
+                del infos
+                args = (job, idx, imgs, metas, condition, outs[jn])
+                if overlap:
+                    pending.append(poster.submit(self._post_group, *args))
                 else:
-                    with ThreadPoolExecutor(max_workers=_host_workers(len(work))) as pool:
-                        images = list(pool.map(finish, work))
-                for (j, i, k), image in zip(work, images):
-                    m = job.modes[k]
-                    res = ImageResult(image=image, grid=job.models[i].grid, mode=m,
-                                      meta=self._meta(job.specs[i], engine, job.shots[i],
-                                                      infos[j], condition, m))
-                    res.validate()
-                    if results[i] is None:
-                        results[i] = {}
-                    results[i][m] = res
-                timing["postprocess_s"] += time.perf_counter() - t_post
+                    timing["postprocess_s"] += self._post_group(*args)
+            for fut in pending:
+                timing["postprocess_s"] += fut.result()
         timing["run_s"] = time.perf_counter() - t_start
         self.last_timing = timing
-        return results
+        return outs
+
+    def _post_group(self, job: "PreparedBatch", idx: list[int], imgs, metas, condition: str,
+                    results: list) -> float:
+        """Normalisation and post-processing of one launch group's images, in
+        host threads (NumPy releases the GIL in these array operations; every
+        image is computed exactly as in a serial loop).  Returns the time."""
+        t_post = time.perf_counter()
+        work = [(j, i, k) for j, i in enumerate(idx) for k in range(len(job.modes))]
+        finish = lambda w: self._finish_image(  # noqa: E731
+            imgs[w[0]], w[2], condition, job.models[w[1]], job.specs[w[1]])
+        if self.gpu_config.device_postprocess:
+            images = [finish(w) for w in work]  # kernels on this thread's stream
+        else:
+            with ThreadPoolExecutor(max_workers=_host_workers(len(work))) as pool:
+                images = list(pool.map(finish, work))
+        for (j, i, k), image in zip(work, images):
+            m = job.modes[k]
+            res = ImageResult(image=image, grid=job.models[i].grid, mode=m, meta=metas[j][k])
+            res.validate()
+            if results[i] is None:
+                results[i] = {}
+            results[i][m] = res
+        return time.perf_counter() - t_post
 
     def _batch_key(self, model: ElasticModel, spec: PropagationSpec) -> tuple[int, int]:
         """(nt, checkpoint interval): shots with equal keys run in one launch.

@@ -156,3 +156,32 @@ def test_forward_batch_equals_single_shots(tag):
         alone = solver.forward(model, s, rcv, components=comps)
         for name in comps:
             assert np.array_equal(rec.components[name], alone.components[name])
+
+
+@pytest.mark.parametrize("tag", ["ac_zero_lag", "el_ps_divcurl"])
+def test_pipelined_batches_equal_run_batch(tag):
+    """run_batches overlaps one group's host post-processing with the next
+    group's GPU passes (here three groups: max_batch 2 over 3 shots, then a
+    second job); every image and its metadata equal run_batch's."""
+    _say("pipelined batches", tag)
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case(tag)
+    shot, model, imaging, solver, q = migrate_objects(c)
+    imager = RTMImager(imaging, solver, q, GpuConfig(max_batch=2))
+    modes = ("pp", "ps") if c["engine"] == "elastic" else ("pp",)
+    job_a = imager.prepare_batch([shot, _moved(shot, 2 * model.grid.dx),
+                                  _moved(shot, -3 * model.grid.dx)], model, modes=modes)
+    job_b = imager.prepare_batch([_moved(shot, model.grid.dx)], model, modes=modes)
+    alone = [imager.run_batch(job_a), imager.run_batch(job_b)]
+    piped = imager.run_batches([job_a, job_b])
+    assert imager.last_timing["launch_groups"] == 3.0
+    for got_job, ref_job in zip(piped, alone):
+        assert len(got_job) == len(ref_job)
+        for got, ref in zip(got_job, ref_job):
+            for m in modes:
+                assert np.array_equal(got[m].image, ref[m].image)
+                assert got[m].meta == ref[m].meta
+    # the first shot is the golden one
+    assert np.array_equal(piped[0][0][c["imaging"]["mode"]].image, c["image"])

@@ -20,8 +20,10 @@ the kernels see exactly the reference's float32 inputs.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import hashlib
+import threading
 import weakref
 
 import numpy as np
@@ -88,17 +90,49 @@ except ImportError:  # pragma: no cover - xxhash ships with this image
     _xxhash = None
 
 # Device copies of medium arrays shared by every propagator built on the same
-# values (the shots of one survey share one model): one HBM copy, and the
-# batched step kernel, seeing equal medium pointers, orders its blocks so the
-# shots' blocks over one tile run together and read the medium through L2.
+# values (the shots of one survey share one model): one HBM copy, small
+# enough to stay in the Infinity Cache while a batched launch reads it for
+# every shot.
 _MEDIA: "weakref.WeakValueDictionary" = weakref.WeakValueDictionary()
+
+# Per-thread scope of model objects the caller has validated (see
+# ``trusted_models``), with the digests computed inside it.
+_TRUST = threading.local()
+
+
+@contextlib.contextmanager
+def trusted_models(models):
+    """Within the block, propagators built on these model objects (validated
+    by the caller and not modified meanwhile) skip re-validation, and each
+    array's digest is computed once.  The cache holds a reference to every
+    array it keys, so an id cannot be reused while the scope lasts."""
+    prev = getattr(_TRUST, "scope", None)
+    _TRUST.scope = {"ids": {id(m) for m in models}, "models": list(models), "digests": {}}
+    try:
+        yield
+    finally:
+        _TRUST.scope = prev
+
+
+def is_trusted(model) -> bool:
+    scope = getattr(_TRUST, "scope", None)
+    return scope is not None and id(model) in scope["ids"]
 
 
 def _digest(a: np.ndarray) -> bytes:
+    scope = getattr(_TRUST, "scope", None)
+    if scope is not None:
+        hit = scope["digests"].get(id(a))
+        if hit is not None and hit[0] is a:
+            return hit[1]
     buf = np.ascontiguousarray(a).view(np.uint8).reshape(-1)
     if _xxhash is not None:
-        return _xxhash.xxh3_128_digest(buf)
-    return hashlib.blake2b(buf, digest_size=16).digest()
+        d = _xxhash.xxh3_128_digest(buf)
+    else:
+        d = hashlib.blake2b(buf, digest_size=16).digest()
+    if scope is not None:
+        scope["digests"][id(a)] = (a, d)
+    return d
 
 
 def shared_padded_rows(a: np.ndarray, ld: int, key=None):

@@ -405,11 +405,14 @@ class RTMImager:
         with ThreadPoolExecutor(max_workers=1) as poster:
             for jn, idx in units:
                 job = jobs[jn]
-                imgs, infos = self._run_shots(
-                    [job.models[i] for i in idx], [job.specs[i] for i in idx], engine,
-                    [job.shots[i] for i in idx], [job.records[i] for i in idx],
-                    [job.waves[i] for i in idx], job.swave, job.rwaves, condition,
-                    self.config.separation)
+                # prepare_batch validated the models: the propagators built on
+                # them skip re-validation and share each array's digest
+                with dev.trusted_models([job.models[i] for i in idx]):
+                    imgs, infos = self._run_shots(
+                        [job.models[i] for i in idx], [job.specs[i] for i in idx], engine,
+                        [job.shots[i] for i in idx], [job.records[i] for i in idx],
+                        [job.waves[i] for i in idx], job.swave, job.rwaves, condition,
+                        self.config.separation)
                 for key in ("forward_s", "replay_backward_s", "cell_updates", "shots"):
                     timing[key] += self.last_timing.get(key, 0.0)
                 timing["launch_groups"] += 1.0

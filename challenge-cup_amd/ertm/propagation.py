@@ -172,7 +172,8 @@ class Propagator2D:
     _engine = ""
 
     def __init__(self, model: ElasticModel, spec: PropagationSpec) -> None:
-        model.validate()
+        if not dev.is_trusted(model):  # batches validate their models once
+            model.validate()
         self.model = model
         self.spec = spec
         self.grid = model.grid

@@ -439,3 +439,49 @@ def test_migrate_with_captured_time_loop_matches_reference(tag, kv, ring, gsteps
     # the cached graphs are replayed again by a second migration
     again = imager.migrate(shot, model)
     assert np.array_equal(again.image, c["image"])
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_fused_variants_on_random_small_grids_match_two_launch(seed):
+    """Odd grid sizes (down to a few chunks of 8 rows), every boundary, every
+    engine and elastic source mechanism, sources near edges and corners: the
+    whole state (fields and CPML memories) of every fused launch shape equals
+    the two-launch kernels' after 40 steps."""
+    import torch
+    from ertm import device as dev
+    from ertm.propagation import PropagationSpec, QOptions, build_propagator
+    from ertm.types import ElasticModel, GridSpec, SourceSpec
+
+    rng = np.random.default_rng(100 + seed)
+    engine = ("elastic", "acoustic", "viscoacoustic")[seed % 3]
+    boundary = ("cpml", "free", "none")[(seed // 3 + seed) % 3]
+    nz, nx = int(rng.integers(23, 90)), int(rng.integers(19, 260))
+    mech = ("explosive", "shear", "force_z")[seed % 3] if engine == "elastic" else "explosive"
+    _say("random small grid", seed, engine, boundary, nz, nx, mech)
+    vp = rng.uniform(2000, 4000, (nz, nx)).astype(np.float32)
+    m = ElasticModel(GridSpec(nx=nx, nz=nz, dx=5.0, dz=4.0), vp=vp,
+                     vs=(vp / rng.uniform(1.6, 2.0, (nz, nx))).astype(np.float32),
+                     rho=rng.uniform(1800, 2600, (nz, nx)).astype(np.float32),
+                     qp=rng.choice([30.0, 60.0, 120.0, 200.0], (nz, nx)).astype(np.float32),
+                     qs=np.full_like(vp, 50.0))
+    npml = int(rng.integers(5, 12))
+    spec = PropagationSpec(dt=3e-4, nt=40, order=8, npml=npml, boundary=boundary)
+    sx = float(rng.choice([0.0, (nx - 1) * 5.0, rng.uniform(0, (nx - 1) * 5.0)]))
+    # one cell below the top: a source on a free-surface row is zeroed every step
+    sz = float(rng.choice([4.0, (nz - 1) * 4.0, rng.uniform(4.0, (nz - 1) * 4.0)]))
+    src = SourceSpec(x=sx, z=sz, f0=30.0, delay=0.01, mechanism=mech)
+    q = QOptions(enabled=True, f_ref=30.0) if engine == "viscoacoustic" else None
+    w = (np.sin(np.arange(40) * 0.4) * 1e5).astype(np.float32)
+    shapes = [(0, 64), (1, 9), (2, 8), (3, 13), (4, 48), (2, 31), (4, 10)]
+    outs = []
+    for v in shapes:
+        p = build_propagator(m, spec, engine=engine, q_options=q)
+        p.set_source(src)
+        p._batches["step"] = dev.Batch([p._core], variant=v)
+        for s in range(40):
+            p.step(float(w[s]))
+        torch.cuda.synchronize()
+        outs.append(p._core.state[p._core.cur].clone())
+    assert float(outs[0].abs().max()) > 0.0
+    for v, o in zip(shapes[1:], outs[1:]):
+        assert torch.equal(o, outs[0]), v

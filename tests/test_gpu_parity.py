@@ -485,3 +485,51 @@ def test_fused_variants_on_random_small_grids_match_two_launch(seed):
     assert float(outs[0].abs().max()) > 0.0
     for v, o in zip(shapes[1:], outs[1:]):
         assert torch.equal(o, outs[0]), v
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_fused_imaging_on_random_small_grids_matches_two_launch(seed):
+    """Random odd-sized grids: migration with imaging inside the fused step
+    (replay-slot extraction, receiver ring of random length, flush) equals
+    migration on the two-launch kernels with a separate imaging pass, bit
+    for bit, for PP and PS (elastic) or PP (acoustic) and every condition."""
+    from ertm.config import GpuConfig, ImagingConfig, SolverConfig
+    from ertm.imaging import RTMImager
+    from ertm.propagation import PropagationSpec, forward_shot
+    from ertm.types import ElasticModel, GridSpec, ReceiverSpec, SourceSpec
+
+    rng = np.random.default_rng(200 + seed)
+    engine = ("elastic", "acoustic")[seed % 2]
+    boundary = ("cpml", "free")[(seed // 2) % 2]
+    condition = ("normalized", "zero_lag", "excitation")[seed % 3]
+    nz, nx = int(rng.integers(30, 80)), int(rng.integers(25, 200))
+    _say("random small migration", seed, engine, boundary, condition, nz, nx)
+    vp = rng.uniform(2000, 3500, (nz, nx)).astype(np.float32)
+    vp[nz // 2:] += 800.0  # an interface to image
+    m = ElasticModel(GridSpec(nx=nx, nz=nz, dx=5.0, dz=5.0), vp=vp,
+                     vs=(vp / 1.8).astype(np.float32),
+                     rho=rng.uniform(1900, 2500, (nz, nx)).astype(np.float32),
+                     qp=np.full_like(vp, 100.0), qs=np.full_like(vp, 50.0))
+    nt = 400
+    solver = SolverConfig(nt=nt, dt=3e-4, order=8, boundary=boundary, engine=engine, npml=10)
+    src = SourceSpec(x=float(rng.uniform(0.2, 0.8) * (nx - 1) * 5.0), z=10.0, f0=30.0,
+                     delay=0.035)
+    x = np.linspace(0.0, (nx - 1) * 5.0, max(5, nx // 3))
+    comps = ("vx", "vz") if engine == "elastic" else ("p",)
+    rec = ReceiverSpec(x=x, z=np.full_like(x, 10.0), component=comps[-1])
+    shot = forward_shot(m, src, rec, PropagationSpec.from_config(solver, m), engine=engine,
+                        components=comps)
+    modes = ("pp", "ps") if engine == "elastic" and condition != "excitation" else ("pp",)
+    # a budget of four fields-only states: four replay blocks
+    per_state = (5 if engine == "elastic" else 3) * (nz + 10) * (nx + 10) * 4
+    imaging = ImagingConfig(mode="pp", condition=condition, checkpoint_mb=4 * per_state / 2**20)
+    ref = RTMImager(imaging, solver, None, GpuConfig(kernel_variant=(0, 64))).migrate_batch(
+        [shot], m, modes=modes)[0]
+    ring = int(rng.choice([1, 5, 16, 32]))
+    for kv in [(2, 8), (4, 13), (1, 30)]:
+        got = RTMImager(imaging, solver, None,
+                        GpuConfig(kernel_variant=kv, image_ring=ring)).migrate_batch(
+            [shot], m, modes=modes)[0]
+        for md in modes:
+            assert np.array_equal(got[md].image, ref[md].image), (kv, ring, md)
+        assert np.abs(ref[modes[0]].image).max() > 0.0

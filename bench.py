@@ -273,6 +273,60 @@ def c5_sweep(budgets_mb, nt):
     return out
 
 
+def engine_sweep(nshots, steps, warmup=3):
+    """Plain step throughput of every engine's single-launch kernel on the
+    64-shot workload's grid (600 x 1200, the layered model shared by all
+    shots, free surface and 20-cell CPML, no imaging): one JSON line per
+    engine with the launch shape, microseconds per launch (HIP events on the
+    launch stream), cell-updates/s and the fraction of the 8 TB/s roofline at
+    the engine's algorithmic bytes per cell-update (acoustic 32, elastic 52,
+    viscoacoustic with 3 mechanisms 68)."""
+    import torch
+    from ertm import device as dev
+    from ertm import tuning
+    from ertm.propagation import PropagationSpec, QOptions, build_propagator
+
+    model = layered_model()
+    spec = PropagationSpec(dt=DT, nt=steps + warmup, order=8, npml=20, boundary="free")
+    out = []
+    for engine, bpc in (("acoustic", 32), ("elastic", 52), ("viscoacoustic", 68)):
+        q = QOptions(enabled=True, f_ref=F0, n_mechanisms=3) if engine == "viscoacoustic" else None
+        props = [build_propagator(model, spec, engine=engine, q_options=q) for _ in range(nshots)]
+        cores = [p._core for p in props]
+        b = dev.Batch(cores)
+        tuning.maybe_autotune(b, 1e30)
+        c0 = cores[0]
+        b.set_variant(*tuning.lookup(engine, 8, c0.nzp, c0.nxp, nshots, c0.nmech))
+        amps = [0.0] * nshots
+        for c in cores:
+            c.state[c.cur].normal_(0.0, 1e-3)
+        for _ in range(warmup):
+            b.step(c0.cur, amps)
+            for c in cores:
+                c.cur ^= 1
+        torch.cuda.synchronize()
+        with dev.StepTimer():
+            for _ in range(steps):
+                b.step(c0.cur, amps)
+                for c in cores:
+                    c.cur ^= 1
+            torch.cuda.synchronize()
+        kt = dev.StepTimer.read()
+        us = kt["total_ms"] / max(kt["launches"], 1) * 1e3
+        cells = NZ * NX * nshots
+        line = {"workload": f"{engine} step, {nshots} shots of {NZ}x{NX}",
+                "launch_shape": list(b.variant), "launches": kt["launches"],
+                "us_per_launch": round(us, 2),
+                "gcell_updates_per_s": round(cells / (us * 1e-6) / 1e9, 2),
+                "bytes_per_cell_update": bpc,
+                "roofline_frac": round(cells * bpc / (us * 1e-6) / (HBM_PEAK_GBS * 1e9), 4)}
+        print(json.dumps(line), flush=True)
+        out.append(line)
+        b.close()
+        del props, cores, b
+    return out
+
+
 def read_traffic(shape):
     """HBM bytes per step launch of launch shape ``shape`` = (variant, rows
     per chunk) from the committed FETCH_SIZE/WRITE_SIZE profile summary (a
@@ -334,6 +388,10 @@ def main(argv=None) -> int:
                          "(2000 x 6000, nt 10000), excitation vs zero_lag at checkpoint "
                          "budgets 1/4/16/64 GB, one JSON line per run")
     ap.add_argument("--c5-nt", type=int, default=10000)
+    ap.add_argument("--engine-sweep", action="store_true",
+                    help="instead of the RTM bench: plain step throughput of each engine's "
+                         "single-launch kernel on the 64-shot grid")
+    ap.add_argument("--sweep-steps", type=int, default=200)
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="also time this many shots migrated one at a time (the plain per-shot "
                          "path) for the batched/plain ratio; 0 = skip")
@@ -366,6 +424,9 @@ def main(argv=None) -> int:
 
     if args.c5_sweep:
         c5_sweep([1024, 4096, 16384, 65536], args.c5_nt)
+        return 0
+    if args.engine_sweep:
+        engine_sweep(args.shots, args.sweep_steps)
         return 0
 
     if args.copy_probe_mib > 0:

@@ -371,7 +371,7 @@ def cpu_baseline(model, sources, receivers, sample_nt):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=NT)

@@ -462,6 +462,44 @@ static void use_clock(LaunchArgs& la, const DevClock* dc, int qoff = 0) {
     la.qoff = dc->q + qoff;
 }
 
+// Copy state[cur] of every propagator to (to_arena) or from its arena at
+// arenas[k] + slot * stride.  Aligned batches go through one k_copy_batch
+// launch per ERTM_MAX_LAUNCH propagators; otherwise one async copy each.
+static int copy_states(ertm_batch_t b, int cur, float* const* arenas, int64_t slot,
+                       int64_t stride, bool to_arena, hipStream_t s) {
+    const int n = (int)b->props.size();
+    bool fast = true;
+    for (int k = 0; k < n && fast; ++k) {
+        const ErtmProp& P = b->props[k];
+        fast = P.state_floats % 4 == 0 &&
+               ((uintptr_t)P.state[cur] | (uintptr_t)(arenas[k] + slot * stride)) % 16 == 0;
+    }
+    if (!fast) {
+        for (int k = 0; k < n; ++k) {
+            const ErtmProp& P = b->props[k];
+            float* a = arenas[k] + slot * stride;
+            HIPCHK(hipMemcpyAsync(to_arena ? a : P.state[cur], to_arena ? P.state[cur] : a,
+                                  sizeof(float) * (size_t)P.state_floats,
+                                  hipMemcpyDeviceToDevice, s));
+        }
+        return ERTM_OK;
+    }
+    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+        CopyArgs a{};
+        a.n = std::min(ERTM_MAX_LAUNCH, n - base);
+        for (int k = 0; k < a.n; ++k) {
+            const ErtmProp& P = b->props[base + k];
+            float* ar = arenas[base + k] + slot * stride;
+            a.src[k] = (const float4*)(to_arena ? P.state[cur] : ar);
+            a.dst[k] = (float4*)(to_arena ? ar : P.state[cur]);
+            a.n4[k] = P.state_floats / 4;
+        }
+        launch_copy_batch(a, s);
+        LAUNCHCHK("copy_batch");
+    }
+    return ERTM_OK;
+}
+
 static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t s,
                     const DevClock* dc = nullptr) {
     const int n = (int)b->props.size();
@@ -810,14 +848,9 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
     int cur = r->cur0 & 1;
     int step_index = r->step_index0;
     int rc = ERTM_OK;
-    if (r->phase == 1) {
-        for (int k = 0; k < n; ++k) {
-            const ErtmProp& P = b->props[k];
-            HIPCHK(hipMemcpyAsync(P.state[cur],
-                                  r->restore_arenas[k] + (int64_t)r->restore_slot * r->restore_stride,
-                                  sizeof(float) * (size_t)P.state_floats, hipMemcpyDeviceToDevice, s));
-        }
-    }
+    if (r->phase == 1 && (rc = copy_states(b, cur, r->restore_arenas, r->restore_slot,
+                                           r->restore_stride, false, s)) != ERTM_OK)
+        return rc;
     const int nsteps = r->t_end - r->t_begin;
     // backward correlation through the receiver-field ring
     const bool use_ring = r->phase == 2 && r->op_after == ERTM_IMG_CORRELATE && !r->stabilize &&
@@ -879,12 +912,9 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
             } else if (r->ck_interval > 0 && t >= r->ck_origin &&
                        (t - r->ck_origin) % r->ck_interval == 0) {
                 const int64_t slot = (t - r->ck_origin) / r->ck_interval;
-                for (int k = 0; k < n; ++k) {
-                    const ErtmProp& P = b->props[k];
-                    HIPCHK(hipMemcpyAsync(r->save_arenas[k] + slot * r->save_stride, P.state[cur],
-                                          sizeof(float) * (size_t)P.state_floats,
-                                          hipMemcpyDeviceToDevice, s));
-                }
+                if ((rc2 = copy_states(b, cur, r->save_arenas, slot, r->save_stride, true, s)) !=
+                    ERTM_OK)
+                    return rc2;
             }
         }
         if (r->op_after >= 0 && !imaged) {

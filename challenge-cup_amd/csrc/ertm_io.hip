@@ -652,6 +652,29 @@ __global__ void k_checkpoint(const ErtmProp* __restrict__ props, LaunchArgs la,
         dst[i] = src[i];
 }
 
+// Eager checkpoint saves and restores of a whole batch in one launch
+// (blockIdx.y = propagator): one hipMemcpyAsync per shot ran a 64-shot
+// 600x1200 elastic save (2.3 GB) at a few hundred GB/s.
+__global__ void k_copy_batch(CopyArgs a) {
+    const int k = blockIdx.y;
+    const float4* __restrict__ src = a.src[k];
+    float4* __restrict__ dst = a.dst[k];
+    const int64_t n4 = a.n4[k];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy_batch(const CopyArgs& a, hipStream_t s) {
+    int64_t most = 1;
+    for (int k = 0; k < a.n; ++k) most = std::max<int64_t>(most, a.n4[k]);
+    // about 8192 blocks over the batch: 32 per CU, enough loads in flight to
+    // stream HBM, and never more blocks than a propagator has float4 groups
+    const int64_t per = std::max<int64_t>(1, 8192 / std::max(a.n, 1));
+    const int64_t gx = std::min<int64_t>(per, (most + 255) / 256);
+    hipLaunchKernelGGL(k_copy_batch, dim3((unsigned)gx, a.n), dim3(256), 0, s, a);
+}
+
 void launch_checkpoint(const ErtmProp* dprops, const LaunchArgs& la, float* const* darenas,
                        int64_t stride, int origin, int interval, int64_t max_floats,
                        hipStream_t s) {

@@ -79,6 +79,15 @@ void launch_tick(int32_t* clk, int steps, hipStream_t s);
 void launch_checkpoint(const ErtmProp* dprops, const LaunchArgs& la, float* const* darenas,
                        int64_t stride, int origin, int interval, int64_t max_floats,
                        hipStream_t s);
+// n copies of n4[k] float4 groups, src[k] -> dst[k], in one launch (both
+// 16-byte aligned); the host checks alignment and size before using it
+struct CopyArgs {
+    int32_t n;
+    const float4* src[ERTM_MAX_LAUNCH];
+    float4* dst[ERTM_MAX_LAUNCH];
+    int64_t n4[ERTM_MAX_LAUNCH];
+};
+void launch_copy_batch(const CopyArgs& a, hipStream_t s);
 void launch_derivative(const void* f, void* out, int n0, int n1, int axis, int plus, int order,
                        const DerivCoef& c, double h, int is_f64, hipStream_t s);
 // Viscoacoustic stabiliser pieces (ertm/propagation.py:726-749).

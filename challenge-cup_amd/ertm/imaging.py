@@ -437,6 +437,7 @@ class RTMImager:
         host threads (NumPy releases the GIL in these array operations; every
         image is computed exactly as in a serial loop).  Returns the time."""
         t_post = time.perf_counter()
+        imgs = self._fetch_host(imgs)
         work = [(j, i, k) for j, i in enumerate(idx) for k in range(len(job.modes))]
         finish = lambda w: self._finish_image(  # noqa: E731
             imgs[w[0]], w[2], condition, job.models[w[1]], job.specs[w[1]])
@@ -627,10 +628,10 @@ class RTMImager:
                 # accumulators stay in HBM; _finish_image runs ertm_postprocess
                 all_outs.append(("device", list(num_t), ill, list(rill)))
             else:
-                # raw float64 accumulators on the host; normalised by _finish_image
-                all_outs.append(([x.cpu().numpy() for x in num_t],
-                                 ill.cpu().numpy() if use_ill else None,
-                                 [r.cpu().numpy() for r in rill] if use_rill else None))
+                # raw float64 accumulators, copied to the host by _fetch_host
+                # and normalised by _finish_image
+                all_outs.append(("fetch", list(num_t), ill if use_ill else None,
+                                 list(rill) if use_rill else None))
             nz, nx = models[j].shape
             infos.append(dict(store=stores[j], start_step=starts[j], prop=fwds[j], back=backs[j],
                               replayed_steps=replayed, nt=nt,
@@ -694,6 +695,34 @@ class RTMImager:
         padded = np.zeros((nt, data.shape[1]), dtype=data.dtype)
         padded[: data.shape[0]] = data
         return padded
+
+    @staticmethod
+    def _fetch_host(imgs: list) -> list:
+        """Host copies of a launch group's float64 accumulators.
+
+        The copies run on a stream of their own: in ``run_batches`` the next
+        group's launches are already queued on the launching stream, and the
+        copies must not wait behind them.  ``_run_shots`` synchronised its
+        stream before returning, so the accumulators are complete; the side
+        stream is synchronised before the device tensors are released."""
+        if not any(raw[0] == "fetch" for raw in imgs):
+            return imgs
+        import torch
+
+        dev_t = next(raw[1][0] for raw in imgs if raw[0] == "fetch")
+        side = torch.cuda.Stream(device=dev_t.device)
+        out = []
+        with torch.cuda.stream(side):
+            for raw in imgs:
+                if raw[0] != "fetch":
+                    out.append(raw)
+                    continue
+                _tag, nums, ill, rill = raw
+                out.append(([x.cpu().numpy() for x in nums],
+                            None if ill is None else ill.cpu().numpy(),
+                            None if rill is None else [r.cpu().numpy() for r in rill]))
+        side.synchronize()
+        return out
 
     def _finish_image(self, raw, k: int, condition: str, model: ElasticModel,
                       spec: PropagationSpec) -> np.ndarray:

@@ -147,6 +147,18 @@ def shared_padded_rows(a: np.ndarray, ld: int, key=None):
     return out
 
 
+def shared_vector(a: np.ndarray):
+    """``to_device`` through the cache of ``shared_padded_rows``: the CPML
+    profiles are the same for every propagator on one grid, spacing and time
+    step, and kernels only read them."""
+    key = ("vector", _digest(a), a.shape, a.dtype.str)
+    out = _MEDIA.get(key)
+    if out is None:
+        out = to_device(a)
+        _MEDIA[key] = out
+    return out
+
+
 def media_digest(*parts) -> tuple:
     """Key of a medium: the engine, padding and the bytes of the model arrays
     it is computed from."""
@@ -198,7 +210,7 @@ class Core:
         self.wzt = wz if (enabled and not free) else 0
         self.wzb = wz if enabled else 0
         self.prof = {"ax": ax, "bx": bx, "az": az, "bz": bz}
-        self.d_prof = {k: to_device(v) for k, v in self.prof.items()}
+        self.d_prof = {k: shared_vector(v) for k, v in self.prof.items()}
         self.field_names = FIELDS[engine]
         self.nfields = len(self.field_names) + (self.nmech if engine == "viscoacoustic" else 0)
         self.fstride = _rup(self.nzp * self.ld, 64)
@@ -384,6 +396,23 @@ class RecvTables:
         self.keep.append(t)
         return L.ptr(t)
 
+    def _dev_pack(self, items) -> list:
+        """Several small host arrays ((array, dtype) pairs) in one upload:
+        pointers into one device buffer, each array on a 256-byte boundary.
+        One copy instead of one per array, for the tables of many shots."""
+        arrs = [np.ascontiguousarray(np.asarray(a, dtype=dt)).reshape(-1) for a, dt in items]
+        offs, total = [], 0
+        for a in arrs:
+            offs.append(total)
+            total += _rup(max(a.nbytes, 1), 256)
+        host = np.zeros(total, dtype=np.uint8)
+        for a, o in zip(arrs, offs):
+            host[o:o + a.nbytes] = a.view(np.uint8)
+        t = to_device(host)
+        self.keep.append(t)
+        base = L.ptr(t)
+        return [base + o for o in offs]
+
     def set_records(self, records: list):
         """records[k]: (nt, nrec) float32 device tensors, one per component."""
         self.s.ncomp = len(records)
@@ -409,14 +438,11 @@ class RecvTables:
         s.nrec = nrec
         s.nsamp = nrec * len(comps)
         cat = np.concatenate
-        s.s_i0 = self._dev(cat(i0), np.int32)
-        s.s_j0 = self._dev(cat(j0), np.int32)
-        s.s_field = self._dev(cat(fld), np.int32)
-        s.s_rec = self._dev(cat(rec), np.int32)
-        s.s_col = self._dev(cat(col), np.int32)
-        s.s_valid = self._dev(cat(val), np.int32)
-        s.s_tz = self._dev(cat(tz), np.float64)
-        s.s_tx = self._dev(cat(tx), np.float64)
+        (s.s_i0, s.s_j0, s.s_field, s.s_rec, s.s_col, s.s_valid, s.s_tz,
+         s.s_tx) = self._dev_pack([(cat(i0), np.int32), (cat(j0), np.int32),
+                                   (cat(fld), np.int32), (cat(rec), np.int32),
+                                   (cat(col), np.int32), (cat(val), np.int32),
+                                   (cat(tz), np.float64), (cat(tx), np.float64)])
 
     def build_injection(self, core: Core, rx, rz, comps, k_relaxed: np.ndarray | None = None):
         """Ordered scatter tables reproducing np.add.at (ertm/propagation.py:331-369,
@@ -463,13 +489,13 @@ class RecvTables:
         ptrs = np.concatenate([start, [key.size]]).astype(np.int32)
         s.ncells = int(cells.size)
         s.nentries = int(key.size)
-        s.cell_ptr = self._dev(ptrs, np.int32)
-        s.cell_field = self._dev(cells // plane, np.int32)
-        s.cell_off = self._dev(cells % plane, np.int64)
-        s.e_comp = self._dev(np.concatenate(comp_i)[order], np.int32)
-        s.e_rec = self._dev(np.concatenate(rec_i)[order], np.int32)
-        s.e_wa = self._dev(np.concatenate(wa_l)[order], np.float64)
-        s.e_wb = self._dev(np.concatenate(wb_l)[order], np.float64)
+        (s.cell_ptr, s.cell_field, s.cell_off, s.e_comp, s.e_rec, s.e_wa,
+         s.e_wb) = self._dev_pack([(ptrs, np.int32), (cells // plane, np.int32),
+                                   (cells % plane, np.int64),
+                                   (np.concatenate(comp_i)[order], np.int32),
+                                   (np.concatenate(rec_i)[order], np.int32),
+                                   (np.concatenate(wa_l)[order], np.float64),
+                                   (np.concatenate(wb_l)[order], np.float64)])
 
 
 class ImageBuffers:

@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import math
 import os
+import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -153,13 +154,14 @@ def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
 
 
 def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarray], *,
-                             budget_mb: float, variant=None,
-                             graph_steps: int = 0) -> list[CheckpointStore]:
+                             budget_mb: float, variant=None, graph_steps: int = 0,
+                             during=None) -> list[CheckpointStore]:
     """``record_source_wavefield`` for several shots advanced in one launch per step.
 
     All shots must share nt and the checkpoint interval (the interval follows
     from nt and the fields-only state size, so shots on grids of the same
-    size always do).
+    size always do).  ``during``, if given, is called once the pass has been
+    launched and before waiting for it (host set-up to overlap with it).
     """
     import torch
 
@@ -195,7 +197,11 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     kw = batch_run_kwargs(props)
     cur = b.run(0, 0, nt, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
                 save=arenas, save_stride=stride, step_index0=step0.pop(), **kw)
-    b.close()
+    try:
+        if during is not None:
+            during()
+    finally:
+        b.close()
     batch_after_run(props, kw)
     stores = []
     for p, c, arena, ps in zip(props, cores, arenas, per_state):
@@ -206,6 +212,27 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
                                       interval=interval, bytes_per_state=ps, arena=arena,
                                       state_floats=stride))
     return stores
+
+
+_SIDE_STREAMS: dict = {}
+_SIDE_LOCK = threading.Lock()
+
+
+def side_stream(role: str):
+    """A stream beside the calling thread's current stream, one per role,
+    kept for the life of the process (default priority).  Work on it
+    overlaps a pass queued on the current stream; because it is reused,
+    device blocks the caching allocator keeps for it are reused as well."""
+    import torch
+
+    cur = torch.cuda.current_stream()
+    key = (role, cur.device_index, cur.cuda_stream)
+    with _SIDE_LOCK:
+        s = _SIDE_STREAMS.get(key)
+        if s is None:
+            s = torch.cuda.Stream(device=cur.device)
+            _SIDE_STREAMS[key] = s
+    return s
 
 
 def _host_workers(n_items: int) -> int:
@@ -505,21 +532,30 @@ class RTMImager:
             fwds.append(fwd)
         kv = self.gpu_config.kernel_variant
         gsteps = int(self.gpu_config.graph_steps)
+        side = side_stream("setup")
+
+        def build_backs():
+            # while the forward pass runs: the uploads of this set-up go
+            # through a stream of their own instead of waiting behind it
+            with torch.cuda.stream(side):
+                for model, spec, shot, record in zip(models, specs, shots, records):
+                    back = build_propagator(model, spec, engine=engine,
+                                            q_options=self.q_options(compensation=True))
+                    back.set_source(shot.source)
+                    back.set_receivers(shot.receivers)
+                    names = list(record)
+                    recs = [record[nm] for nm in names]
+                    tab = dev.RecvTables()
+                    tab.build_injection(back._core, shot.receivers.x, shot.receivers.z,
+                                        [self._routing(nm) for nm in names],
+                                        back._k_relaxed_host())
+                    tab.set_records(recs)
+                    backs.append(back)
+                    tabs.append(tab)
+
         stores = record_source_wavefields(fwds, waves, budget_mb=float(self.config.checkpoint_mb),
-                                          variant=kv, graph_steps=gsteps)
-        for model, spec, shot, record in zip(models, specs, shots, records):
-            back = build_propagator(model, spec, engine=engine,
-                                    q_options=self.q_options(compensation=True))
-            back.set_source(shot.source)
-            back.set_receivers(shot.receivers)
-            names = list(record)
-            recs = [record[nm] for nm in names]
-            tab = dev.RecvTables()
-            tab.build_injection(back._core, shot.receivers.x, shot.receivers.z,
-                                [self._routing(nm) for nm in names], back._k_relaxed_host())
-            tab.set_records(recs)
-            backs.append(back)
-            tabs.append(tab)
+                                          variant=kv, graph_steps=gsteps, during=build_backs)
+        torch.cuda.current_stream().wait_stream(side)
         starts = []
         for shot, spec in zip(shots, specs):
             st = int(np.floor((float(shot.source.delay) + float(self.config.start_periods)
@@ -710,7 +746,8 @@ class RTMImager:
         import torch
 
         dev_t = next(raw[1][0] for raw in imgs if raw[0] == "fetch")
-        side = torch.cuda.Stream(device=dev_t.device)
+        with torch.cuda.device(dev_t.device):
+            side = side_stream("fetch")
         out = []
         with torch.cuda.stream(side):
             for raw in imgs:

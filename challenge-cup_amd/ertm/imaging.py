@@ -218,15 +218,24 @@ _SIDE_STREAMS: dict = {}
 _SIDE_LOCK = threading.Lock()
 
 
-def side_stream(role: str):
-    """A stream beside the calling thread's current stream, one per role,
-    kept for the life of the process (default priority).  Work on it
-    overlaps a pass queued on the current stream; because it is reused,
-    device blocks the caching allocator keeps for it are reused as well."""
+def side_stream(role: str, launch=None):
+    """The stream for set-up or copy work beside passes launched on ``launch``
+    (default: the current stream).
+
+    For the device's default stream: a side stream, one per role and device,
+    kept for the life of the process (default priority), so that the work
+    overlaps passes queued on the default stream and device blocks the
+    caching allocator keeps for it are reused.  For any other stream:
+    ``launch`` itself.  Callers that launch on streams of their own (the
+    serving workers) already run several streams, and with a few hardware
+    queues per process more streams would share queues with other workers'
+    kernels and wait behind them."""
     import torch
 
-    cur = torch.cuda.current_stream()
-    key = (role, cur.device_index, cur.cuda_stream)
+    cur = torch.cuda.current_stream() if launch is None else launch
+    if cur.cuda_stream != torch.cuda.default_stream(cur.device).cuda_stream:
+        return cur
+    key = (role, cur.device_index)
     with _SIDE_LOCK:
         s = _SIDE_STREAMS.get(key)
         if s is None:
@@ -667,7 +676,8 @@ class RTMImager:
                 # raw float64 accumulators, copied to the host by _fetch_host
                 # and normalised by _finish_image
                 all_outs.append(("fetch", list(num_t), ill if use_ill else None,
-                                 list(rill) if use_rill else None))
+                                 list(rill) if use_rill else None,
+                                 torch.cuda.current_stream()))
             nz, nx = models[j].shape
             infos.append(dict(store=stores[j], start_step=starts[j], prop=fwds[j], back=backs[j],
                               replayed_steps=replayed, nt=nt,
@@ -736,25 +746,26 @@ class RTMImager:
     def _fetch_host(imgs: list) -> list:
         """Host copies of a launch group's float64 accumulators.
 
-        The copies run on a stream of their own: in ``run_batches`` the next
-        group's launches are already queued on the launching stream, and the
-        copies must not wait behind them.  ``_run_shots`` synchronised its
-        stream before returning, so the accumulators are complete; the side
-        stream is synchronised before the device tensors are released."""
+        For groups launched on the default stream the copies run on a side
+        stream: in ``run_batches`` the next group's launches are already
+        queued on the launching stream, and the copies must not wait behind
+        them (``side_stream`` says when the launching stream is used
+        instead).  ``_run_shots`` synchronised its stream before returning,
+        so the accumulators are complete; the copy stream is synchronised
+        before the device tensors are released."""
         if not any(raw[0] == "fetch" for raw in imgs):
             return imgs
         import torch
 
-        dev_t = next(raw[1][0] for raw in imgs if raw[0] == "fetch")
-        with torch.cuda.device(dev_t.device):
-            side = side_stream("fetch")
+        launch = next(raw[4] for raw in imgs if raw[0] == "fetch")
+        side = side_stream("fetch", launch)
         out = []
         with torch.cuda.stream(side):
             for raw in imgs:
                 if raw[0] != "fetch":
                     out.append(raw)
                     continue
-                _tag, nums, ill, rill = raw
+                _tag, nums, ill, rill, _launch = raw
                 out.append(([x.cpu().numpy() for x in nums],
                             None if ill is None else ill.cpu().numpy(),
                             None if rill is None else [r.cpu().numpy() for r in rill]))

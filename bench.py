@@ -18,9 +18,10 @@ One JSON line on stdout:
                  divided by its mean launch time, measured with HIP events
                  on the launch stream during the timed steps, against 8 TB/s
   cpu_baseline   the NumPy oracle, one core, on a truncated sample of one
-                 shot, scaled to shots/s
+                 shot, scaled to shots/s; "pool": the same sample on N shots
+                 at once in a process pool (N = affinity cores, at most 16)
 
-  python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--shots 64] [--nt 4000]
+  python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--shots 64] [--nt 4000]
 """
 
 from __future__ import annotations
@@ -347,25 +348,64 @@ def cpu_baseline(model, sources, receivers, sample_nt):
     ``sample_nt`` steps (forward + replay + backward with imaging every step),
     scaled by nt/sample_nt and by 2 because the reference images PP and PS
     with two migrate calls."""
-    from oracle import rtm as orc
-
-    m = orc.Model(nz=NZ, nx=NX, dx=DX, dz=DX, vp=np.asarray(model.vp), vs=np.asarray(model.vs),
-                  rho=np.asarray(model.rho), qp=np.asarray(model.qp))
-    s = sources[len(sources) // 2]
-    src = dict(x=s.x, z=s.z, f0=s.f0, delay=s.delay, amplitude=s.amplitude, kind="ricker",
-               mechanism=s.mechanism)
-    rng = np.random.default_rng(0)
-    rec = {c: rng.standard_normal((sample_nt, NX)) * 1e-9 for c in ("vx", "vz")}
-    cfg = orc.Imaging(mode="pp", condition="normalized", separation="divcurl")
-    t0 = time.perf_counter()
-    orc.migrate("elastic", m, DT, sample_nt, 8, "free", cfg, rec, src, receivers.x,
-                receivers.z)
-    secs = time.perf_counter() - t0
+    secs = _oracle_shot(_oracle_task(model, sources, receivers, len(sources) // 2, sample_nt))
     per_shot = secs * (NT / sample_nt) * 2.0
     return {"value": 1.0 / per_shot, "unit": "shots/s", "cores": 1, "kind": "port",
             "sample": (f"oracle elastic migrate (pp, normalized, divcurl) of one {NZ}x{NX} shot "
                        f"truncated to nt={sample_nt} ({secs:.1f} s), scaled x{NT}/{sample_nt} "
                        f"and x2 for the separate PS call")}
+
+
+def _oracle_task(model, sources, receivers, i, sample_nt):
+    s = sources[i]
+    src = dict(x=s.x, z=s.z, f0=s.f0, delay=s.delay, amplitude=s.amplitude, kind="ricker",
+               mechanism=s.mechanism)
+    return (np.asarray(model.vp), np.asarray(model.vs), np.asarray(model.rho),
+            np.asarray(model.qp), src, np.asarray(receivers.x), np.asarray(receivers.z),
+            sample_nt)
+
+
+def _oracle_shot(task) -> float:
+    """Seconds of one truncated oracle PP migration (also run in pool workers)."""
+    from oracle import rtm as orc
+
+    vp, vs, rho, qp, src, rx, rz, sample_nt = task
+    m = orc.Model(nz=NZ, nx=NX, dx=DX, dz=DX, vp=vp, vs=vs, rho=rho, qp=qp)
+    rng = np.random.default_rng(0)
+    rec = {c: rng.standard_normal((sample_nt, NX)) * 1e-9 for c in ("vx", "vz")}
+    cfg = orc.Imaging(mode="pp", condition="normalized", separation="divcurl")
+    t0 = time.perf_counter()
+    orc.migrate("elastic", m, DT, sample_nt, 8, "free", cfg, rec, src, rx, rz)
+    return time.perf_counter() - t0
+
+
+def cpu_pool_baseline(model, sources, receivers, sample_nt):
+    """The oracle over a process pool: N independent shots (N = the affinity
+    core count, at most 16), each the one-core sample above, run at once.
+    Called before the process touches the GPU (the pool forks)."""
+    import multiprocessing as mp
+
+    affinity = len(os.sched_getaffinity(0))
+    n = max(1, min(16, affinity))
+    tasks = [_oracle_task(model, sources, receivers, (i * len(sources)) // n, sample_nt)
+             for i in range(n)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(n) as pool:
+        secs = pool.map(_oracle_shot, tasks)
+    wall = time.perf_counter() - t0
+    per_batch = wall * (NT / sample_nt) * 2.0
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f
+                              if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": n / per_batch, "unit": "shots/s", "processes": n,
+            "affinity_cores": affinity, "os_cpu_count": os.cpu_count(), "cpu_model": cpu_model,
+            "sample": (f"{n} shots at once, each the one-core sample: wall {wall:.1f} s "
+                       f"(per process {min(secs):.1f}-{max(secs):.1f} s), scaled "
+                       f"x{NT}/{sample_nt} and x2")}
 
 
 def main(argv=None) -> int:
@@ -414,6 +454,15 @@ def main(argv=None) -> int:
 
     import torch
 
+    pool_baseline = None
+    if (not args.no_cpu_baseline and not args.c5_sweep and not args.engine_sweep
+            and torch.cuda.device_count() > 0):
+        # before any HIP call: the pool forks
+        print(f"# cpu baseline (process pool): oracle, nt={args.cpu_sample_nt}",
+              file=sys.stderr, flush=True)
+        model = layered_model()
+        sources, receivers = acquisition(args.shots)
+        pool_baseline = cpu_pool_baseline(model, sources, receivers, args.cpu_sample_nt)
     if not torch.cuda.is_available():
         print("bench.py needs a GPU (HIP device)", file=sys.stderr)
         return 1
@@ -554,6 +603,7 @@ def main(argv=None) -> int:
     if not args.no_cpu_baseline:
         print(f"# cpu baseline: oracle, nt={args.cpu_sample_nt}", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(model, sources, receivers, args.cpu_sample_nt)
+        line["cpu_baseline"]["pool"] = pool_baseline
     print(json.dumps(line), flush=True)
     return 0
 

@@ -147,6 +147,20 @@ def shared_padded_rows(a: np.ndarray, ld: int, key=None):
     return out
 
 
+def pack_host(items) -> tuple[np.ndarray, list[int]]:
+    """((array, dtype) pairs) -> one byte buffer holding each array, cast to
+    its dtype, from a 256-byte-aligned offset; returns (buffer, offsets)."""
+    arrs = [np.ascontiguousarray(np.asarray(a, dtype=dt)).reshape(-1) for a, dt in items]
+    offs, total = [], 0
+    for a in arrs:
+        offs.append(total)
+        total += _rup(max(a.nbytes, 1), 256)
+    host = np.zeros(total, dtype=np.uint8)
+    for a, o in zip(arrs, offs):
+        host[o:o + a.nbytes] = a.view(np.uint8)
+    return host, offs
+
+
 def shared_vector(a: np.ndarray):
     """``to_device`` through the cache of ``shared_padded_rows``: the CPML
     profiles are the same for every propagator on one grid, spacing and time
@@ -400,14 +414,7 @@ class RecvTables:
         """Several small host arrays ((array, dtype) pairs) in one upload:
         pointers into one device buffer, each array on a 256-byte boundary.
         One copy instead of one per array, for the tables of many shots."""
-        arrs = [np.ascontiguousarray(np.asarray(a, dtype=dt)).reshape(-1) for a, dt in items]
-        offs, total = [], 0
-        for a in arrs:
-            offs.append(total)
-            total += _rup(max(a.nbytes, 1), 256)
-        host = np.zeros(total, dtype=np.uint8)
-        for a, o in zip(arrs, offs):
-            host[o:o + a.nbytes] = a.view(np.uint8)
+        host, offs = pack_host(items)
         t = to_device(host)
         self.keep.append(t)
         base = L.ptr(t)

@@ -2,7 +2,7 @@
 and the host post-processing: CPML recursion coefficients against the
 reference's (golden vectors), stencil constants against their closed forms,
 and the image Laplacian and wavenumber low-pass against the oracle, bit for
-bit."""
+bit; and the byte layout of the receiver tables' single upload."""
 
 from __future__ import annotations
 
@@ -64,3 +64,23 @@ def test_cfl_dt_formula():
     c = num.coefficient_l1(8)
     expected = 0.5 / (3000.0 * np.hypot(c / 5.0, c / 4.0))
     assert abs(num.cfl_dt(3000.0, 5.0, 4.0, 8) - expected) < 1e-15
+
+
+def test_packed_table_upload_layout():
+    """The receiver tables go to the device as one buffer: every array starts
+    on a 256-byte boundary, holds its values in its own dtype, and the next
+    array starts after it (empty arrays included)."""
+    from ertm.device import pack_host
+
+    rng = np.random.default_rng(7)
+    items = [(rng.integers(-5, 5, 37), np.int32), (np.arange(0), np.int32),
+             (rng.integers(0, 2 ** 40, 11), np.int64), (rng.standard_normal(65), np.float64),
+             (rng.standard_normal(3), np.float32)]
+    host, offs = pack_host(items)
+    assert host.dtype == np.uint8 and all(o % 256 == 0 for o in offs)
+    sizes = [np.asarray(a, dtype=dt).nbytes for a, dt in items]
+    assert all(offs[k] + sizes[k] <= offs[k + 1] for k in range(len(items) - 1))
+    assert offs[-1] + sizes[-1] <= host.size
+    for (a, dt), o, nb in zip(items, offs, sizes):
+        back = host[o:o + nb].view(dt)
+        assert np.array_equal(back, np.asarray(a, dtype=dt))

@@ -100,6 +100,7 @@ class CheckpointStore:
     bytes_per_state: int = 0
     arena: Any = None
     state_floats: int = 0
+    forward_steps: int = 0  # steps the checkpointing pass ran
 
     @property
     def memory_mb(self) -> float:
@@ -155,13 +156,18 @@ def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
 
 def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarray], *,
                              budget_mb: float, variant=None, graph_steps: int = 0,
-                             during=None) -> list[CheckpointStore]:
+                             during=None, to_last_checkpoint: bool = False
+                             ) -> list[CheckpointStore]:
     """``record_source_wavefield`` for several shots advanced in one launch per step.
 
     All shots must share nt and the checkpoint interval (the interval follows
     from nt and the fields-only state size, so shots on grids of the same
     size always do).  ``during``, if given, is called once the pass has been
     launched and before waiting for it (host set-up to overlap with it).
+    ``to_last_checkpoint`` ends the pass at the step that saves the last
+    checkpoint instead of at nt: a migration restarts every block from its
+    checkpoint, so the steps after the last one produce nothing it reads
+    (the checkpoints are the same either way).
     """
     import torch
 
@@ -195,7 +201,9 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
                                          len(cores), cores[0].nmech))
     ws = [np.asarray(w, dtype=np.float32) for w in wavelets]
     kw = batch_run_kwargs(props)
-    cur = b.run(0, 0, nt, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
+    # the checkpoint of start s holds the state after step s
+    t_end = starts[-1] + 1 if to_last_checkpoint else nt
+    cur = b.run(0, 0, t_end, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
                 save=arenas, save_stride=stride, step_index0=step0.pop(), **kw)
     try:
         if during is not None:
@@ -207,10 +215,10 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     for p, c, arena, ps in zip(props, cores, arenas, per_state):
         c.cur = cur
         p.pml.touched = True
-        p.step_index += nt
+        p.step_index += t_end
         stores.append(CheckpointStore(states=[(s, k) for k, s in enumerate(starts)],
                                       interval=interval, bytes_per_state=ps, arena=arena,
-                                      state_floats=stride))
+                                      state_floats=stride, forward_steps=t_end))
     return stores
 
 
@@ -563,7 +571,8 @@ class RTMImager:
                     tabs.append(tab)
 
         stores = record_source_wavefields(fwds, waves, budget_mb=float(self.config.checkpoint_mb),
-                                          variant=kv, graph_steps=gsteps, during=build_backs)
+                                          variant=kv, graph_steps=gsteps, during=build_backs,
+                                          to_last_checkpoint=True)
         torch.cuda.current_stream().wait_stream(side)
         starts = []
         for shot, spec in zip(shots, specs):
@@ -681,7 +690,8 @@ class RTMImager:
             nz, nx = models[j].shape
             infos.append(dict(store=stores[j], start_step=starts[j], prop=fwds[j], back=backs[j],
                               replayed_steps=replayed, nt=nt,
-                              cell_updates=(nt + replayed + back_steps) * nz * nx))
+                              cell_updates=(stores[j].forward_steps + replayed + back_steps)
+                              * nz * nx))
         timing["cell_updates"] = float(sum(i["cell_updates"] for i in infos))
         timing["shots"] = float(n)
         self.last_timing = timing

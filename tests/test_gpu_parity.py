@@ -120,6 +120,32 @@ def test_replay_blocks_match_reference():
         assert np.array_equal(np.stack([f["P"] for f in fields]), d[f"block{b}"]), b
 
 
+def test_forward_to_last_checkpoint_saves_the_same_checkpoints():
+    """A migration's checkpointing pass may end at the step that saves the
+    last checkpoint: every checkpoint equals the full pass's, bit for bit,
+    and the last block replays to the reference's fields."""
+    _say("forward to the last checkpoint")
+    from ertm.imaging import record_source_wavefields, replay_block
+    from ertm.propagation import source_wavelet
+    from product_cases import source_of
+
+    d = load("replay")
+    c = forward_case("ac_cpml")
+    p_full, spec = propagator_of(c)
+    p_cut, _spec = propagator_of(c)
+    w = source_wavelet(source_of(c), spec.dt, spec.nt)
+    full = record_source_wavefields([p_full], [w], budget_mb=1.0)[0]
+    cut = record_source_wavefields([p_cut], [w], budget_mb=1.0, to_last_checkpoint=True)[0]
+    last = cut.states[-1][0]
+    assert full.forward_steps == spec.nt and cut.forward_steps == last + 1 < spec.nt
+    assert p_cut.step_index == last + 1
+    assert cut.states == full.states and cut.interval == full.interval
+    assert np.array_equal(cut.arena.cpu().numpy(), full.arena.cpu().numpy())
+    b = len(cut.states) - 1
+    fields = replay_block(p_cut, w, cut, b, "stress")
+    assert np.array_equal(np.stack([f["P"] for f in fields]), d[f"block{b}"])
+
+
 @pytest.mark.parametrize("tag", tags("migrate"))
 def test_migrate_matches_reference(tag):
     _say("migrate golden", tag)

@@ -185,3 +185,33 @@ def test_pipelined_batches_equal_run_batch(tag):
                 assert got[m].meta == ref[m].meta
     # the first shot is the golden one
     assert np.array_equal(piped[0][0][c["imaging"]["mode"]].image, c["image"])
+
+
+def test_pipelined_batches_on_a_caller_stream_equal_default_stream():
+    """Batched calls launched on a stream of the caller's keep set-up and
+    host copies on that stream (no side streams); results equal those of
+    the default stream, where side streams overlap them with the passes."""
+    _say("pipelined batches on a caller stream")
+    import torch
+
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager, side_stream
+
+    c = migrate_case("el_ps_divcurl")
+    shot, model, imaging, solver, q = migrate_objects(c)
+    imager = RTMImager(imaging, solver, q, GpuConfig(max_batch=2))
+    job_a = imager.prepare_batch([shot, _moved(shot, 2 * model.grid.dx),
+                                  _moved(shot, -3 * model.grid.dx)], model, modes=("pp", "ps"))
+    job_b = imager.prepare_batch([_moved(shot, model.grid.dx)], model, modes=("pp", "ps"))
+    ref = imager.run_batches([job_a, job_b])
+    assert side_stream("setup") != torch.cuda.current_stream()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        assert side_stream("setup") == s and side_stream("fetch") == s
+        got = imager.run_batches([job_a, job_b])
+    s.synchronize()
+    for got_job, ref_job in zip(got, ref):
+        for g, r in zip(got_job, ref_job):
+            for m in ("pp", "ps"):
+                assert np.array_equal(g[m].image, r[m].image)
+                assert g[m].meta == r[m].meta

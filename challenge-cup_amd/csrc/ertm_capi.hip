@@ -673,8 +673,6 @@ static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int
     return ERTM_OK;
 }
 
-extern "C" {
-
 // hipFFT plans for image post-processing, one per shape, shared by every
 // stream (a plan is bound to a stream for the duration of one call).
 namespace {
@@ -682,16 +680,15 @@ std::mutex g_post_m;
 std::map<std::pair<int, int>, hipfftHandle> g_post_plans;
 }  // namespace
 
-int ertm_postprocess(const ErtmPost* p, void* stream) {
-    if (!p || p->nz <= 0 || p->nx <= 0 || !p->num || !p->work || !p->out)
-        return fail(ERTM_EINVAL, "postprocess: bad arguments");
-    if ((p->div_ill && !p->ill) || (p->div_rill && !p->rill))
-        return fail(ERTM_EINVAL, "postprocess: normalisation array missing");
-    hipStream_t s = (hipStream_t)stream;
+// Post-processing of one image with the scratch at p->work: the normalised
+// image in [0, n) doubles, then the Laplacian (n doubles, or 2n as the
+// complex buffer of the low-pass).  Every scratch double a call reads was
+// written earlier in the same call.
+static int post_run(const ErtmPost* p, hipStream_t s) {
     const int nz = p->nz, nx = p->nx;
     const int64_t n = (int64_t)nz * nx;
-    double* a = p->work;          // n doubles: normalised image
-    double* c = p->work + 2 * n;  // 2n doubles: complex buffer (or the Laplacian)
+    double* a = p->work;      // n doubles: normalised image
+    double* c = p->work + n;  // n doubles (Laplacian) or 2n (complex buffer)
     launch_post_norm(p->num, p->ill, p->rill, n, p->eps, p->div_ill, p->div_rill, a, s);
     LAUNCHCHK("post_norm");
     const double* img = a;
@@ -726,6 +723,72 @@ int ertm_postprocess(const ErtmPost* p, void* stream) {
     launch_post_finish(img, from_complex, nz, nx, p->npml, p->mask_top, p->out, s);
     LAUNCHCHK("post_finish");
     return ERTM_OK;
+}
+
+static int post_check(const ErtmPost* p, const char* who) {
+    if (!p || p->nz <= 0 || p->nx <= 0 || !p->num || !p->out)
+        return fail(ERTM_EINVAL, "%s: bad arguments", who);
+    if ((p->div_ill && !p->ill) || (p->div_rill && !p->rill))
+        return fail(ERTM_EINVAL, "%s: normalisation array missing", who);
+    return ERTM_OK;
+}
+
+// Static device scratch of ertm_postprocess_noalloc: the normalised image and
+// the Laplacian, 2 * ERTM_NOALLOC_MAX_CELLS doubles (32 MiB).  Every GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// the name `scratch` is only a host-side stand-in: kernels get the device
+// address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[2 * (size_t)ERTM_NOALLOC_MAX_CELLS];
+
+namespace {
+std::mutex g_scratch_m;
+std::vector<double*> g_scratch_dev;  // per device id; nullptr until looked up
+}  // namespace
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+static int scratch_of_device(double** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_m);
+    if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_dev[dev]) {
+        void* addr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+        if (!addr) return fail(ERTM_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_dev[dev] = (double*)addr;
+    }
+    *out = g_scratch_dev[dev];
+    return ERTM_OK;
+}
+
+extern "C" {
+
+int ertm_postprocess(const ErtmPost* p, void* stream) {
+    int rc = post_check(p, "postprocess");
+    if (rc != ERTM_OK) return rc;
+    if (!p->work) return fail(ERTM_EINVAL, "postprocess: work is NULL");
+    return post_run(p, (hipStream_t)stream);
+}
+
+int ertm_postprocess_noalloc(const ErtmPost* p, void* stream) {
+    int rc = post_check(p, "postprocess_noalloc");
+    if (rc != ERTM_OK) return rc;
+    const int64_t n = (int64_t)p->nz * p->nx;
+    if (n > (int64_t)ERTM_NOALLOC_MAX_CELLS)
+        return fail(ERTM_EINVAL,
+                    "postprocess_noalloc: %d x %d image (%lld cells) exceeds the static scratch "
+                    "of %d cells; use ertm_postprocess with a work buffer",
+                    p->nz, p->nx, (long long)n, ERTM_NOALLOC_MAX_CELLS);
+    if (p->lowpass)
+        return fail(ERTM_EINVAL,
+                    "postprocess_noalloc: the wavenumber low-pass needs a hipFFT plan, which "
+                    "allocates; use ertm_postprocess");
+    double* w = nullptr;
+    if ((rc = scratch_of_device(&w)) != ERTM_OK) return rc;
+    ErtmPost q = *p;
+    q.work = w;
+    return post_run(&q, (hipStream_t)stream);
 }
 
 int ertm_prepare_record(float* rec, int32_t nt, int32_t nrec, const double* t,

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libertm_hip.so")
 ABI_VERSION = 4
 MAX_LAUNCH = 64
 MAX_MECH = 8
+NOALLOC_MAX_CELLS = 1 << 21  # ERTM_NOALLOC_MAX_CELLS: image cells of ertm_postprocess_noalloc
 
 ACOUSTIC, ELASTIC, VISCO = 0, 1, 2
 SRC_EXPLOSIVE, SRC_SHEAR, SRC_FORCE_Z = 0, 1, 2
@@ -126,6 +127,7 @@ EXPORTS = [
     ("ertm_absmax", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("ertm_sync", C.c_int, [C.c_void_p]),
     ("ertm_postprocess", C.c_int, [C.POINTER(ErtmPost), C.c_void_p]),
+    ("ertm_postprocess_noalloc", C.c_int, [C.POINTER(ErtmPost), C.c_void_p]),
     ("ertm_prepare_record", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                       C.c_double, C.c_int32, C.c_void_p]),
     ("ertm_rotate_components", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_double,

@@ -350,6 +350,21 @@ typedef struct ErtmPost {
 } ErtmPost;
 int ertm_postprocess(const ErtmPost* p, void* stream);
 
+/* Allocation-free variant of ertm_postprocess.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  p->work is ignored (may be NULL);
+ * the scratch is a static device array of the library holding
+ * 2 * ERTM_NOALLOC_MAX_CELLS doubles (32 MiB) on every GPU the library runs
+ * on, for the life of the process.  Limits: nz * nx <= ERTM_NOALLOC_MAX_CELLS
+ * and lowpass = 0 (the wavenumber low-pass needs a hipFFT plan, which
+ * allocates); either violation returns ERTM_EINVAL before any GPU work.
+ * Calls no hipMalloc, hipFree or allocating library routine.  All calls on
+ * one GPU share that GPU's scratch: calls that overlap on one GPU (on
+ * different streams, or from several threads without ordering) race.  Same
+ * kernels and results as ertm_postprocess. */
+#define ERTM_NOALLOC_MAX_CELLS (1 << 21)
+int ertm_postprocess_noalloc(const ErtmPost* p, void* stream);
+
 /* ---- record preparation ------------------------------------------------------
  * On a device record (nt, nrec) float32, row-major: the direct-wave top mute
  * (mute_direct_wave, ertm/preprocess.py:121-169: weight 0.5 - 0.5 cos(pi r),

@@ -56,6 +56,25 @@ def test_abi_version_and_error_path_without_gpu():
     assert lib.ertm_derivative(None, None, 4, 4, 0, 1, 8, 1.0, 0, None) == 1
 
 
+def test_noalloc_postprocess_rejects_oversize_and_lowpass_without_gpu():
+    """The allocation-free post-processing refuses, on the host, what its
+    static scratch cannot hold and the low-pass (whose hipFFT plan allocates)."""
+    from ertm import _lib
+
+    lib = _lib.load_library()
+    p = _lib.ErtmPost()
+    p.num, p.out = 0x1000, 0x2000  # never dereferenced: rejected before any HIP call
+    p.nz, p.nx = 2048, _lib.NOALLOC_MAX_CELLS // 2048 + 1
+    assert lib.ertm_postprocess_noalloc(C.byref(p), None) == 1
+    assert b"exceeds the static scratch" in lib.ertm_last_error()
+    p.nz, p.nx, p.lowpass = 64, 64, 1
+    assert lib.ertm_postprocess_noalloc(C.byref(p), None) == 1
+    assert b"low-pass" in lib.ertm_last_error()
+    p.lowpass, p.num = 0, None
+    assert lib.ertm_postprocess_noalloc(C.byref(p), None) == 1
+    assert b"bad arguments" in lib.ertm_last_error()
+
+
 def test_product_has_no_cpu_fallback():
     """Without a GPU the engine refuses to run instead of computing on the CPU."""
     import pytest

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <mutex>
 #include <vector>
@@ -673,11 +674,12 @@ static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int
     return ERTM_OK;
 }
 
-// hipFFT plans for image post-processing, one per shape, shared by every
-// stream (a plan is bound to a stream for the duration of one call).
+// hipFFT plans for image post-processing, one per (stream, shape).  A plan
+// owns a work area that its transforms use on the GPU, so two streams never
+// share one: their transforms may run at the same time.
 namespace {
 std::mutex g_post_m;
-std::map<std::pair<int, int>, hipfftHandle> g_post_plans;
+std::map<std::tuple<hipStream_t, int, int>, hipfftHandle> g_post_plans;
 }  // namespace
 
 // Post-processing of one image with the scratch at p->work: the normalised
@@ -704,7 +706,7 @@ static int post_run(const ErtmPost* p, hipStream_t s) {
             LAUNCHCHK("post_to_complex");
         }
         std::lock_guard<std::mutex> lk(g_post_m);
-        hipfftHandle& plan = g_post_plans[std::make_pair(nz, nx)];
+        hipfftHandle& plan = g_post_plans[std::make_tuple(s, nz, nx)];
         if (!plan && hipfftPlan2d(&plan, nz, nx, HIPFFT_Z2Z) != HIPFFT_SUCCESS) {
             plan = 0;
             return fail(ERTM_EHIP, "hipfftPlan2d(%d,%d) failed", nz, nx);
@@ -1012,7 +1014,10 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
         if (r->phase != 2) {
             const size_t bytes = sizeof(float) * (size_t)nchunks * S * n;
             if (bytes > b->damp_bytes) {
-                // the amplitude table is baked into the graphs: drop them
+                // the amplitude table is baked into the graphs: drop them,
+                // after the launches already enqueued with this batch that
+                // may still read the old table have finished
+                for (hipStream_t t : b->streams) HIPCHK(hipStreamSynchronize(t));
                 for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
                 b->graphs.clear();
                 pool_put(b->damp, b->damp_bytes);
@@ -1041,6 +1046,7 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             if (b->graphs.size() >= 32) {  // keep the cache small
+                for (hipStream_t t : b->streams) HIPCHK(hipStreamSynchronize(t));
                 for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
                 b->graphs.clear();
             }

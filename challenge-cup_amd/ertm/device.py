@@ -135,15 +135,36 @@ def _digest(a: np.ndarray) -> bytes:
     return d
 
 
+def _cache_put(key, t):
+    """Enter ``t`` in the shared cache with an event recorded on the current
+    stream after the work that filled it (``padded_rows`` fills the tensor
+    asynchronously on this stream)."""
+    torch = torch_mod()
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream())
+    t._ertm_ready = ev
+    _MEDIA[key] = t
+    return t
+
+
+def _cache_get(key):
+    """A cached tensor, with the current stream made to wait until its upload
+    has finished: a propagator built on another stream (a serving worker)
+    must not read a medium that is still being written."""
+    out = _MEDIA.get(key)
+    if out is not None:
+        torch_mod().cuda.current_stream().wait_event(out._ertm_ready)
+    return out
+
+
 def shared_padded_rows(a: np.ndarray, ld: int, key=None):
     """``padded_rows`` through a cache keyed by the array's bytes (or by
     ``key``, a digest of what the array was computed from), shape and the row
     pitch; the cached tensor lives while some propagator holds it."""
     key = (_digest(a) if key is None else key, a.shape, a.dtype.str, int(ld))
-    out = _MEDIA.get(key)
+    out = _cache_get(key)
     if out is None:
-        out = padded_rows(a, ld)
-        _MEDIA[key] = out
+        out = _cache_put(key, padded_rows(a, ld))
     return out
 
 
@@ -166,10 +187,9 @@ def shared_vector(a: np.ndarray):
     profiles are the same for every propagator on one grid, spacing and time
     step, and kernels only read them."""
     key = ("vector", _digest(a), a.shape, a.dtype.str)
-    out = _MEDIA.get(key)
+    out = _cache_get(key)
     if out is None:
-        out = to_device(a)
-        _MEDIA[key] = out
+        out = _cache_put(key, to_device(a))
     return out
 
 

@@ -10,8 +10,10 @@ and ``forward_shot``.
 Underneath, every propagator is a ``device.Core``: its fields, CPML strips
 and medium live in HBM and every update is a HIP kernel of libertm_hip.so.
 Field attributes (``prop.p`` ...) and ``state()`` return host copies.  The
-engine computes in float32; a float64 spec is accepted and computed in
-float32 (reported in the record meta as ``compute_dtype``).
+engine computes in float32.  A float64 spec (``SolverConfig.dtype``, which
+the reference honours with float64 wavefields, ertm/propagation.py:52, :207)
+is refused with ``SolverError`` rather than computed at a narrower precision
+than asked for.
 """
 
 from __future__ import annotations
@@ -164,6 +166,15 @@ class _Profile:
     npml: int
 
 
+def check_compute_dtype(spec) -> None:
+    """The GPU engine's kernels compute in float32 only; a float64 request is
+    an error, not a silent downgrade."""
+    if np.dtype(spec.dtype) != np.float32:
+        raise SolverError(
+            f"dtype={spec.dtype!r}: the GPU engine computes in float32 only; a float64 run "
+            "needs the reference's NumPy engine")
+
+
 class Propagator2D:
     """Common machinery of the GPU propagators (ertm/propagation.py:190-391)."""
 
@@ -172,6 +183,7 @@ class Propagator2D:
     _engine = ""
 
     def __init__(self, model: ElasticModel, spec: PropagationSpec) -> None:
+        check_compute_dtype(spec)
         if not dev.is_trusted(model):  # batches validate their models once
             model.validate()
         self.model = model

@@ -109,6 +109,37 @@ def gaussian(t: np.ndarray, f0: float, t0: float) -> np.ndarray:
     return np.exp(-0.5 * u ** 2).astype(np.float32)
 
 
+def resample_sinc(data: np.ndarray, dt_in: float, dt_out: float, half_length: int = 8,
+                  beta: float = 8.6) -> np.ndarray:
+    """ertm/numerics.py:337-392 along axis 0 (1-D or 2-D): Kaiser-windowed
+    sinc taps j = floor(p) - L + 1 .. floor(p) + L around each output
+    position p = k * dt_out / dt_in, weights normalised to unit sum, indices
+    clamped to the record, float64.  The taps are summed one at a time in
+    order, starting from zero."""
+    x = np.asarray(data, dtype=np.float64)
+    one = x.ndim == 1
+    if one:
+        x = x[:, None]
+    n_in = x.shape[0]
+    if n_in < 3:
+        raise ValueError("too few samples")
+    n_out = int(np.floor((n_in - 1) * dt_in / dt_out + 1e-9)) + 1
+    p = np.arange(n_out, dtype=np.float64) * (dt_out / dt_in)
+    j0 = np.floor(p).astype(np.int64)
+    taps = 2 * half_length
+    window = np.kaiser(taps, beta)
+    j = j0[:, None] + (np.arange(taps) - half_length + 1)[None, :]
+    w = np.sinc(p[:, None] - j) * window[None, :]
+    # NumPy's row sum (pairwise over the taps), as the reference normalises
+    total = w.sum(axis=1)
+    total[total == 0] = 1.0
+    w = w / total[:, None]
+    out = np.zeros((n_out, x.shape[1]))
+    for k in range(taps):
+        out = out + w[:, k, None] * x[np.clip(j[:, k], 0, n_in - 1)]
+    return out[:, 0] if one else out
+
+
 def bilinear(field: np.ndarray, zi: np.ndarray, xi: np.ndarray) -> np.ndarray:
     """ertm/numerics.py:478-511: float64 bilinear gather, 0 where out of range.
 

@@ -177,14 +177,30 @@ def start_step(delay, f0, start_periods, dt, nt):
     return int(np.clip(s, 0, nt))
 
 
+def align_record(rec, rec_dt, dt):
+    """ertm/imaging.py:396-415 without the caller's preprocess: each
+    component in float64, sinc-resampled to ``dt`` when its step differs by
+    more than 1e-12 s (and it has more than one sample)."""
+    out = {}
+    for k, v in rec.items():
+        d = np.asarray(v, np.float64)
+        if abs(float(rec_dt) - float(dt)) > 1e-12 and d.shape[0] > 1:
+            d = onum.resample_sinc(d, float(rec_dt), float(dt))
+        out[k] = d
+    return out
+
+
 def migrate_raw(engine, m: Model, dt, nt, order, boundary, cfg: Imaging, rec, src, rx, rz,
-                w=None, q: QSpec | None = None, q_enabled=False, dtype="float32"):
+                w=None, q: QSpec | None = None, q_enabled=False, dtype="float32", rec_dt=None):
     """ertm/imaging.py:205-331 up to (and including) normalisation.
 
     ``src`` = dict(x, z, f0, delay, amplitude, kind, mechanism); ``rec`` maps
-    component name -> (nt_rec, nrec) array already aligned to (nt, dt).
+    component name -> (nt_rec, nrec) array sampled at ``rec_dt`` (default:
+    ``dt``), aligned here to (nt, dt).
     Returns (image float64 before post-processing, info dict).
     """
+    if rec_dt is not None:
+        rec = align_record(rec, rec_dt, dt)
     npml = adaptive_npml(m.nz, m.nx)
     if w is None:
         w = wavelet(src.get("kind", "ricker"), src["f0"], src["delay"], src["amplitude"], dt, nt)
@@ -288,12 +304,12 @@ def postprocess(img, m: Model, npml, boundary, cfg: Imaging):
 
 
 def migrate(engine, m, dt, nt, order, boundary, cfg, rec, src, rx, rz, w=None, q=None,
-            q_enabled=False, dtype="float32"):
+            q_enabled=False, dtype="float32", rec_dt=None):
     """Full single-shot migration; returns the float32 image (ImageResult.image).
     dtype="float64" runs the propagation in float64 (the reference's
     PropagationSpec.dtype), the accuracy reference for the float32 engine."""
     img, info = migrate_raw(engine, m, dt, nt, order, boundary, cfg, rec, src, rx, rz, w, q,
-                            q_enabled, dtype)
+                            q_enabled, dtype, rec_dt)
     out = postprocess(img, m, info["npml"], boundary, cfg)
     return np.asarray(out, np.float32), info
 

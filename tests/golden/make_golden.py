@@ -19,6 +19,12 @@ What is pinned (reference file:line of the function whose output is stored):
   * sample/inject with duplicate receivers  ertm/propagation.py:331-391, 472-494
   * record_source_wavefield + replay_block  ertm/imaging.py:113-162
   * RTMImager.migrate images                ertm/imaging.py:205-349
+    (one case with a shot sampled at 36 us, aligned by _prepare_record's
+    resample_sinc, ertm/imaging.py:396-415)
+  * gaussian_pulse, source_wavelet           ertm/numerics.py:472-475,
+                                            ertm/propagation.py:1004-1015
+  * resample_sinc                           ertm/numerics.py:337-392
+  * cfl_dt                                  ertm/numerics.py:314-326
 """
 
 from __future__ import annotations
@@ -120,6 +126,39 @@ def main() -> None:
         out[f"cpml{k}_a"] = axis.a.ravel()
         out[f"cpml{k}_b"] = axis.b.ravel()
         out[f"cpml{k}_params"] = np.array([n, h, npml, dt, vmax, f0, sides[0], sides[1]], np.float64)
+    # source wavelets: Gaussian pulse and Ricker through source_wavelet
+    t = np.arange(400, dtype=np.float64) * 2.5e-4
+    for k, (f0, t0) in enumerate([(25.0, 0.048), (40.0, 0.03), (7.5, 0.0)]):
+        out[f"gauss{k}_params"] = np.array([f0, t0], np.float64)
+        out[f"gauss{k}"] = num.gaussian_pulse(t, f0, t0)
+    for k, (kind, f0, delay, amp, dt, nt) in enumerate([
+            ("gaussian", 30.0, 0.04, 2.5, 4e-4, 300), ("gaussian", 50.0, 0.01, 1.0, 2.5e-4, 150),
+            ("ricker", 30.0, 0.04, 2.5, 4e-4, 300)]):
+        srcw = R["SourceSpec"](x=0.0, z=0.0, f0=f0, delay=delay, amplitude=amp, kind=kind)
+        out[f"wavelet{k}_params"] = np.array([f0, delay, amp, dt, nt], np.float64)
+        out[f"wavelet{k}_kind"] = np.array(kind)
+        out[f"wavelet{k}"] = prop_mod.source_wavelet(srcw, dt, nt)
+    # sinc resampling: C3's 36 us records to the 0.35 ms solver step and
+    # back, the reference test's 0.4 ms -> 0.1 ms, a non-divisible pair, and
+    # the three-sample minimum
+    rngS = np.random.default_rng(36)
+    for k, (dt_in, dt_out, n_in, ntr) in enumerate([
+            (3.6e-5, 3.5e-4, 1500, 5), (3.5e-4, 3.6e-5, 120, 3), (4.0e-4, 1.0e-4, 500, 1),
+            (2.5e-4, 3.3e-4, 257, 4), (1.0e-3, 4.0e-4, 3, 2)]):
+        tr = np.cumsum(rngS.standard_normal((n_in, ntr)), axis=0).astype(np.float32)
+        out[f"sinc{k}_params"] = np.array([dt_in, dt_out], np.float64)
+        out[f"sinc{k}_in"] = tr
+        out[f"sinc{k}_out"] = num.resample_sinc(tr, dt_in, dt_out)
+        out[f"sinc{k}_in1"] = tr[:, 0].astype(np.float64)
+        out[f"sinc{k}_out1"] = num.resample_sinc(tr[:, 0].astype(np.float64), dt_in, dt_out)
+    # CFL limits at the workloads' media and steps (and a few orders)
+    cfl = []
+    for vmax, dx, dz, order, safety in [(3500.0, 5.0, 5.0, 8, 0.55), (4500.0, 5.0, 5.0, 8, 0.55),
+                                        (5000.0, 4.0, 4.0, 8, 0.55), (5500.0, 2.5, 2.5, 8, 0.55),
+                                        (4000.0, 10.0, 10.0, 8, 0.55), (3000.0, 5.0, 7.0, 4, 0.5),
+                                        (2500.0, 10.0, 10.0, 2, 1.0), (4500.0, 5.0, 5.0, 6, 1.0)]:
+        cfl.append([vmax, dx, dz, order, safety, num.cfl_dt(vmax, dx, dz, order, safety=safety)])
+    out["cfl_table"] = np.array(cfl, np.float64)
     np.savez_compressed(HERE / "numerics.npz", **out)
 
     # ------------------------------------------------------------ GSLS fit
@@ -173,6 +212,7 @@ def main() -> None:
             [source.x, source.z, source.f0, source.amplitude, source.delay], np.float64
         )
         fwd[f"{tag}/mechanism"] = np.array(source.mechanism)
+        fwd[f"{tag}/kind"] = np.array(source.kind)
         fwd[f"{tag}/rx"], fwd[f"{tag}/rz"] = receivers.x, receivers.z
         fwd[f"{tag}/components"] = np.array(list(components))
         if q_options is not None:
@@ -224,6 +264,14 @@ def main() -> None:
     spC3 = PropagationSpec(dt=2e-4, nt=90, order=6, npml=6, boundary="free")
     srcC3 = SourceSpec(x=151.0, z=35.0, f0=40.0, delay=0.02, amplitude=1.0, mechanism="force_z")
     run_case("el_force", "elastic", mC, spC3, srcC3, recA, ("vx", "vz"))
+
+    # C4: elastic, free, force_z source, 8th order (the fused kernels' order)
+    spC4 = PropagationSpec(dt=2e-4, nt=120, order=8, npml=6, boundary="free")
+    run_case("el_force8", "elastic", mC, spC4, srcC3, recA, ("vx", "vz", "sxz"))
+
+    # B5: acoustic, cpml, random medium, Gaussian source pulse
+    srcG = SourceSpec(x=89.5, z=97.0, f0=45.0, delay=0.02, amplitude=2.5, kind="gaussian")
+    run_case("ac_gauss", "acoustic", mB, spB, srcG, recB, ("p", "vx"))
 
     # D: viscoacoustic, forward (no compensation), layered Q
     arrs = layered(31, 41, 10.0, [0.0, 150.0], [2400.0, 3000.0], qps=[30.0, 80.0])
@@ -293,8 +341,11 @@ def main() -> None:
     mig = {}
 
     def migrate_case(tag, engine, model, solver_cfg, imaging_cfg, source, receivers, comps,
-                     q_cfg=None):
+                     q_cfg=None, fwd_dt=None):
         spec_fwd = PropagationSpec.from_config(solver_cfg, model, npml=solver_cfg.npml)
+        if fwd_dt is not None:  # shot sampled at another step than the solver's
+            duration = (solver_cfg.nt - 1) * solver_cfg.dt
+            spec_fwd = replace(spec_fwd, dt=fwd_dt, nt=int(np.floor(duration / fwd_dt)) + 2)
         qo = None
         if engine == "viscoacoustic":
             qo = QOptions(enabled=True, f_ref=q_cfg.f_ref, n_mechanisms=q_cfg.n_mechanisms,
@@ -327,6 +378,8 @@ def main() -> None:
         mig[f"{tag}/source"] = np.array(
             [source.x, source.z, source.f0, source.amplitude, source.delay], np.float64)
         mig[f"{tag}/mechanism"] = np.array(source.mechanism)
+        mig[f"{tag}/kind"] = np.array(source.kind)
+        mig[f"{tag}/shot_dt"] = np.array(float(shot.dt))
         mig[f"{tag}/rx"], mig[f"{tag}/rz"] = receivers.x, receivers.z
         mig[f"{tag}/components"] = np.array(list(comps))
         if q_cfg is not None:
@@ -348,6 +401,11 @@ def main() -> None:
                  ImagingConfig(condition="normalized", checkpoint_mb=1.0, laplacian=False,
                                lowpass_fraction=1.0, receiver_normalization=True),
                  srcM, recM, ("p",))
+    # the shot sampled at 36 us: _prepare_record aligns it to the solver's
+    # 0.25 ms with resample_sinc before the backward pass
+    migrate_case("ac_resample", "acoustic", mM, solA,
+                 ImagingConfig(condition="zero_lag", checkpoint_mb=1.0), srcM, recM, ("p",),
+                 fwd_dt=3.6e-5)
     solAc = replace(solA, boundary="cpml")
     migrate_case("ac_cpml_zl", "acoustic", mM, solAc,
                  ImagingConfig(condition="zero_lag", checkpoint_mb=1.0, laplacian=False,

@@ -27,6 +27,12 @@ def tags(name: str):
     return sorted({k.split("/")[0] for k in load(name) if "/" in k})
 
 
+def _kind(d, tag: str) -> str:
+    """Source wavelet kind of a case (cases written before kinds were stored
+    are Ricker)."""
+    return str(d[f"{tag}/kind"]) if f"{tag}/kind" in d else "ricker"
+
+
 def forward_case(tag: str) -> dict:
     d = load("forward")
     g = d[f"{tag}/grid"]
@@ -41,7 +47,7 @@ def forward_case(tag: str) -> dict:
         vp=d[f"{tag}/vp"], vs=d[f"{tag}/vs"], rho=d[f"{tag}/rho"], qp=d[f"{tag}/qp"],
         qs=d[f"{tag}/qs"],
         src=dict(x=float(src[0]), z=float(src[1]), f0=float(src[2]), amplitude=float(src[3]),
-                 delay=float(src[4]), kind="ricker", mechanism=str(d[f"{tag}/mechanism"])),
+                 delay=float(src[4]), kind=_kind(d, tag), mechanism=str(d[f"{tag}/mechanism"])),
         rx=d[f"{tag}/rx"], rz=d[f"{tag}/rz"],
         components=tuple(str(c) for c in d[f"{tag}/components"]),
         rec={str(c): d[f"{tag}/rec_{c}"] for c in d[f"{tag}/components"]},
@@ -82,9 +88,10 @@ def migrate_case(tag: str) -> dict:
                      lowpass_fraction=float(im[6]), checkpoint_mb=float(im[7]), eps=float(im[8]),
                      start_periods=float(im[9])),
         src=dict(x=float(src[0]), z=float(src[1]), f0=float(src[2]), amplitude=float(src[3]),
-                 delay=float(src[4]), kind="ricker", mechanism=str(d[f"{tag}/mechanism"])),
+                 delay=float(src[4]), kind=_kind(d, tag), mechanism=str(d[f"{tag}/mechanism"])),
         rx=d[f"{tag}/rx"], rz=d[f"{tag}/rz"],
         components=comps,
+        shot_dt=float(d[f"{tag}/shot_dt"]) if f"{tag}/shot_dt" in d else float(so[1]),
         shot={c: d[f"{tag}/shot_{c}"] for c in comps},
         shot_data=d[f"{tag}/shot_data"],
         image=d[f"{tag}/image"],

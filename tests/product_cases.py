@@ -18,7 +18,7 @@ def source_of(c):
 
     s = c["src"]
     return SourceSpec(x=s["x"], z=s["z"], f0=s["f0"], amplitude=s["amplitude"], delay=s["delay"],
-                      mechanism=s["mechanism"])
+                      mechanism=s["mechanism"], kind=s.get("kind", "ricker"))
 
 
 def receivers_of(c):
@@ -63,7 +63,8 @@ def migrate_objects(c):
                     amplitude_cap=qq["amplitude_cap"], lowpass_interval=qq["lowpass_interval"],
                     f_max=qq["f_max"], gain_budget_db=qq["gain_budget_db"])
     nt = c["shot_data"].shape[0]
-    shot = ShotRecord(data=c["shot_data"], dt=c["dt"], t=np.arange(nt) * c["dt"],
+    sdt = c.get("shot_dt", c["dt"])
+    shot = ShotRecord(data=c["shot_data"], dt=sdt, t=np.arange(nt) * sdt,
                       receivers=receivers_of(c), source=source_of(c),
                       components={k: v for k, v in c["shot"].items()})
     return shot, model, imaging, solver, q

@@ -260,7 +260,7 @@ def test_viscoacoustic_measures_target_q():
     from ertm.propagation import QOptions
 
     model = homogeneous_q(40.0, nx=141, nz=121)
-    spec = PropagationSpec(dt=4e-4, nt=900, order=8, npml=12, dtype="float64", boundary="cpml")
+    spec = PropagationSpec(dt=4e-4, nt=900, order=8, npml=12, boundary="cpml")
     source = SourceSpec(x=600.0, z=600.0, f0=25.0, delay=0.03)
     receivers = ReceiverSpec(x=np.array([800.0, 1000.0]), z=np.array([600.0, 600.0]))
     data = visco_traces(model, spec, source, receivers, QOptions(enabled=True, f_ref=25.0))
@@ -284,7 +284,7 @@ def test_round_trip_gain_matches_theory_and_is_frequency_dependent():
     twice the 10 Hz gain (tests/test_rtm.py:492-539)."""
     from ertm.propagation import QOptions
 
-    spec = PropagationSpec(dt=4e-4, nt=800, order=8, npml=12, dtype="float64", boundary="cpml")
+    spec = PropagationSpec(dt=4e-4, nt=800, order=8, npml=12, boundary="cpml")
     source = SourceSpec(x=500.0, z=500.0, f0=25.0, delay=0.03)
     receivers = ReceiverSpec(x=np.array([900.0]), z=np.array([500.0]))
 

@@ -49,7 +49,7 @@ def _oracle_forward64(c):
     p.set_source(c["src"]["x"], c["src"]["z"], c["src"].get("mechanism", "explosive"))
     p.set_receivers(c["rx"], c["rz"])
     s = c["src"]
-    w = ortm.wavelet("ricker", s["f0"], s["delay"], s["amplitude"], c["dt"], c["nt"])
+    w = ortm.wavelet(s["kind"], s["f0"], s["delay"], s["amplitude"], c["dt"], c["nt"])
     return ortm.forward(p, w, c["components"])
 
 

@@ -298,8 +298,8 @@ def test_acoustic_c1_shape_forward_matches_oracle():
 
 @pytest.mark.parametrize("variant,zc", [(1, 64), (1, 10), (2, 19), (3, 10), (3, 64), (2, 4096),
                                         (2, 8), (1, 9), (3, 13), (2, 20), (4, 48), (4, 13)])
-@pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force", "ac_cpml", "ac_free",
-                                 "vi_fwd"])
+@pytest.mark.parametrize("tag", ["el_free", "el_shear", "el_force8", "ac_cpml", "ac_free",
+                                 "vi_fwd", "ac_gauss"])
 def test_fused_elastic_variants_bit_identical(tag, variant, zc):
     """Every fused launch shape (elastic, acoustic, viscoacoustic) reproduces
     the two-launch kernels (and thus the reference) bit for bit, across chunk
@@ -310,8 +310,7 @@ def test_fused_elastic_variants_bit_identical(tag, variant, zc):
     from product_cases import source_of
 
     c = forward_case(tag)
-    if c["order"] != 8:
-        pytest.skip("fused kernels are order 8")
+    assert c["order"] == 8  # the fused kernels' order
     p, spec = propagator_of(c)
     p._batches["step"] = dev.Batch([p._core], variant=(variant, zc))
     w = source_wavelet(source_of(c), spec.dt, spec.nt)
@@ -398,6 +397,35 @@ def test_device_postprocess_without_lowpass_is_bit_exact():
     host = RTMImager(imaging, solver, q).migrate(shot, model)
     devi = RTMImager(imaging, solver, q, GpuConfig(device_postprocess=True)).migrate(shot, model)
     assert np.array_equal(host.image, devi.image)
+
+
+def test_device_postprocess_on_two_streams_at_once_equals_serial():
+    """Two streams post-process different images of one shape at the same
+    time (low-pass on, so both run hipFFT transforms): each result equals the
+    serial one bit for bit.  Each stream gets its own plan, so the transforms
+    never share a plan's work area."""
+    _say("device postprocess, two streams")
+    import torch
+
+    from ertm import device as dev
+
+    nz, nx = 600, 1200
+    rng = np.random.default_rng(4)
+    imgs = [torch.from_numpy(rng.standard_normal((nz, nx))).cuda() for _ in range(4)]
+    ills = [torch.from_numpy(rng.uniform(0.5, 2.0, (nz, nx))).cuda() for _ in range(4)]
+    kw = dict(dx=5.0, dz=5.0, eps=1e-10, div_ill=True, div_rill=False, laplacian=True,
+              lowpass=True, fraction=0.85, npml=20, mask_top=False)
+    serial = [dev.postprocess(a, b, None, **kw) for a, b in zip(imgs, ills)]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [None] * 4
+    for rep in range(3):
+        for k in range(4):
+            with torch.cuda.stream(streams[k % 2]):
+                outs[k] = dev.postprocess(imgs[k], ills[k], None, **kw)
+        torch.cuda.synchronize()
+        for k in range(4):
+            assert torch.equal(outs[k], serial[k]), (rep, k)
 
 
 @pytest.mark.parametrize("kv", [(0, 64), (2, 19), (1, 64)])

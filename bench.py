@@ -100,7 +100,7 @@ def model_records(model, sources, receivers, nt):
     return solver.forward_batch(model, sources, receivers, components=("vx", "vz"))
 
 
-def serving_request(base_seed, i, scale):
+def serving_request(base_seed, i, scale, engine=None, nz=None, nx=None, nt_max=None):
     """One serving request (the mixed acoustic/elastic request stream), grid
     sizes and nt scaled by ``scale``: engine acoustic or elastic (p = 0.5),
     nz U{256..2000}, nx U{256..4000}, dx U[2.5, 10], nt U{1500..8000}; 2-8
@@ -108,18 +108,25 @@ def serving_request(base_seed, i, scale):
     elliptical anomalies (radii U[20, 200] m, vp and vs x0.6, density x0.85);
     f0 = clip(vmin / (12.5 dx), 5, 40); dt from the CFL rule with safety 0.5;
     a surface source at a random x; receivers every 2 cells at z = 2 dz;
-    zero_lag imaging; a seeded random record."""
+    zero_lag imaging; a seeded random record.  ``engine``, ``nz`` and ``nx``
+    replace the drawn values and ``nt_max`` caps nt (the draws are made all
+    the same, so the rest of the request does not change)."""
     from ertm import numerics as num
     from ertm.config import ImagingConfig, SolverConfig
     from ertm.serving import ShotRequest
     from ertm.types import ElasticModel, GridSpec, ReceiverSpec, ShotRecord, SourceSpec
 
     rng = np.random.default_rng([base_seed, i])
-    engine = "acoustic" if rng.random() < 0.5 else "elastic"
-    nz = max(64, int(rng.integers(256, 2001) * scale))
-    nx = max(64, int(rng.integers(256, 4001) * scale))
+    drawn = "acoustic" if rng.random() < 0.5 else "elastic"
+    engine = engine or drawn
+    drawn = max(64, int(rng.integers(256, 2001) * scale))
+    nz = nz or drawn
+    drawn = max(64, int(rng.integers(256, 4001) * scale))
+    nx = nx or drawn
     dx = float(rng.uniform(2.5, 10.0))
     nt = max(300, int(rng.integers(1500, 8001) * scale))
+    if nt_max is not None:
+        nt = min(nt, int(nt_max))
     z = (np.arange(nz) * dx)[:, None]
     x = (np.arange(nx) * dx)[None, :]
     nl = int(rng.integers(2, 9))

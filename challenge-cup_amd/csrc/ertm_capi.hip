@@ -331,19 +331,20 @@ int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc) {
 // ---------------------------------------------------------------------------
 // launch helpers (chunks of ERTM_MAX_LAUNCH propagators)
 // ---------------------------------------------------------------------------
-// Step-kernel timing: while enabled, every step launch is bracketed by a pair
-// of HIP events recorded on the launch stream; reading sums their elapsed
-// times.  Event pairs are pooled and reused across reads.
+// Launch timing: while enabled, every launch made outside a captured loop is
+// bracketed by a pair of HIP events recorded on its stream and tagged with
+// its kind (ERTM_TK_*); reading sums the elapsed times per kind.  Event
+// pairs are pooled and reused across reads.
 namespace {
-struct StepTimer {
+struct LaunchTimer {
     std::mutex m;
     bool on = false;
-    std::vector<hipEvent_t> ev;  // 2 per recorded launch
-    size_t used = 0;             // events in use
-    int64_t cells = 0;           // physical cell-updates of the recorded launches
-    int32_t fused = -1;          // 1 fused kernel, 0 two-launch, 2 both seen
+    std::vector<hipEvent_t> ev;     // 2 per recorded launch
+    std::vector<int8_t> kind;       // 1 per recorded launch
+    size_t used = 0;                // events in use
+    int64_t cells[ERTM_TK_COUNT] = {};  // physical cells covered, per kind
 };
-StepTimer g_timer;
+LaunchTimer g_timer;
 
 int timer_event(hipEvent_t* out) {
     if (g_timer.used == g_timer.ev.size()) {
@@ -354,6 +355,45 @@ int timer_event(hipEvent_t* out) {
     *out = g_timer.ev[g_timer.used++];
     return ERTM_OK;
 }
+
+// One timed launch: the constructor takes the timer's lock and records the
+// start event if timing is on (and the launch is not being captured); done()
+// records the end event under the given kind.
+class Timed {
+  public:
+    Timed(hipStream_t s, bool captured) : s_(s), lk_(g_timer.m, std::defer_lock) {
+        if (captured) return;
+        lk_.lock();
+        if (!g_timer.on) {
+            lk_.unlock();
+            return;
+        }
+        rc_ = timer_event(&e0_);
+        if (rc_ == ERTM_OK) {
+            const hipError_t e = hipEventRecord(e0_, s_);
+            if (e != hipSuccess) rc_ = fail(ERTM_EHIP, "hipEventRecord: %s", hipGetErrorString(e));
+        }
+        if (rc_ != ERTM_OK) e0_ = nullptr;
+    }
+    int status() const { return rc_; }
+    int done(int kind, int64_t cells) {
+        if (!e0_) return rc_;
+        hipEvent_t e1 = nullptr;
+        int rc = timer_event(&e1);
+        if (rc != ERTM_OK) return rc;
+        HIPCHK(hipEventRecord(e1, s_));
+        g_timer.kind.push_back((int8_t)kind);
+        g_timer.cells[kind] += cells;
+        e0_ = nullptr;
+        return ERTM_OK;
+    }
+
+  private:
+    hipStream_t s_;
+    std::unique_lock<std::mutex> lk_;
+    hipEvent_t e0_ = nullptr;
+    int rc_ = ERTM_OK;
+};
 }  // namespace
 
 // Can the fused elastic kernel do imaging op `op` inside the step?  Needs a
@@ -463,6 +503,15 @@ static void use_clock(LaunchArgs& la, const DevClock* dc, int qoff = 0) {
     la.qoff = dc->q + qoff;
 }
 
+// Physical cells of propagators [base, base + n) of a batch.
+namespace {
+int64_t cells_of(ertm_batch_t b, int base, int n) {
+    int64_t c = 0;
+    for (int k = base; k < base + n; ++k) c += (int64_t)b->props[k].nz * b->props[k].nx;
+    return c;
+}
+}  // namespace
+
 // Copy state[cur] of every propagator to (to_arena) or from its arena at
 // arenas[k] + slot * stride.  Aligned batches go through one k_copy_batch
 // launch per ERTM_MAX_LAUNCH propagators; otherwise one async copy each.
@@ -495,8 +544,11 @@ static int copy_states(ertm_batch_t b, int cur, float* const* arenas, int64_t sl
             a.dst[k] = (float4*)(to_arena ? ar : P.state[cur]);
             a.n4[k] = P.state_floats / 4;
         }
+        Timed tm(s, false);
+        if (tm.status() != ERTM_OK) return tm.status();
         launch_copy_batch(a, s);
         LAUNCHCHK("copy_batch");
+        if (int rc = tm.done(ERTM_TK_COPY, cells_of(b, base, a.n))) return rc;
     }
     return ERTM_OK;
 }
@@ -511,10 +563,26 @@ static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t 
         la.op = count;
         for (int k = 0; k < la.n; ++k) la.t[k] = t_hi[base + k];
         use_clock(la, dc, -(count - 1));  // the group's first (latest) step
+        Timed tm(s, dc != nullptr);
+        if (tm.status() != ERTM_OK) return tm.status();
         launch_image_flush(b->dprops, b->dimg, la, b->max_img, s);
         LAUNCHCHK("image_flush");
+        // cells: the physical cells of the flushed steps
+        if (int rc = tm.done(ERTM_TK_FLUSH, count * cells_of(b, base, la.n))) return rc;
     }
     return ERTM_OK;
+}
+
+// Timing kind of a step launch: the fused kernel's imaging variant, or the
+// two-launch pair.
+static int step_kind(bool fused, int kind) {
+    if (!fused) return ERTM_TK_TWO_LAUNCH;
+    switch (kind) {
+        case 0: return ERTM_TK_PLAIN;
+        case 1: case 4: return ERTM_TK_EXTRACT;
+        case 3: case 5: return ERTM_TK_RING;
+        default: return ERTM_TK_CORRELATE;
+    }
 }
 
 // One step; with img_op >= 0 and *imaged set on return, the step launches
@@ -523,14 +591,9 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
                    int img_op = -1, const int32_t* t = nullptr, bool* imaged = nullptr,
                    const DevClock* dc = nullptr) {
     const int n = (int)b->props.size();
-    std::unique_lock<std::mutex> lk(g_timer.m, std::defer_lock);
     const bool fuse_img = img_op >= 0 && t &&
                           (img_op == ERTM_IMG_RING_STORE ? ring_ok(b) : fused_image_ok(b, img_op));
     if (imaged) *imaged = fuse_img;
-    // only plain step launches are timed (the 52 B/cell algorithmic figure);
-    // nothing is timed inside a captured loop
-    const bool timed = g_timer.on && !fuse_img && !dc;
-    if (timed) lk.lock();
     for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
         la.base = base;
@@ -540,28 +603,18 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
         if (fuse_img)
             for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
         use_clock(la, dc);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timed) {
-            int rc = timer_event(&e0);
-            if (rc == ERTM_OK) rc = timer_event(&e1);
-            if (rc != ERTM_OK) return rc;
-            HIPCHK(hipEventRecord(e0, s));
-        }
+        const int kind = fuse_img ? img_kind(b, img_op, base, la.n) : 0;
+        Timed tm(s, dc != nullptr);
+        if (tm.status() != ERTM_OK) return tm.status();
         bool fused = b->variant != 0 &&
                      launch_step_fused(b->dprops, b->props.data(), la, b->order, b->variant, b->zc,
-                                       b->dimg, fuse_img ? img_kind(b, img_op, base, la.n) : 0, s);
+                                       b->dimg, kind, s);
         if (!fused) {
             if (fuse_img) return fail(ERTM_ESTATE, "step: fused imaging launch refused");
             launch_step(b->dprops, la, b->order, b->max_tiles, b->any_force, s);
         }
         LAUNCHCHK("step");
-        if (timed) {
-            HIPCHK(hipEventRecord(e1, s));
-            for (int k = 0; k < la.n; ++k)
-                g_timer.cells += (int64_t)b->props[base + k].nz * b->props[base + k].nx;
-            const int32_t f = fused ? 1 : 0;
-            g_timer.fused = g_timer.fused < 0 ? f : (g_timer.fused == f ? f : 2);
-        }
+        if (int rc = tm.done(step_kind(fused, kind), cells_of(b, base, la.n))) return rc;
     }
     return ERTM_OK;
 }
@@ -576,8 +629,11 @@ static int do_sample(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s,
         la.cur = cur;
         for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
         use_clock(la, dc);
+        Timed tm(s, dc != nullptr);
+        if (tm.status() != ERTM_OK) return tm.status();
         launch_sample(b->dprops, b->drecv, la, b->max_samp, s);
         LAUNCHCHK("sample");
+        if (int rc = tm.done(ERTM_TK_SAMPLE, cells_of(b, base, la.n))) return rc;
     }
     return ERTM_OK;
 }
@@ -592,8 +648,11 @@ static int do_inject(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s,
         la.cur = cur;
         for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
         use_clock(la, dc);
+        Timed tm(s, dc != nullptr);
+        if (tm.status() != ERTM_OK) return tm.status();
         launch_inject(b->dprops, b->drecv, la, b->max_cells, s);
         LAUNCHCHK("inject");
+        if (int rc = tm.done(ERTM_TK_INJECT, cells_of(b, base, la.n))) return rc;
     }
     return ERTM_OK;
 }
@@ -609,8 +668,11 @@ static int do_image(ertm_batch_t b, int cur, const int32_t* t, int op, hipStream
         la.op = op;
         for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
         use_clock(la, dc);
+        Timed tm(s, dc != nullptr);
+        if (tm.status() != ERTM_OK) return tm.status();
         launch_image(b->dprops, b->dimg, la, b->order, b->max_img, s);
         LAUNCHCHK("image");
+        if (int rc = tm.done(ERTM_TK_IMAGE, cells_of(b, base, la.n))) return rc;
     }
     return ERTM_OK;
 }
@@ -814,10 +876,28 @@ int ertm_step_timing(int32_t enable) {
     std::lock_guard<std::mutex> lk(g_timer.m);
     if (enable) {
         g_timer.used = 0;
-        g_timer.cells = 0;
-        g_timer.fused = -1;
+        g_timer.kind.clear();
+        for (int64_t& c : g_timer.cells) c = 0;
     }
     g_timer.on = enable != 0;
+    return ERTM_OK;
+}
+
+int ertm_timing_read(double* ms, int64_t* launches, int64_t* cells) {
+    if (!ms || !launches || !cells) return fail(ERTM_EINVAL, "timing_read: bad arguments");
+    std::lock_guard<std::mutex> lk(g_timer.m);
+    for (int k = 0; k < ERTM_TK_COUNT; ++k) {
+        ms[k] = 0.0;
+        launches[k] = 0;
+        cells[k] = g_timer.cells[k];
+    }
+    for (size_t i = 0; i < g_timer.kind.size(); ++i) {
+        HIPCHK(hipEventSynchronize(g_timer.ev[2 * i + 1]));
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, g_timer.ev[2 * i], g_timer.ev[2 * i + 1]));
+        ms[g_timer.kind[i]] += t;
+        launches[g_timer.kind[i]] += 1;
+    }
     return ERTM_OK;
 }
 
@@ -825,18 +905,14 @@ int ertm_step_timing_read(double* total_ms, int64_t* launches, int64_t* cell_upd
                           int32_t* fused) {
     if (!total_ms || !launches || !cell_updates || !fused)
         return fail(ERTM_EINVAL, "step_timing_read: bad arguments");
-    std::lock_guard<std::mutex> lk(g_timer.m);
-    double sum = 0.0;
-    for (size_t i = 0; i + 1 < g_timer.used; i += 2) {
-        HIPCHK(hipEventSynchronize(g_timer.ev[i + 1]));
-        float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, g_timer.ev[i], g_timer.ev[i + 1]));
-        sum += ms;
-    }
-    *total_ms = sum;
-    *launches = (int64_t)(g_timer.used / 2);
-    *cell_updates = g_timer.cells;
-    *fused = g_timer.fused;
+    double ms[ERTM_TK_COUNT];
+    int64_t n[ERTM_TK_COUNT], c[ERTM_TK_COUNT];
+    if (int rc = ertm_timing_read(ms, n, c)) return rc;
+    *total_ms = ms[ERTM_TK_PLAIN] + ms[ERTM_TK_TWO_LAUNCH];
+    *launches = n[ERTM_TK_PLAIN] + n[ERTM_TK_TWO_LAUNCH];
+    *cell_updates = c[ERTM_TK_PLAIN] + c[ERTM_TK_TWO_LAUNCH];
+    *fused = n[ERTM_TK_PLAIN] > 0 ? (n[ERTM_TK_TWO_LAUNCH] > 0 ? 2 : 1)
+                                  : (n[ERTM_TK_TWO_LAUNCH] > 0 ? 0 : -1);
     return ERTM_OK;
 }
 

@@ -135,7 +135,13 @@ EXPORTS = [
     ("ertm_step_timing", C.c_int, [C.c_int32]),
     ("ertm_step_timing_read", C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    ("ertm_timing_read", C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64)]),
 ]
+
+# launch-timing kinds (ERTM_TK_*), in enum order
+TIMING_KINDS = ("two_launch", "plain", "extract", "ring", "correlate", "flush", "inject", "copy",
+                "sample", "image")
 
 _lock = threading.Lock()
 _lib = None

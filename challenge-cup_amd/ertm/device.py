@@ -758,6 +758,8 @@ class StepTimer:
 
     @staticmethod
     def read() -> dict:
+        """The step launches without imaging: summed ms, launches, cell-updates,
+        fused (1) / two-launch (0) / both (2) / none (-1)."""
         ms = C.c_double(0.0)
         n = C.c_int64(0)
         cells = C.c_int64(0)
@@ -766,4 +768,15 @@ class StepTimer:
                                               C.byref(fused)), "ertm_step_timing_read")
         return {"total_ms": float(ms.value), "launches": int(n.value),
                 "cell_updates": int(cells.value), "fused": int(fused.value)}
+
+    @staticmethod
+    def read_kinds() -> dict:
+        """Every timed launch kind: {kind: {"ms", "launches", "cells"}}."""
+        k = len(L.TIMING_KINDS)
+        ms = (C.c_double * k)()
+        n = (C.c_int64 * k)()
+        cells = (C.c_int64 * k)()
+        L.check(L.lib().ertm_timing_read(ms, n, cells), "ertm_timing_read")
+        return {name: {"ms": float(ms[i]), "launches": int(n[i]), "cells": int(cells[i])}
+                for i, name in enumerate(L.TIMING_KINDS)}
 

@@ -380,14 +380,32 @@ int ertm_rotate_components(float* vx, float* vz, int64_t n, double cos_a, double
                            void* stream);
 
 /* ---- measurement (no counterpart in the reference) ------------------------ */
-/* Step-kernel timing.  While enabled, each step launch of ertm_step / ertm_run
- * (one launch per group of up to ERTM_MAX_LAUNCH propagators) is bracketed by
- * two HIP events recorded on the launch stream.  Enabling clears earlier
- * records.  ertm_step_timing_read waits for the recorded events and returns
- * the summed kernel time (ms), the number of launches, the physical
- * cell-updates (nz*nx per propagator) they covered and whether they were the
- * fused kernel (1), the two-launch pair (0), both (2) or none (-1). */
+/* Launch timing.  While enabled, every launch that ertm_step / ertm_run /
+ * ertm_sample / ertm_inject / ertm_image make outside a captured time loop
+ * (one launch per group of up to ERTM_MAX_LAUNCH propagators) is bracketed
+ * by two HIP events recorded on its stream and tagged with its kind.
+ * Enabling clears earlier records. */
+enum {
+    ERTM_TK_TWO_LAUNCH = 0,  /* k_velocity + k_stress (one pair = one launch) */
+    ERTM_TK_PLAIN = 1,       /* fused step, no imaging */
+    ERTM_TK_EXTRACT = 2,     /* fused step + source-field extraction (replay) */
+    ERTM_TK_RING = 3,        /* fused step + receiver-field ring store (backward) */
+    ERTM_TK_CORRELATE = 4,   /* fused step + correlation */
+    ERTM_TK_FLUSH = 5,       /* ring flush into the accumulators */
+    ERTM_TK_INJECT = 6,      /* receiver injection */
+    ERTM_TK_COPY = 7,        /* checkpoint save / restore (k_copy_batch) */
+    ERTM_TK_SAMPLE = 8,      /* receiver sampling */
+    ERTM_TK_IMAGE = 9,       /* separate imaging pass */
+    ERTM_TK_COUNT = 10
+};
 int ertm_step_timing(int32_t enable);
+/* Waits for the recorded events; per kind k (arrays of ERTM_TK_COUNT): summed
+ * milliseconds, launches and physical cells covered (nz*nx per propagator;
+ * for a flush, times the steps it folds). */
+int ertm_timing_read(double* ms, int64_t* launches, int64_t* cells);
+/* The step launches without imaging (ERTM_TK_PLAIN + ERTM_TK_TWO_LAUNCH):
+ * summed ms, launches, cell-updates, and whether they were the fused kernel
+ * (1), the two-launch pair (0), both (2) or none (-1). */
 int ertm_step_timing_read(double* total_ms, int64_t* launches, int64_t* cell_updates,
                           int32_t* fused);
 

@@ -105,6 +105,13 @@ struct ertm_batch_s {
     bool all_acoustic = false;
     int variant = 0;
     int zc = 64;
+    // two-step launches (ertm_batch_set_two_step): launch shape, 0 = off; and
+    // the scratch for the CPML memories after the first step, one CPML part
+    // of a state buffer per propagator
+    int v2 = 0, zc2 = 0;
+    void* psi1 = nullptr;
+    size_t psi1_bytes = 0;
+    std::vector<float*> psi1_ptr;
     float* scratch = nullptr;  // per propagator: peak, scale
     void* block = nullptr;     // pool block holding dprops, drecv, dimg, scratch
     size_t block_bytes = 0;
@@ -131,6 +138,7 @@ struct ertm_batch_s {
         pool_put(damp, damp_bytes);
         pool_put(dark, sizeof(float*) * props.size());
         pool_put(block, block_bytes);
+        pool_put(psi1, psi1_bytes);
         for (auto& kv : fft) {
             if (kv.second.plan) hipfftDestroy(kv.second.plan);
             if (kv.second.buf) (void)hipFree(kv.second.buf);
@@ -323,6 +331,36 @@ int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc) {
     if (zc < 8 || zc > 4096) return fail(ERTM_EINVAL, "z chunk %d out of [8, 4096]", zc);
     b->variant = variant;
     b->zc = zc;
+    return ERTM_OK;
+}
+
+int ertm_batch_set_two_step(ertm_batch_t b, int32_t variant, int32_t zc) {
+    if (!b) return fail(ERTM_EINVAL, "set_two_step: NULL batch");
+    if (variant == 0) {
+        b->v2 = 0;
+        return ERTM_OK;
+    }
+    if (variant != 2) return fail(ERTM_EINVAL, "two-step launch shape %d: only 2 is built", variant);
+    if (!(b->all_elastic && b->order == 8))
+        return fail(ERTM_EINVAL, "two-step launches need an elastic batch at order 8");
+    if (zc < 8 || zc > 4096) return fail(ERTM_EINVAL, "z chunk %d out of [8, 4096]", zc);
+    if (!b->psi1) {
+        // one CPML part per propagator, each from a 256-byte boundary
+        size_t total = 0;
+        std::vector<size_t> off;
+        for (const ErtmProp& P : b->props) {
+            off.push_back(total);
+            const int64_t floats = P.state_floats - (int64_t)P.nfields * P.fstride;
+            total += ((size_t)std::max<int64_t>(floats, 1) * 4 + 255) / 256 * 256;
+        }
+        b->psi1 = pool_get(total);
+        if (!b->psi1) return fail(ERTM_ENOMEM, "set_two_step: CPML scratch allocation failed");
+        b->psi1_bytes = total;
+        b->psi1_ptr.clear();
+        for (size_t o : off) b->psi1_ptr.push_back((float*)((char*)b->psi1 + o));
+    }
+    b->v2 = variant;
+    b->zc2 = zc;
     return ERTM_OK;
 }
 
@@ -615,6 +653,36 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
         }
         LAUNCHCHK("step");
         if (int rc = tm.done(step_kind(fused, kind), cells_of(b, base, la.n))) return rc;
+    }
+    return ERTM_OK;
+}
+
+// Two steps t, t + 1 (amplitudes a1, a2): state[cur] -> state[cur ^ 1]; with
+// img_op = ERTM_IMG_EXTRACT the replay slots of both steps are written too.
+static int do_step2(ertm_batch_t b, int cur, const float* a1, const float* a2, hipStream_t s,
+                    int img_op, const int32_t* t) {
+    const int n = (int)b->props.size();
+    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+        LaunchArgs la{};
+        El2Args e2{};
+        la.base = base;
+        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
+        la.cur = cur;
+        for (int k = 0; k < la.n; ++k) {
+            la.amp[k] = a1[base + k];
+            e2.amp2[k] = a2[base + k];
+            e2.psi1[k] = b->psi1_ptr[base + k];
+            la.t[k] = t[base + k];
+        }
+        const int kind = img_op == ERTM_IMG_EXTRACT ? img_kind(b, img_op, base, la.n) : 0;
+        Timed tm(s, false);
+        if (tm.status() != ERTM_OK) return tm.status();
+        if (!launch_step2_fused(b->dprops, b->props.data(), la, e2, b->v2, b->zc2, b->dimg, kind,
+                                s))
+            return fail(ERTM_ESTATE, "two-step launch refused");
+        LAUNCHCHK("step2");
+        if (int rc = tm.done(kind ? ERTM_TK_EXTRACT2 : ERTM_TK_PLAIN2, 2 * cells_of(b, base, la.n)))
+            return rc;
     }
     return ERTM_OK;
 }
@@ -1164,8 +1232,39 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
         q = nchunks * S;
         step_index += q;  // S is even: the parity is back where it started
     }
-    for (; q < nsteps; ++q)
-        if ((rc = one_step(q, nullptr, s)) != ERTM_OK) return rc;
+    // Two steps per launch where the pass allows it: forward or replay
+    // phase, no sampling or stabiliser, imaging none or the fused
+    // extraction; the pair never ends between a step and its checkpoint.
+    const bool pairs = b->v2 != 0 && r->phase != 2 && !r->do_sample && !r->stabilize &&
+                       (r->op_after < 0 ||
+                        (r->op_after == ERTM_IMG_EXTRACT && fused_image_ok(b, ERTM_IMG_EXTRACT)));
+    auto ck_at = [&](int t) {
+        return r->ck_interval > 0 && t >= r->ck_origin && (t - r->ck_origin) % r->ck_interval == 0;
+    };
+    std::vector<float> amp2(n, 0.0f);
+    while (q < nsteps) {
+        const int t = t0 + dir * q;
+        if (!pairs || q + 1 >= nsteps || ck_at(t)) {
+            if ((rc = one_step(q, nullptr, s)) != ERTM_OK) return rc;
+            ++q;
+            continue;
+        }
+        for (int k = 0; k < n; ++k) {
+            tt[k] = t;
+            amp[k] = r->wavelets[k][t];
+            amp2[k] = r->wavelets[k][t + 1];
+        }
+        if ((rc = do_step2(b, cur, amp.data(), amp2.data(), s, r->op_after, tt.data())) != ERTM_OK)
+            return rc;
+        cur ^= 1;
+        step_index += 2;
+        if (ck_at(t + 1)) {
+            const int64_t slot = (t + 1 - r->ck_origin) / r->ck_interval;
+            if ((rc = copy_states(b, cur, r->save_arenas, slot, r->save_stride, true, s)) != ERTM_OK)
+                return rc;
+        }
+        q += 2;
+    }
     if (cur_out) *cur_out = cur;
     return ERTM_OK;
 }

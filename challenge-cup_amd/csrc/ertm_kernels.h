@@ -50,6 +50,19 @@ void launch_step(const ErtmProp* dprops, const LaunchArgs& la, int order, int ma
 bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
                        int order, int variant, int zc, const ErtmImage* dimg, int img,
                        hipStream_t s);
+// Extra per-launch arguments of the two-step elastic kernel (ertm_fused2.hip):
+// the second step's source amplitudes and, per propagator, a scratch for the
+// CPML memories after the first step (the CPML part of a state buffer).
+struct El2Args {
+    float amp2[ERTM_MAX_LAUNCH];
+    float* psi1[ERTM_MAX_LAUNCH];
+};
+// Two steps state[cur] -> state[cur ^ 1] (time t -> t + 2, la.t = t, la.amp
+// = first step's amplitudes); img = ERTM_FIMG_NONE or an extraction kind
+// (slots t - t0 and t + 1 - t0).  Returns false if it cannot launch.
+bool launch_step2_fused(const ErtmProp* dprops, const ErtmProp* hprops, const LaunchArgs& la,
+                        const El2Args& a2, int variant, int zc, const ErtmImage* dimg, int img,
+                        hipStream_t s);
 void launch_sample(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,
                    int max_samp, hipStream_t s);
 void launch_inject(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchArgs& la,

@@ -114,6 +114,7 @@ EXPORTS = [
     ("ertm_batch_update_image", C.c_int, [C.c_void_p, C.POINTER(ErtmImage), C.c_void_p]),
     ("ertm_batch_destroy", C.c_int, [C.c_void_p]),
     ("ertm_batch_set_variant", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("ertm_batch_set_two_step", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     ("ertm_step", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.c_void_p]),
     ("ertm_stabilize", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                  C.c_void_p, C.c_void_p]),
@@ -141,7 +142,7 @@ EXPORTS = [
 
 # launch-timing kinds (ERTM_TK_*), in enum order
 TIMING_KINDS = ("two_launch", "plain", "extract", "ring", "correlate", "flush", "inject", "copy",
-                "sample", "image")
+                "sample", "image", "plain2", "extract2")
 
 _lock = threading.Lock()
 _lib = None

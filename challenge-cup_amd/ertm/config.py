@@ -181,6 +181,12 @@ class GpuConfig:
                    For one host thread driving the GPU: a capture can be
                    invalidated by work other host threads issue meanwhile,
                    which the serving loop's workers do.
+    two_step       (2, zc) runs the forward and replay passes of elastic
+                   migrations at order 8 two steps per launch (128-lane
+                   blocks, zc rows per chunk; ertm_batch_set_two_step);
+                   (0, 0) = one step per launch; None = the tuner's choice
+                   (one step: two per launch measured slower on MI355X).
+                   Same results.
     """
 
     strict_cfl: bool = True
@@ -192,6 +198,7 @@ class GpuConfig:
     image_ring: int = 32
     device_postprocess: bool = False
     graph_steps: int = 0
+    two_step: tuple | None = None
 
     def validate(self) -> None:
         if self.device_mem_gb <= 0:

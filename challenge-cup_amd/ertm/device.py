@@ -582,7 +582,8 @@ class Batch:
     """A set of propagators advanced together (one ertm_batch_t)."""
 
     def __init__(self, cores: list, recvs: list | None = None, imgs: list | None = None,
-                 variant: tuple[int, int] | None = None, graph_steps: int = 0):
+                 variant: tuple[int, int] | None = None, graph_steps: int = 0,
+                 two_step: tuple[int, int] | None = (0, 0)):
         lib = L.lib()
         n = len(cores)
         self.cores = list(cores)
@@ -610,6 +611,19 @@ class Batch:
         elif int(variant[0]) != 0 and not fusable:
             variant = (0, 64)  # no single-launch kernel for this engine/order
         self.set_variant(*variant)
+        self.two_step = (0, 0)
+        if two_step is None:
+            two_step = tuning.two_step_lookup(c0.engine, c0.order, max(c.nzp for c in cores),
+                                              max(c.nxp for c in cores), n)
+        if int(two_step[0]) != 0 and same and c0.engine == "elastic" and c0.order == 8:
+            self.set_two_step(*two_step)
+
+    def set_two_step(self, variant: int, zc: int) -> None:
+        """Two time steps per launch in the forward and replay passes of
+        ``run`` (ertm_batch_set_two_step); (0, 0) = off."""
+        L.check(L.lib().ertm_batch_set_two_step(self.h, int(variant), int(zc)),
+                "ertm_batch_set_two_step")
+        self.two_step = (int(variant), int(zc))
 
     def set_variant(self, variant: int, zc: int = 64) -> None:
         L.check(L.lib().ertm_batch_set_variant(self.h, int(variant), int(zc)),

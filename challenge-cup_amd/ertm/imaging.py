@@ -156,8 +156,8 @@ def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
 
 def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarray], *,
                              budget_mb: float, variant=None, graph_steps: int = 0,
-                             during=None, to_last_checkpoint: bool = False
-                             ) -> list[CheckpointStore]:
+                             during=None, to_last_checkpoint: bool = False,
+                             two_step=(0, 0)) -> list[CheckpointStore]:
     """``record_source_wavefield`` for several shots advanced in one launch per step.
 
     All shots must share nt and the checkpoint interval (the interval follows
@@ -190,7 +190,7 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     curs = {c.cur for c in cores}
     if len(step0) != 1 or len(curs) != 1:
         raise ImagingError("batched propagators must be at the same step")
-    b = dev.Batch(cores, variant=variant, graph_steps=graph_steps)
+    b = dev.Batch(cores, variant=variant, graph_steps=graph_steps, two_step=two_step)
     if variant is None:
         from . import tuning
 
@@ -549,6 +549,7 @@ class RTMImager:
             fwds.append(fwd)
         kv = self.gpu_config.kernel_variant
         gsteps = int(self.gpu_config.graph_steps)
+        ts = self.gpu_config.two_step
         side = side_stream("setup")
 
         def build_backs():
@@ -572,7 +573,7 @@ class RTMImager:
 
         stores = record_source_wavefields(fwds, waves, budget_mb=float(self.config.checkpoint_mb),
                                           variant=kv, graph_steps=gsteps, during=build_backs,
-                                          to_last_checkpoint=True)
+                                          to_last_checkpoint=True, two_step=ts)
         torch.cuda.current_stream().wait_stream(side)
         starts = []
         for shot, spec in zip(shots, specs):
@@ -628,7 +629,7 @@ class RTMImager:
                    for m, st, rb, bc in zip(models, starts, rbufs, bcores)]
             fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, 0,
                                       rbuf=rb, nslots=slots) for m, rb in zip(models, rbufs)]
-            fb = dev.Batch(fcores, imgs=fimgs, variant=kv, graph_steps=gsteps)
+            fb = dev.Batch(fcores, imgs=fimgs, variant=kv, graph_steps=gsteps, two_step=ts)
             bb = dev.Batch(bcores, recvs=tabs, imgs=ibs, variant=kv, graph_steps=gsteps)
             subs = None
             if slots < store0.interval:

@@ -61,6 +61,19 @@ def heuristic(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1,
     return (0, 64)
 
 
+def two_step_lookup(engine: str, order: int, nzp: int, nxp: int, nbatch: int = 1
+                    ) -> tuple[int, int]:
+    """Launch shape of the two-steps-per-launch elastic kernel for a batch
+    (forward and replay passes), (0, 0) = one step per launch."""
+    if engine != "elastic" or int(order) != 8:
+        return (0, 0)
+    return TWO_STEP_DEFAULT
+
+
+# (variant, zc) of the two-step kernel when the batch allows it
+TWO_STEP_DEFAULT = (0, 0)
+
+
 def key(engine, order, nzp, nxp, nbatch) -> str:
     return f"{engine}:{order}:{nzp}x{nxp}:b{nbatch}"
 

@@ -237,6 +237,17 @@ int ertm_batch_destroy(ertm_batch_t b);
  * chunks of zc rows (8 <= zc <= 4096), alternate chunks in opposite
  * directions.  Results are bit-identical across variants. */
 int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc);
+/* Two time steps per launch in ertm_run's forward and replay passes
+ * (elastic batches at order 8): variant 2 = 128 threads per block streaming
+ * chunks of zc rows; 0 = off (the default: on MI355X it measured slower than
+ * one step per launch, 756 against 619 us per step on 64 shots of 600 x
+ * 1200, being VALU-bound at 3 waves per SIMD).  The state after
+ * the first step of a pair is not stored; a pair never spans a checkpoint
+ * save, receiver sampling or the stabiliser, which keep one step per launch.
+ * The library allocates a scratch for the CPML memories between the two
+ * steps (the CPML part of a state buffer per propagator) on first use.
+ * Results are bit-identical to one step per launch. */
+int ertm_batch_set_two_step(ertm_batch_t b, int32_t variant, int32_t zc);
 
 /*
  * Per-launch time table: propagator i of the batch is at time index t[i] and
@@ -396,7 +407,9 @@ enum {
     ERTM_TK_COPY = 7,        /* checkpoint save / restore (k_copy_batch) */
     ERTM_TK_SAMPLE = 8,      /* receiver sampling */
     ERTM_TK_IMAGE = 9,       /* separate imaging pass */
-    ERTM_TK_COUNT = 10
+    ERTM_TK_PLAIN2 = 10,     /* two-step launch, no imaging (cells: 2 per cell) */
+    ERTM_TK_EXTRACT2 = 11,   /* two-step launch + extraction of both steps */
+    ERTM_TK_COUNT = 12
 };
 int ertm_step_timing(int32_t enable);
 /* Waits for the recorded events; per kind k (arrays of ERTM_TK_COUNT): summed

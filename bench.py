@@ -227,7 +227,8 @@ def c5_model(seed=5):
                         qs=f32(np.full((nz, nx), 50.0)))
 
 
-def c5_sweep(budgets_mb, nt):
+def c5_sweep(budgets_mb, nt, kernel_variant=None, conditions=("zero_lag", "excitation"),
+             segment_check=True, two_step=None):
     """Excitation vs zero_lag imaging of the large shot at several checkpoint
     budgets: one JSON line per run with the checkpoint interval, replayed
     steps / nt (recompute overhead), time and RTM cell-update rate; plus a
@@ -244,13 +245,13 @@ def c5_sweep(budgets_mb, nt):
     src = SourceSpec(x=7500.0, z=5.0, f0=40.0, delay=0.03, mechanism="explosive")
     recv = ReceiverSpec(x=np.arange(nx) * 2.5, z=np.full(nx, 5.0))
     solver = SolverConfig(nt=nt, dt=dt, order=8, boundary="free", engine="elastic", npml=20)
-    gpu = GpuConfig(strict_cfl=False)
+    gpu = GpuConfig(strict_cfl=False, kernel_variant=kernel_variant, two_step=two_step)
     t0 = time.perf_counter()
     shot = build_solver(solver, gpu_config=gpu).forward(model, src, recv, components=("vx", "vz"))
     print(f"# c5 record modelled in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     out = []
     images = {}
-    for cond in ("zero_lag", "excitation"):
+    for cond in conditions:
         for mb in budgets_mb:
             imaging = ImagingConfig(mode="pp", condition=cond, checkpoint_mb=float(mb))
             imager = RTMImager(imaging, solver, gpu_config=gpu)
@@ -261,6 +262,8 @@ def c5_sweep(budgets_mb, nt):
             secs = time.perf_counter() - t0
             meta = res.meta
             line = {"workload": "elastic 2000x6000 single shot", "condition": cond,
+                    "kernel_variant": list(kernel_variant) if kernel_variant else "tuner",
+                    "two_step": list(two_step) if two_step else "tuner",
                     "checkpoint_mb": mb, "interval": meta["checkpoints"]["interval"],
                     "n_checkpoints": meta["checkpoints"]["n_states"],
                     "arena_gib": round(meta["checkpoints"]["device_arena_mb"] / 1024, 2),
@@ -271,6 +274,8 @@ def c5_sweep(budgets_mb, nt):
             images[(cond, mb)] = res.image
             print(json.dumps(line), flush=True)
             out.append(line)
+    if not segment_check or "zero_lag" not in conditions:
+        return out
     # segment replay (sub-checkpoints, small replay slots) == whole-block replay
     mb = budgets_mb[0]
     imaging = ImagingConfig(mode="pp", condition="zero_lag", checkpoint_mb=float(mb))
@@ -487,6 +492,10 @@ def main(argv=None) -> int:
                          "(2000 x 6000, nt 10000), excitation vs zero_lag at checkpoint "
                          "budgets 1/4/16/64 GB, one JSON line per run")
     ap.add_argument("--c5-nt", type=int, default=10000)
+    ap.add_argument("--c5-budgets", default="1024,4096,16384,65536",
+                    help="--c5-sweep: checkpoint budgets (MB)")
+    ap.add_argument("--c5-conditions", default="zero_lag,excitation",
+                    help="--c5-sweep: imaging conditions")
     ap.add_argument("--engine-sweep", action="store_true",
                     help="instead of the RTM bench: plain step throughput of each engine's "
                          "single-launch kernel on the 64-shot grid")
@@ -501,6 +510,9 @@ def main(argv=None) -> int:
                     help="GpuConfig.image_ring for the migrations (default: the config's)")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="GpuConfig.graph_steps for the migrations (default: the config's)")
+    ap.add_argument("--two-step", default=None,
+                    help="GpuConfig.two_step 'variant,zc' for the migrations (default: the "
+                         "tuner's choice; '0,0' = one step per launch)")
     ap.add_argument("--kernel-variant", default=None,
                     help="force the step-kernel launch shape 'variant,zc' (default: the tuner's "
                          "choice for the shape)")
@@ -531,7 +543,12 @@ def main(argv=None) -> int:
     from ertm.imaging import RTMImager
 
     if args.c5_sweep:
-        c5_sweep([1024, 4096, 16384, 65536], args.c5_nt)
+        kv = tuple(int(v) for v in args.kernel_variant.split(",")) if args.kernel_variant else None
+        budgets = [int(v) for v in args.c5_budgets.split(",")]
+        conds = tuple(args.c5_conditions.split(","))
+        ts = tuple(int(v) for v in args.two_step.split(",")) if args.two_step else None
+        c5_sweep(budgets, args.c5_nt, kv, conds, segment_check=not args.kernel_variant,
+                 two_step=ts)
         return 0
     if args.engine_sweep:
         engine_sweep(args.shots, args.sweep_steps)
@@ -559,6 +576,8 @@ def main(argv=None) -> int:
     solver = SolverConfig(nt=args.nt, dt=DT, order=8, boundary="free", engine="elastic", npml=30)
     kv = tuple(int(v) for v in args.kernel_variant.split(",")) if args.kernel_variant else None
     gkw = {} if args.graph_steps is None else {"graph_steps": args.graph_steps}
+    if args.two_step:
+        gkw["two_step"] = tuple(int(v) for v in args.two_step.split(","))
     if args.image_ring is not None:
         gkw["image_ring"] = args.image_ring
     imager = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False,

@@ -8,7 +8,7 @@ the reference; tests only read the fixtures this script wrote.
 Every case is tiny (at most 51 x 47 padded cells, at most 300 steps) so the
 fixtures stay a few hundred KB.  Run:
 
-    python tests/golden/make_golden.py [--reference /root/reference]
+    python tests/golden/make_golden.py [--reference /root/reference] [--store-only]
 
 What is pinned (reference file:line of the function whose output is stored):
   * fit_gsls tau/delta/fit_error            ertm/viscoelastic.py:132-205
@@ -25,12 +25,17 @@ What is pinned (reference file:line of the function whose output is stored):
                                             ertm/propagation.py:1004-1015
   * resample_sinc                           ertm/numerics.py:337-392
   * cfl_dt                                  ertm/numerics.py:314-326
+  * NumpyDataStore model/shot/image files   ertm/data.py:84-282
+    (written by the reference's own store into store/; the raw inputs go
+    beside them in store/inputs.npz, written with plain numpy)
 """
 
 from __future__ import annotations
 
 import argparse
+import shutil
 import sys
+import tempfile
 from dataclasses import replace
 from pathlib import Path
 
@@ -45,9 +50,13 @@ def _import_reference(root: str):
     import ertm.numerics as num  # noqa: F401
     from ertm import imaging, numerics, propagation, viscoelastic
     from ertm.config import ImagingConfig, QConfig, SolverConfig
-    from ertm.types import ElasticModel, GridSpec, ReceiverSpec, SourceSpec
+    from ertm.data import NumpyDataStore
+    from ertm.types import ElasticModel, GridSpec, ImageResult, ReceiverSpec, ShotRecord, SourceSpec
 
     return dict(
+        NumpyDataStore=NumpyDataStore,
+        ImageResult=ImageResult,
+        ShotRecord=ShotRecord,
         imaging=imaging,
         numerics=numerics,
         propagation=propagation,
@@ -92,8 +101,13 @@ def random_model(rng, nz, nx):
 def main() -> None:
     parser = argparse.ArgumentParser()
     parser.add_argument("--reference", default="/root/reference")
+    parser.add_argument("--store-only", action="store_true",
+                        help="write only the NumpyDataStore files under store/")
     args = parser.parse_args()
     R = _import_reference(args.reference)
+    if args.store_only:
+        write_store_files(R)
+        return
     num = R["numerics"]
     prop_mod = R["propagation"]
     img_mod = R["imaging"]
@@ -440,7 +454,48 @@ def main() -> None:
                  ImagingConfig(condition="zero_lag", checkpoint_mb=1.0, laplacian=False,
                                lowpass_fraction=1.0), srcM, recM, ("p", "vx", "vz"), qV0)
     np.savez_compressed(HERE / "migrate.npz", **mig)
+    write_store_files(R)
     print("golden vectors written to", HERE)
+
+
+def write_store_files(R) -> None:
+    """A model, a three-component shot and an image saved by the reference's
+    NumpyDataStore (ertm/data.py:84-114, 149-184, 231-253), copied to
+    store/{model,shot,image}.npz; the inputs they were made from go to
+    store/inputs.npz so the loader test compares against data the reference
+    did not write."""
+    rng = np.random.default_rng(2024)
+    GridSpec = R["GridSpec"]
+    grid = GridSpec(nx=9, nz=7, dx=5.0, dz=4.0, x0=10.0, z0=2.0)
+    vp = rng.uniform(2000, 4000, (7, 9)).astype(np.float32)
+    vs = (vp / 1.8).astype(np.float32)
+    rho = rng.uniform(1800, 2600, (7, 9)).astype(np.float32)
+    qp = rng.uniform(30, 200, (7, 9)).astype(np.float32)
+    qs = (qp * 0.5).astype(np.float32)
+    model = R["ElasticModel"](grid=grid, vp=vp, vs=vs, rho=rho, qp=qp, qs=qs, name="layers",
+                              meta={"builder": "golden", "seed": 2024})
+    nt, nrec, dt = 12, 5, 2.5e-4
+    t = np.arange(nt) * dt
+    rx = np.linspace(10.0, 50.0, nrec)
+    rz = np.full(nrec, 6.0)
+    comps = {c: rng.standard_normal((nt, nrec)).astype(np.float32) for c in ("p", "vx", "vz")}
+    rec = R["ReceiverSpec"](x=rx, z=rz, component="vz")
+    src = R["SourceSpec"](x=30.0, z=8.0, f0=25.0, kind="ricker", amplitude=2.0, delay=0.04,
+                          mechanism="force_z")
+    shot = R["ShotRecord"](data=comps["vz"], dt=dt, t=t, receivers=rec, source=src,
+                           meta={"shot_index": 3}, components=comps)
+    image = rng.standard_normal((7, 9)).astype(np.float32)
+    img = R["ImageResult"](image=image, grid=grid, mode="ps", meta={"condition": "normalized"})
+    out = HERE / "store"
+    out.mkdir(exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        store = R["NumpyDataStore"](tmp)
+        for kind, path in (("model", store.save_model(model, "m")),
+                           ("shot", store.save_shot(shot, "s")),
+                           ("image", store.save_image(img, "i"))):
+            shutil.copyfile(path, out / f"{kind}.npz")
+    np.savez_compressed(out / "inputs.npz", vp=vp, vs=vs, rho=rho, qp=qp, qs=qs, t=t, rx=rx,
+                        rz=rz, image=image, **{f"comp_{c}": a for c, a in comps.items()})
 
 
 if __name__ == "__main__":

@@ -10,6 +10,8 @@ from __future__ import annotations
 import json
 from pathlib import Path
 
+STORE = Path(__file__).resolve().parent / "golden" / "store"
+
 import numpy as np
 import pytest
 
@@ -186,3 +188,51 @@ def test_npz_key_layout_matches_the_reference_files(tmp_path):
     with pytest.raises(DataError):
         store.load_model(tmp_path / "img.npz")
     assert ensure_output_dirs(tmp_path / "x")["reports"] == tmp_path / "x" / "reports"
+
+
+def _npz(path):
+    with np.load(path, allow_pickle=False) as d:
+        return {k: d[k] for k in d.files}
+
+
+def test_files_written_by_the_reference_store_load(tmp_path):
+    """model/shot/image .npz files saved by the reference's NumpyDataStore
+    (ertm/data.py:84-114, 149-184, 231-253) load into the same objects: the
+    arrays equal the inputs they were made from (stored separately with plain
+    numpy), and grid, names, source, receivers and meta are those written."""
+    store = NumpyDataStore(tmp_path)
+    raw = _npz(STORE / "inputs.npz")
+    m = store.load_model(STORE / "model.npz")
+    assert m.grid == GridSpec(nx=9, nz=7, dx=5.0, dz=4.0, x0=10.0, z0=2.0)
+    assert m.name == "layers" and m.meta == {"builder": "golden", "seed": 2024}
+    for f in ("vp", "vs", "rho", "qp", "qs"):
+        assert getattr(m, f).dtype == np.float32 and np.array_equal(getattr(m, f), raw[f]), f
+    s = store.load_shot(STORE / "shot.npz")
+    assert list(s.components) == ["p", "vx", "vz"]
+    for c in ("p", "vx", "vz"):
+        assert np.array_equal(s.components[c], raw[f"comp_{c}"]), c
+    assert np.array_equal(s.data, raw["comp_vz"]) and s.dt == 2.5e-4
+    assert np.array_equal(s.t, raw["t"])
+    assert np.array_equal(s.receivers.x, raw["rx"]) and np.array_equal(s.receivers.z, raw["rz"])
+    assert s.receivers.component == "vz" and s.meta == {"shot_index": 3}
+    assert s.source == SourceSpec(x=30.0, z=8.0, f0=25.0, kind="ricker", amplitude=2.0,
+                                  delay=0.04, mechanism="force_z")
+    im = store.load_image(STORE / "image.npz")
+    assert im.mode == "ps" and im.meta == {"condition": "normalized"} and im.grid == m.grid
+    assert np.array_equal(im.image, raw["image"])
+
+
+@pytest.mark.parametrize("kind", ["model", "shot", "image"])
+def test_store_rewrites_reference_files_with_the_same_keys_and_meta(tmp_path, kind):
+    """Loading a reference-written file and saving it again with this store
+    gives the same keys, the same arrays (dtype included) and the same JSON
+    meta as the reference's file."""
+    store = NumpyDataStore(tmp_path)
+    obj = getattr(store, f"load_{kind}")(STORE / f"{kind}.npz")
+    ref, mine = _npz(STORE / f"{kind}.npz"), _npz(getattr(store, f"save_{kind}")(obj, "x"))
+    assert set(mine) == set(ref)
+    for k in ref:
+        if k == "meta":
+            assert json.loads(str(mine[k].item())) == json.loads(str(ref[k].item()))
+        else:
+            assert mine[k].dtype == ref[k].dtype and np.array_equal(mine[k], ref[k]), k

@@ -190,7 +190,64 @@ def serving_leg(n, scale, load, gcells_per_s, seed=2024):
             "p50_s": round(stats.get("p50_s", float("nan")), 3),
             "p99_s": round(stats.get("p99_s", float("nan")), 3),
             "mean_group": round(stats.get("mean_group", 0.0), 3),
+            "lanes_per_launch": round(stats.get("lanes_per_launch", 0.0), 3),
             "errors": sorted({r.error for r in res if not r.ok})[:3]}
+
+
+def _serve(reqs, arrivals, lanes, t0=None):
+    """Submit ``reqs`` at ``arrivals`` (seconds after the start; None = all at
+    once) to one ServingLoop; returns its stats plus the makespan."""
+    from ertm.config import GpuConfig
+    from ertm.serving import ServingLoop
+
+    with ServingLoop(hbm_budget_gb=200.0, streams=4, gpu_config=GpuConfig(replay_budget_gb=4.0),
+                     lanes=lanes) as loop:
+        t0 = time.perf_counter()
+        futs = []
+        for k, req in enumerate(reqs):
+            due = t0 + (0.0 if arrivals is None else float(arrivals[k]))
+            time.sleep(max(0.0, due - time.perf_counter()))
+            futs.append(loop.submit(req, arrival=due))
+        res = [f.result() for f in futs]
+        stats = loop.stats()
+    makespan = max(r.finish for r in res) - t0
+    stats["makespan_s"] = makespan
+    stats["errors"] = sorted({r.error for r in res if not r.ok})[:3]
+    return stats
+
+
+def serving_sweep(n, scale, loads, lanes=True, seed=2024):
+    """The mixed request stream's capacity and latency under load: first a
+    closed loop (all ``n`` requests submitted at once; capacity = n / the
+    makespan), then open-loop Poisson arrivals at each ``load`` x that
+    capacity, ``n`` requests per point.  One JSON line per point."""
+    reqs = [serving_request(seed, i, scale)[0] for i in range(n)]
+    mean_cells = float(np.mean([3 * r.solver.nt * r.model.shape[0] * r.model.shape[1]
+                                for r in reqs]))
+    mode = "lanes" if lanes else "groups"
+
+    def line(kind, load, rate, st):
+        out = {"metric": "serving_sweep", "mode": mode, "kind": kind, "load": load,
+               "requests": n, "scale": scale, "offered_per_s": None if rate is None else
+               round(rate, 4), "mean_cell_updates": mean_cells,
+               "completed": st["completed"], "failed": st["failed"],
+               "served_per_s": round(st.get("shots_per_s", 0.0), 4),
+               "gcell_updates_per_s": round(st.get("shots_per_s", 0.0) * mean_cells / 1e9, 2),
+               "p50_s": round(st.get("p50_s", float("nan")), 3),
+               "p99_s": round(st.get("p99_s", float("nan")), 3),
+               "mean_group": round(st.get("mean_group", 0.0), 3),
+               "lanes_per_launch": round(st.get("lanes_per_launch", 0.0), 3),
+               "makespan_s": round(st["makespan_s"], 2),
+               "peak_reserved_gib": st.get("peak_reserved_gib"), "errors": st["errors"]}
+        print(json.dumps(out), flush=True)
+
+    st = _serve(reqs, None, lanes)
+    capacity = st["completed"] / st["makespan_s"]
+    line("closed_loop", None, None, st)
+    for load in loads:
+        rate = load * capacity
+        gaps = np.random.default_rng([seed, int(load * 1000)]).exponential(1.0 / rate, n)
+        line("open_loop", load, rate, _serve(reqs, np.cumsum(gaps), lanes))
 
 
 def c5_model(seed=5):
@@ -487,6 +544,13 @@ def main(argv=None) -> int:
                     help="serving leg: grid sizes and nt scaled by this factor")
     ap.add_argument("--serve-load", type=float, default=0.5,
                     help="serving leg: arrival rate as a fraction of the measured throughput")
+    ap.add_argument("--serve-sweep", action="store_true",
+                    help="instead of the batched benchmark: the mixed request stream's "
+                         "closed-loop capacity, then Poisson arrivals at --serve-loads x that "
+                         "capacity, --serve requests per point, one JSON line per point")
+    ap.add_argument("--serve-loads", default="0.25,0.5,1,2,4")
+    ap.add_argument("--serve-groups", action="store_true",
+                    help="serving: launch groups of one launch key only (no lanes)")
     ap.add_argument("--c5-sweep", action="store_true",
                     help="instead of the batched benchmark: the large single elastic shot "
                          "(2000 x 6000, nt 10000), excitation vs zero_lag at checkpoint "
@@ -527,7 +591,7 @@ def main(argv=None) -> int:
 
     pool_baseline = None
     if (not args.no_cpu_baseline and not args.c5_sweep and not args.engine_sweep
-            and torch.cuda.device_count() > 0):
+            and not args.serve_sweep and torch.cuda.device_count() > 0):
         # before any HIP call: the pool forks
         print(f"# cpu baseline (process pool): oracle, nt={args.cpu_sample_nt}",
               file=sys.stderr, flush=True)
@@ -552,6 +616,11 @@ def main(argv=None) -> int:
         return 0
     if args.engine_sweep:
         engine_sweep(args.shots, args.sweep_steps)
+        return 0
+    if args.serve_sweep:
+        serving_sweep(args.serve, args.serve_scale,
+                      [float(v) for v in args.serve_loads.split(",")],
+                      lanes=not args.serve_groups)
         return 0
 
     if args.copy_probe_mib > 0:

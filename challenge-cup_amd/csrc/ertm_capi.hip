@@ -4,6 +4,8 @@
 // leaves a thread-local message for ertm_last_error().
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -213,14 +215,16 @@ extern "C" {
 
 int ertm_abi_version(void) { return ERTM_ABI_VERSION; }
 
-int ertm_struct_sizes(int64_t* out6) {
-    if (!out6) return fail(ERTM_EINVAL, "struct_sizes: NULL");
-    out6[0] = (int64_t)sizeof(ErtmProp);
-    out6[1] = (int64_t)sizeof(ErtmRecv);
-    out6[2] = (int64_t)sizeof(ErtmImage);
-    out6[3] = (int64_t)sizeof(ErtmRun);
-    out6[4] = (int64_t)offsetof(ErtmProp, state);
-    out6[5] = (int64_t)offsetof(ErtmRun, ctl);
+int ertm_struct_sizes(int64_t* out8) {
+    if (!out8) return fail(ERTM_EINVAL, "struct_sizes: NULL");
+    out8[0] = (int64_t)sizeof(ErtmProp);
+    out8[1] = (int64_t)sizeof(ErtmRecv);
+    out8[2] = (int64_t)sizeof(ErtmImage);
+    out8[3] = (int64_t)sizeof(ErtmRun);
+    out8[4] = (int64_t)offsetof(ErtmProp, state);
+    out8[5] = (int64_t)offsetof(ErtmRun, ctl);
+    out8[6] = (int64_t)sizeof(ErtmSeg);
+    out8[7] = (int64_t)offsetof(ErtmSeg, wavelet);
     return ERTM_OK;
 }
 
@@ -450,11 +454,49 @@ static bool pressure_fused_ok(ertm_batch_t b) {
     return L >= 1 && L <= 4;
 }
 
-static bool fused_image_ok(ertm_batch_t b, int op) {
+// The propagators a launch sequence covers: every one of the batch in table
+// order (Members == nullptr), or the listed ones (lanes), each at its own
+// parity pcur[p] and replay-slot origin pt0[p] (both indexed by propagator).
+struct Members {
+    const int* idx;
+    int n;
+    const int32_t* pcur;
+    const int32_t* pt0;
+};
+
+static int count_of(ertm_batch_t b, const Members* m) {
+    return m ? m->n : (int)b->props.size();
+}
+
+static int member_at(const Members* m, int i) { return m ? m->idx[i] : i; }
+
+// Chunk [i0, i0 + cnt) of the members into la: table indices, parities and
+// replay-slot origins (time and amplitude are the caller's to fill).
+static void fill_members(LaunchArgs& la, const Members* m, int i0, int cnt, int cur) {
+    la.n = cnt;
+    la.cur = cur;
+    if (!m) {
+        la.base = i0;
+        return;
+    }
+    la.base = 0;
+    la.cur = 0;
+    la.remap = 1;
+    for (int k = 0; k < cnt; ++k) {
+        const int p = m->idx[i0 + k];
+        la.slot[k] = p;
+        if (m->pcur[p]) la.flip |= (uint64_t)1 << k;
+        la.t0[k] = m->pt0 ? m->pt0[p] : 0;
+    }
+}
+
+static bool fused_image_ok(ertm_batch_t b, int op, const Members* m = nullptr) {
     if (op != ERTM_IMG_EXTRACT && op != ERTM_IMG_CORRELATE) return false;
+    const int cnt = count_of(b, m);
     // the fused pressure step extracts (its correlation goes through the ring)
     if (pressure_fused_ok(b) && b->variant >= 1 && b->variant <= 4 && op == ERTM_IMG_EXTRACT) {
-        for (size_t k = 0; k < b->props.size(); ++k) {
+        for (int i = 0; i < cnt; ++i) {
+            const int k = member_at(m, i);
             const ErtmProp& P = b->props[k];
             if (!b->img[k].rbuf || (int64_t)P.nz * P.nx * 4 >= ((int64_t)1 << 31) - 64) return false;
         }
@@ -462,7 +504,8 @@ static bool fused_image_ok(ertm_batch_t b, int op) {
     }
     if (b->variant < 1 || b->variant > 4 || !b->all_elastic || b->order != 8 || b->any_force)
         return false;
-    for (size_t k = 0; k < b->props.size(); ++k) {
+    for (int i = 0; i < cnt; ++i) {
+        const int k = member_at(m, i);
         const ErtmImage& I = b->img[k];
         const ErtmProp& P = b->props[k];
         if (!I.rbuf || I.nrw < (op == ERTM_IMG_CORRELATE ? 1 : 0) || I.nrw > 2) return false;
@@ -490,17 +533,19 @@ static bool fused_image_ok(ertm_batch_t b, int op) {
 // accumulators (same per-cell order as R correlation passes).
 #define ERTM_IMG_RING_STORE 100
 
-static bool ring_ok(ertm_batch_t b) {
+static bool ring_ok(ertm_batch_t b, const Members* m = nullptr) {
     if (b->variant < 1 || b->variant > 4 || !(b->all_elastic || pressure_fused_ok(b)) ||
         b->order != 8 || b->any_force)
         return false;
-    for (size_t k = 0; k < b->props.size(); ++k) {
+    const int cnt = count_of(b, m);
+    for (int i = 0; i < cnt; ++i) {
+        const int k = member_at(m, i);
         const ErtmImage& I = b->img[k];
         const ErtmProp& P = b->props[k];
         if (!I.rbuf || !I.rring || I.ring_len < 1 || I.nrw < 1 || I.nrw > 2 || !I.num[0])
             return false;
         if ((int64_t)P.nz * P.ld * 4 * I.nrw >= ((int64_t)1 << 31) - 64) return false;
-        if (I.ring_len != b->img[0].ring_len) return false;
+        if (I.ring_len != b->img[member_at(m, 0)].ring_len) return false;
     }
     return true;
 }
@@ -508,10 +553,10 @@ static bool ring_ok(ertm_batch_t b) {
 // Fused imaging kind for the kernel (ertm_fused.hip): extraction and ring
 // stores use their no-S variants (fewer registers) unless some propagator of
 // the launch images the divcurl S field.
-static int img_kind(ertm_batch_t b, int op, int base, int n) {
+static int img_kind(ertm_batch_t b, int op, const LaunchArgs& la) {
     bool s = false;
-    for (int k = base; k < base + n; ++k) {
-        const ErtmImage& I = b->img[k];
+    for (int i = 0; i < la.n; ++i) {
+        const ErtmImage& I = b->img[la_prop(la, i)];
         if (I.separation != ERTM_SEP_DIVCURL) continue;
         if (op == ERTM_IMG_EXTRACT) s = s || I.swave == ERTM_WAVE_S;
         else
@@ -541,72 +586,89 @@ static void use_clock(LaunchArgs& la, const DevClock* dc, int qoff = 0) {
     la.qoff = dc->q + qoff;
 }
 
-// Physical cells of propagators [base, base + n) of a batch.
+// Physical cells of the propagators of a launch.
 namespace {
-int64_t cells_of(ertm_batch_t b, int base, int n) {
+int64_t cells_of(ertm_batch_t b, const LaunchArgs& la) {
     int64_t c = 0;
-    for (int k = base; k < base + n; ++k) c += (int64_t)b->props[k].nz * b->props[k].nx;
+    for (int k = 0; k < la.n; ++k) {
+        const ErtmProp& P = b->props[la_prop(la, k)];
+        c += (int64_t)P.nz * P.nx;
+    }
     return c;
 }
+
+// One whole-state copy of propagator `prop` (for the timers' cell count).
+struct CopyItem {
+    const float* src;
+    float* dst;
+    int64_t floats;
+    int prop;
+};
 }  // namespace
 
-// Copy state[cur] of every propagator to (to_arena) or from its arena at
-// arenas[k] + slot * stride.  Aligned batches go through one k_copy_batch
-// launch per ERTM_MAX_LAUNCH propagators; otherwise one async copy each.
-static int copy_states(ertm_batch_t b, int cur, float* const* arenas, int64_t slot,
-                       int64_t stride, bool to_arena, hipStream_t s) {
-    const int n = (int)b->props.size();
+// The copies in one k_copy_batch launch per ERTM_MAX_LAUNCH of them when
+// every one is 16-byte aligned and a whole number of float4; otherwise one
+// async copy each.
+static int copy_list(ertm_batch_t b, const std::vector<CopyItem>& items, hipStream_t s) {
     bool fast = true;
-    for (int k = 0; k < n && fast; ++k) {
-        const ErtmProp& P = b->props[k];
-        fast = P.state_floats % 4 == 0 &&
-               ((uintptr_t)P.state[cur] | (uintptr_t)(arenas[k] + slot * stride)) % 16 == 0;
-    }
+    for (const CopyItem& c : items)
+        fast = fast && c.floats % 4 == 0 && ((uintptr_t)c.src | (uintptr_t)c.dst) % 16 == 0;
     if (!fast) {
-        for (int k = 0; k < n; ++k) {
-            const ErtmProp& P = b->props[k];
-            float* a = arenas[k] + slot * stride;
-            HIPCHK(hipMemcpyAsync(to_arena ? a : P.state[cur], to_arena ? P.state[cur] : a,
-                                  sizeof(float) * (size_t)P.state_floats,
+        for (const CopyItem& c : items)
+            HIPCHK(hipMemcpyAsync(c.dst, c.src, sizeof(float) * (size_t)c.floats,
                                   hipMemcpyDeviceToDevice, s));
-        }
         return ERTM_OK;
     }
-    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+    for (size_t base = 0; base < items.size(); base += ERTM_MAX_LAUNCH) {
         CopyArgs a{};
-        a.n = std::min(ERTM_MAX_LAUNCH, n - base);
+        a.n = (int)std::min<size_t>(ERTM_MAX_LAUNCH, items.size() - base);
+        int64_t cells = 0;
         for (int k = 0; k < a.n; ++k) {
-            const ErtmProp& P = b->props[base + k];
-            float* ar = arenas[base + k] + slot * stride;
-            a.src[k] = (const float4*)(to_arena ? P.state[cur] : ar);
-            a.dst[k] = (float4*)(to_arena ? ar : P.state[cur]);
-            a.n4[k] = P.state_floats / 4;
+            const CopyItem& c = items[base + k];
+            a.src[k] = (const float4*)c.src;
+            a.dst[k] = (float4*)c.dst;
+            a.n4[k] = c.floats / 4;
+            cells += (int64_t)b->props[c.prop].nz * b->props[c.prop].nx;
         }
         Timed tm(s, false);
         if (tm.status() != ERTM_OK) return tm.status();
         launch_copy_batch(a, s);
         LAUNCHCHK("copy_batch");
-        if (int rc = tm.done(ERTM_TK_COPY, cells_of(b, base, a.n))) return rc;
+        if (int rc = tm.done(ERTM_TK_COPY, cells)) return rc;
     }
     return ERTM_OK;
 }
 
+// Copy state[cur] of every propagator to (to_arena) or from its arena at
+// arenas[k] + slot * stride.
+static int copy_states(ertm_batch_t b, int cur, float* const* arenas, int64_t slot,
+                       int64_t stride, bool to_arena, hipStream_t s) {
+    std::vector<CopyItem> items;
+    for (size_t k = 0; k < b->props.size(); ++k) {
+        const ErtmProp& P = b->props[k];
+        float* a = arenas[k] + slot * stride;
+        items.push_back({to_arena ? P.state[cur] : a, to_arena ? a : P.state[cur], P.state_floats,
+                         (int)k});
+    }
+    return copy_list(b, items, s);
+}
+
+// t_hi, amp and t arrays of the do_* helpers are indexed by propagator.
 static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t s,
-                    const DevClock* dc = nullptr) {
-    const int n = (int)b->props.size();
-    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+                    const DevClock* dc = nullptr, const Members* m = nullptr) {
+    const int n = count_of(b, m);
+    for (int i0 = 0; i0 < n; i0 += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
-        la.base = base;
-        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
+        fill_members(la, m, i0, std::min(ERTM_MAX_LAUNCH, n - i0), 0);
         la.op = count;
-        for (int k = 0; k < la.n; ++k) la.t[k] = t_hi[base + k];
+        for (int k = 0; k < la.n; ++k) la.t[k] = t_hi[la_prop(la, k)];
         use_clock(la, dc, -(count - 1));  // the group's first (latest) step
         Timed tm(s, dc != nullptr);
         if (tm.status() != ERTM_OK) return tm.status();
         launch_image_flush(b->dprops, b->dimg, la, b->max_img, s);
         LAUNCHCHK("image_flush");
         // cells: the physical cells of the flushed steps
-        if (int rc = tm.done(ERTM_TK_FLUSH, count * cells_of(b, base, la.n))) return rc;
+        if (int rc = tm.done(ERTM_TK_FLUSH, count * cells_of(b, la))) return rc;
     }
     return ERTM_OK;
 }
@@ -627,21 +689,19 @@ static int step_kind(bool fused, int kind) {
 // also did that imaging op (t = time index per propagator).
 static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
                    int img_op = -1, const int32_t* t = nullptr, bool* imaged = nullptr,
-                   const DevClock* dc = nullptr) {
-    const int n = (int)b->props.size();
+                   const DevClock* dc = nullptr, const Members* m = nullptr) {
+    const int n = count_of(b, m);
     const bool fuse_img = img_op >= 0 && t &&
-                          (img_op == ERTM_IMG_RING_STORE ? ring_ok(b) : fused_image_ok(b, img_op));
+                          (img_op == ERTM_IMG_RING_STORE ? ring_ok(b, m) : fused_image_ok(b, img_op, m));
     if (imaged) *imaged = fuse_img;
-    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+    for (int i0 = 0; i0 < n; i0 += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
-        la.base = base;
-        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
-        la.cur = cur;
-        for (int k = 0; k < la.n; ++k) la.amp[k] = amp ? amp[base + k] : 0.0f;
+        fill_members(la, m, i0, std::min(ERTM_MAX_LAUNCH, n - i0), cur);
+        for (int k = 0; k < la.n; ++k) la.amp[k] = amp ? amp[la_prop(la, k)] : 0.0f;
         if (fuse_img)
-            for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+            for (int k = 0; k < la.n; ++k) la.t[k] = t[la_prop(la, k)];
         use_clock(la, dc);
-        const int kind = fuse_img ? img_kind(b, img_op, base, la.n) : 0;
+        const int kind = fuse_img ? img_kind(b, img_op, la) : 0;
         Timed tm(s, dc != nullptr);
         if (tm.status() != ERTM_OK) return tm.status();
         bool fused = b->variant != 0 &&
@@ -652,7 +712,7 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
             launch_step(b->dprops, la, b->order, b->max_tiles, b->any_force, s);
         }
         LAUNCHCHK("step");
-        if (int rc = tm.done(step_kind(fused, kind), cells_of(b, base, la.n))) return rc;
+        if (int rc = tm.done(step_kind(fused, kind), cells_of(b, la))) return rc;
     }
     return ERTM_OK;
 }
@@ -674,73 +734,67 @@ static int do_step2(ertm_batch_t b, int cur, const float* a1, const float* a2, h
             e2.psi1[k] = b->psi1_ptr[base + k];
             la.t[k] = t[base + k];
         }
-        const int kind = img_op == ERTM_IMG_EXTRACT ? img_kind(b, img_op, base, la.n) : 0;
+        const int kind = img_op == ERTM_IMG_EXTRACT ? img_kind(b, img_op, la) : 0;
         Timed tm(s, false);
         if (tm.status() != ERTM_OK) return tm.status();
         if (!launch_step2_fused(b->dprops, b->props.data(), la, e2, b->v2, b->zc2, b->dimg, kind,
                                 s))
             return fail(ERTM_ESTATE, "two-step launch refused");
         LAUNCHCHK("step2");
-        if (int rc = tm.done(kind ? ERTM_TK_EXTRACT2 : ERTM_TK_PLAIN2, 2 * cells_of(b, base, la.n)))
+        if (int rc = tm.done(kind ? ERTM_TK_EXTRACT2 : ERTM_TK_PLAIN2, 2 * cells_of(b, la)))
             return rc;
     }
     return ERTM_OK;
 }
 
 static int do_sample(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s,
-                     const DevClock* dc = nullptr) {
-    const int n = (int)b->props.size();
-    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+                     const DevClock* dc = nullptr, const Members* m = nullptr) {
+    const int n = count_of(b, m);
+    for (int i0 = 0; i0 < n; i0 += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
-        la.base = base;
-        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
-        la.cur = cur;
-        for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        fill_members(la, m, i0, std::min(ERTM_MAX_LAUNCH, n - i0), cur);
+        for (int k = 0; k < la.n; ++k) la.t[k] = t[la_prop(la, k)];
         use_clock(la, dc);
         Timed tm(s, dc != nullptr);
         if (tm.status() != ERTM_OK) return tm.status();
         launch_sample(b->dprops, b->drecv, la, b->max_samp, s);
         LAUNCHCHK("sample");
-        if (int rc = tm.done(ERTM_TK_SAMPLE, cells_of(b, base, la.n))) return rc;
+        if (int rc = tm.done(ERTM_TK_SAMPLE, cells_of(b, la))) return rc;
     }
     return ERTM_OK;
 }
 
 static int do_inject(ertm_batch_t b, int cur, const int32_t* t, hipStream_t s,
-                     const DevClock* dc = nullptr) {
-    const int n = (int)b->props.size();
-    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+                     const DevClock* dc = nullptr, const Members* m = nullptr) {
+    const int n = count_of(b, m);
+    for (int i0 = 0; i0 < n; i0 += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
-        la.base = base;
-        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
-        la.cur = cur;
-        for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        fill_members(la, m, i0, std::min(ERTM_MAX_LAUNCH, n - i0), cur);
+        for (int k = 0; k < la.n; ++k) la.t[k] = t[la_prop(la, k)];
         use_clock(la, dc);
         Timed tm(s, dc != nullptr);
         if (tm.status() != ERTM_OK) return tm.status();
         launch_inject(b->dprops, b->drecv, la, b->max_cells, s);
         LAUNCHCHK("inject");
-        if (int rc = tm.done(ERTM_TK_INJECT, cells_of(b, base, la.n))) return rc;
+        if (int rc = tm.done(ERTM_TK_INJECT, cells_of(b, la))) return rc;
     }
     return ERTM_OK;
 }
 
 static int do_image(ertm_batch_t b, int cur, const int32_t* t, int op, hipStream_t s,
-                    const DevClock* dc = nullptr) {
-    const int n = (int)b->props.size();
-    for (int base = 0; base < n; base += ERTM_MAX_LAUNCH) {
+                    const DevClock* dc = nullptr, const Members* m = nullptr) {
+    const int n = count_of(b, m);
+    for (int i0 = 0; i0 < n; i0 += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
-        la.base = base;
-        la.n = std::min(ERTM_MAX_LAUNCH, n - base);
-        la.cur = cur;
+        fill_members(la, m, i0, std::min(ERTM_MAX_LAUNCH, n - i0), cur);
         la.op = op;
-        for (int k = 0; k < la.n; ++k) la.t[k] = t[base + k];
+        for (int k = 0; k < la.n; ++k) la.t[k] = t[la_prop(la, k)];
         use_clock(la, dc);
         Timed tm(s, dc != nullptr);
         if (tm.status() != ERTM_OK) return tm.status();
         launch_image(b->dprops, b->dimg, la, b->order, b->max_img, s);
         LAUNCHCHK("image");
-        if (int rc = tm.done(ERTM_TK_IMAGE, cells_of(b, base, la.n))) return rc;
+        if (int rc = tm.done(ERTM_TK_IMAGE, cells_of(b, la))) return rc;
     }
     return ERTM_OK;
 }
@@ -1266,6 +1320,211 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
         q += 2;
     }
     if (cur_out) *cur_out = cur;
+    return ERTM_OK;
+}
+
+// Lanes (ertm_run_lanes): every lane's current segment advances one step per
+// tick; per tick the launches group the lanes by pass kind.
+static int check_seg(ertm_batch_t b, const ErtmSeg& g, int i) {
+    const int n = (int)b->props.size();
+    if (g.prop < 0 || g.prop >= n) return fail(ERTM_EINVAL, "lanes: segment %d: bad propagator %d", i, g.prop);
+    if (g.phase < 0 || g.phase > 2) return fail(ERTM_EINVAL, "lanes: segment %d: bad phase %d", i, g.phase);
+    if (g.t_begin < 0 || g.t_end < g.t_begin)
+        return fail(ERTM_EINVAL, "lanes: segment %d: bad time range", i);
+    if (g.done < 0 || g.done > g.t_end - g.t_begin)
+        return fail(ERTM_EINVAL, "lanes: segment %d: done %d outside its range", i, g.done);
+    const ErtmProp& P = b->props[g.prop];
+    const ErtmImage& I = b->img[g.prop];
+    if (g.phase != 2 && !g.wavelet) return fail(ERTM_EINVAL, "lanes: segment %d: wavelet missing", i);
+    if (g.phase == 1 && !g.restore) return fail(ERTM_EINVAL, "lanes: segment %d: replay without restore", i);
+    if (g.ck_interval < 0 || (g.phase == 2 && g.ck_interval != 0))
+        return fail(ERTM_EINVAL, "lanes: segment %d: bad checkpoint interval", i);
+    if (g.ck_interval > 0 && (!g.save_arena || g.save_stride < P.state_floats))
+        return fail(ERTM_EINVAL, "lanes: segment %d: checkpoint arena missing or stride below the state", i);
+    const int want = g.phase == 2 ? ERTM_IMG_CORRELATE : ERTM_IMG_EXTRACT;
+    if (g.op_after != -1 && g.op_after != want)
+        return fail(ERTM_EINVAL, "lanes: segment %d: image op %d not allowed in phase %d", i, g.op_after,
+                    g.phase);
+    if (g.op_after >= 0) {
+        if (!I.rbuf || g.img_t0 > g.t_begin || (int64_t)g.t_end - g.img_t0 > I.nslots)
+            return fail(ERTM_EINVAL, "lanes: segment %d: replay slots [%d, %d) outside the %d slots",
+                        i, g.t_begin - g.img_t0, g.t_end - g.img_t0, I.nslots);
+        if (g.op_after == ERTM_IMG_CORRELATE && (!I.num[0] || I.nrw < 1 || I.nrw > 2))
+            return fail(ERTM_EINVAL, "lanes: segment %d: no accumulators", i);
+    }
+    return ERTM_OK;
+}
+
+int ertm_run_lanes(ertm_batch_t b, ErtmSeg* segs, int32_t nseg, int32_t* cur, int32_t max_ticks,
+                   int32_t stop_at_lane_end, int64_t* stats3, void* stream) {
+    if (!b || nseg < 0 || (nseg > 0 && !segs) || !cur) return fail(ERTM_EINVAL, "lanes: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    b->note(s);
+    const int n = (int)b->props.size();
+    for (int k = 0; k < n; ++k)
+        if (cur[k] != 0 && cur[k] != 1) return fail(ERTM_EINVAL, "lanes: parity of prop %d not 0/1", k);
+    // lanes in order of first appearance; each propagator in one lane
+    std::vector<int> lane_of(n, INT32_MIN);
+    std::vector<std::vector<int>> lanes;
+    std::map<int, size_t> lane_ix;
+    std::vector<int> ring_members;
+    for (int i = 0; i < nseg; ++i) {
+        const ErtmSeg& g = segs[i];
+        if (int rc = check_seg(b, g, i)) return rc;
+        if (lane_of[g.prop] != INT32_MIN && lane_of[g.prop] != g.lane)
+            return fail(ERTM_EINVAL, "lanes: propagator %d is in lanes %d and %d", g.prop,
+                        lane_of[g.prop], g.lane);
+        lane_of[g.prop] = g.lane;
+        auto it = lane_ix.find(g.lane);
+        if (it == lane_ix.end()) {
+            it = lane_ix.emplace(g.lane, lanes.size()).first;
+            lanes.emplace_back();
+        }
+        lanes[it->second].push_back(i);
+        if (g.phase == 2 && g.op_after == ERTM_IMG_CORRELATE &&
+            std::find(ring_members.begin(), ring_members.end(), g.prop) == ring_members.end())
+            ring_members.push_back(g.prop);
+    }
+    // one decision for every backward lane, so a propagator's accumulation
+    // order never switches between the ring and direct correlation
+    const Members all_back{ring_members.data(), (int)ring_members.size(), cur, nullptr};
+    const bool use_ring = !ring_members.empty() && ring_ok(b, &all_back);
+    const int ring_len = use_ring ? b->img[ring_members[0]].ring_len : 0;
+    auto len_of = [&](int i) { return segs[i].t_end - segs[i].t_begin; };
+    std::vector<size_t> pos(lanes.size(), 0);
+    auto skip_done = [&](size_t l) {
+        while (pos[l] < lanes[l].size() && segs[lanes[l][pos[l]]].done >= len_of(lanes[l][pos[l]]))
+            ++pos[l];
+    };
+    for (size_t l = 0; l < lanes.size(); ++l) skip_done(l);
+
+    std::vector<int32_t> tt(n, 0), t0v(n, 0), pend(n, 0), thi(n, 0);
+    std::vector<float> amp(n, 0.0f);
+    int rc = ERTM_OK;
+    auto flush = [&](std::vector<int>& due) -> int {
+        // one launch per distinct count of pending steps
+        std::sort(due.begin(), due.end(), [&](int a, int c) { return pend[a] < pend[c]; });
+        for (size_t i = 0; i < due.size();) {
+            size_t j = i;
+            while (j < due.size() && pend[due[j]] == pend[due[i]]) ++j;
+            std::vector<int> grp(due.begin() + i, due.begin() + j);
+            std::sort(grp.begin(), grp.end());
+            const Members m{grp.data(), (int)grp.size(), cur, t0v.data()};
+            if (int rc2 = do_flush(b, thi.data(), pend[grp[0]], s, nullptr, &m)) return rc2;
+            for (int p : grp) pend[p] = 0;
+            i = j;
+        }
+        due.clear();
+        return ERTM_OK;
+    };
+    int ticks = 0;
+    int64_t lane_steps = 0, step_groups = 0;
+    bool stop = false;
+    std::vector<int> act, inj, fplain, fext, bimg, bplain, due;
+    std::vector<CopyItem> copies;
+    while (!stop && (max_ticks <= 0 || ticks < max_ticks)) {
+        act.clear();
+        for (size_t l = 0; l < lanes.size(); ++l)
+            if (pos[l] < lanes[l].size()) act.push_back(lanes[l][pos[l]]);
+        if (act.empty()) break;
+        inj.clear(), fplain.clear(), fext.clear(), bimg.clear(), bplain.clear(), copies.clear();
+        for (int i : act) {
+            const ErtmSeg& g = segs[i];
+            const int p = g.prop;
+            const int t = g.phase == 2 ? g.t_end - 1 - g.done : g.t_begin + g.done;
+            tt[p] = t;
+            amp[p] = g.phase == 2 ? 0.0f : g.wavelet[t];
+            t0v[p] = g.img_t0;
+            if (g.phase == 1 && g.done == 0)
+                copies.push_back({g.restore, b->props[p].state[cur[p]], b->props[p].state_floats, p});
+            if (g.phase == 2) {
+                inj.push_back(p);
+                (g.op_after >= 0 ? bimg : bplain).push_back(p);
+            } else {
+                (g.op_after >= 0 ? fext : fplain).push_back(p);
+            }
+        }
+        for (auto* v : {&inj, &fplain, &fext, &bimg, &bplain}) std::sort(v->begin(), v->end());
+        if (!copies.empty() && (rc = copy_list(b, copies, s)) != ERTM_OK) return rc;
+        auto mem = [&](const std::vector<int>& v) {
+            return Members{v.data(), (int)v.size(), cur, t0v.data()};
+        };
+        if (!inj.empty()) {
+            const Members m = mem(inj);
+            if ((rc = do_inject(b, 0, tt.data(), s, nullptr, &m)) != ERTM_OK) return rc;
+        }
+        bool ext_done = false, back_done = false;
+        if (!fplain.empty()) {
+            const Members m = mem(fplain);
+            if ((rc = do_step(b, 0, amp.data(), s, -1, nullptr, nullptr, nullptr, &m)) != ERTM_OK) return rc;
+        }
+        if (!fext.empty()) {
+            const Members m = mem(fext);
+            if ((rc = do_step(b, 0, amp.data(), s, ERTM_IMG_EXTRACT, tt.data(), &ext_done, nullptr, &m)) !=
+                ERTM_OK)
+                return rc;
+        }
+        if (!bimg.empty()) {
+            const Members m = mem(bimg);
+            const int op = use_ring ? ERTM_IMG_RING_STORE : ERTM_IMG_CORRELATE;
+            if ((rc = do_step(b, 0, amp.data(), s, op, tt.data(), &back_done, nullptr, &m)) != ERTM_OK)
+                return rc;
+            if (use_ring && !back_done) return fail(ERTM_ESTATE, "lanes: ring imaging launch refused");
+        }
+        if (!bplain.empty()) {
+            const Members m = mem(bplain);
+            if ((rc = do_step(b, 0, amp.data(), s, -1, nullptr, nullptr, nullptr, &m)) != ERTM_OK) return rc;
+        }
+        for (int i : act) cur[segs[i].prop] ^= 1;
+        lane_steps += (int64_t)act.size();
+        for (auto* v : {&fplain, &fext, &bimg, &bplain}) step_groups += v->empty() ? 0 : 1;
+        if (use_ring) {
+            for (int i : act) {
+                const ErtmSeg& g = segs[i];
+                if (g.phase != 2 || g.op_after < 0) continue;
+                if (pend[g.prop] == 0) thi[g.prop] = tt[g.prop];
+                if (++pend[g.prop] == ring_len || g.done + 1 == len_of(i)) due.push_back(g.prop);
+            }
+            if (!due.empty() && (rc = flush(due)) != ERTM_OK) return rc;
+        }
+        copies.clear();
+        for (int i : act) {
+            const ErtmSeg& g = segs[i];
+            const int t = tt[g.prop];
+            if (g.phase == 2 || g.ck_interval <= 0 || t < g.ck_origin ||
+                (t - g.ck_origin) % g.ck_interval != 0)
+                continue;
+            const ErtmProp& P = b->props[g.prop];
+            copies.push_back({P.state[cur[g.prop]],
+                              g.save_arena + (int64_t)((t - g.ck_origin) / g.ck_interval) * g.save_stride,
+                              P.state_floats, g.prop});
+        }
+        if (!copies.empty() && (rc = copy_list(b, copies, s)) != ERTM_OK) return rc;
+        if (!fext.empty() && !ext_done) {
+            const Members m = mem(fext);
+            if ((rc = do_image(b, 0, tt.data(), ERTM_IMG_EXTRACT, s, nullptr, &m)) != ERTM_OK) return rc;
+        }
+        if (!bimg.empty() && !back_done) {
+            const Members m = mem(bimg);
+            if ((rc = do_image(b, 0, tt.data(), ERTM_IMG_CORRELATE, s, nullptr, &m)) != ERTM_OK) return rc;
+        }
+        for (size_t l = 0; l < lanes.size(); ++l) {
+            if (pos[l] >= lanes[l].size()) continue;
+            ++segs[lanes[l][pos[l]]].done;
+            skip_done(l);
+            if (pos[l] >= lanes[l].size() && stop_at_lane_end) stop = true;
+        }
+        ++ticks;
+    }
+    // a call that stops inside a backward segment folds its ring steps now
+    for (int p : ring_members)
+        if (pend[p] > 0) due.push_back(p);
+    if (!due.empty() && (rc = flush(due)) != ERTM_OK) return rc;
+    if (stats3) {
+        stats3[0] = ticks;
+        stats3[1] = lane_steps;
+        stats3[2] = step_groups;
+    }
     return ERTM_OK;
 }
 

@@ -370,7 +370,7 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
     const int lin = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
     const int ys = lin / gridDim.x;
     const int bid = lin - ys * gridDim.x;
-    const int slot = la.base + ys;
+    const int slot = la_prop(la, ys);
     const ErtmProp& P = props[slot];
     const int nzp = P.nzp, nxp = P.nxp;
     const int nbands = (nxp + TX - 1) / TX;
@@ -393,8 +393,9 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
     B.ld4 = 4 * P.ld;
     const int fs4 = (int)(4 * P.fstride);  // bytes between field slabs
     const int64_t sbytes = 4 * P.state_floats;
-    B.rS = make_rsrc(P.state[la.cur], sbytes);
-    B.rT = make_rsrc(P.state[la.cur ^ 1], sbytes);
+    const int cur = la_cur(la, ys);
+    B.rS = make_rsrc(P.state[cur], sbytes);
+    B.rT = make_rsrc(P.state[cur ^ 1], sbytes);
     const int64_t mbytes = (int64_t)nzp * P.ld * 4;
     B.rIR = make_rsrc(P.irho, mbytes);
     B.rLAM = make_rsrc(P.lam, mbytes);
@@ -455,7 +456,7 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
                              4 * nring * I.nrw);
             B.o_num1 = (int)(4 * nring);
         } else {
-            B.rR = make_rsrc(I.rbuf + (int64_t)(t - I.t0) * ncell, 4 * ncell);
+            B.rR = make_rsrc(I.rbuf + (int64_t)(t - la_t0(la, ys, I)) * ncell, 4 * ncell);
         }
         B.sep = I.separation;
         B.swave = I.swave;
@@ -761,7 +762,8 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
     const int lin = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
     const int ys = lin / gridDim.x;
     const int bid = lin - ys * gridDim.x;
-    const ErtmProp& P = props[la.base + ys];
+    const int slot = la_prop(la, ys);
+    const ErtmProp& P = props[slot];
     const int nzp = P.nzp, nxp = P.nxp;
     const int nbands = (nxp + TX - 1) / TX;
     const int nchunks = (nzp + zc - 1) / zc;
@@ -783,8 +785,9 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
     B.ld4 = 4 * P.ld;
     const int fs4 = (int)(4 * P.fstride);
     const int64_t sbytes = 4 * P.state_floats;
-    B.rS = make_rsrc(P.state[la.cur], sbytes);
-    B.rT = make_rsrc(P.state[la.cur ^ 1], sbytes);
+    const int cur = la_cur(la, ys);
+    B.rS = make_rsrc(P.state[cur], sbytes);
+    B.rT = make_rsrc(P.state[cur ^ 1], sbytes);
     const int64_t mbytes = (int64_t)nzp * P.ld * 4;
     B.rIR = make_rsrc(P.irho, mbytes);
     B.rKA = make_rsrc(P.kappa, mbytes);
@@ -831,7 +834,7 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
     B.axc = B.colv ? gp(P.ax)[B.c] : 0.0f;
     B.bxc = B.colv ? gp(P.bx)[B.c] : 1.0f;
     if constexpr (IMG != ERTM_FIMG_NONE) {
-        const ErtmImage& I = imgs[la.base + ys];
+        const ErtmImage& I = imgs[slot];
         const int t = la_time(la, ys);
         const int64_t ncell = (int64_t)P.nz * P.nx;
         B.pad = P.pad;
@@ -846,7 +849,7 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
             B.rR = make_rsrc(I.rring + (int64_t)(t % I.ring_len) * I.nrw * nring,
                              4 * nring * I.nrw);
         else
-            B.rR = make_rsrc(I.rbuf + (int64_t)(t - I.t0) * ncell, 4 * ncell);
+            B.rR = make_rsrc(I.rbuf + (int64_t)(t - la_t0(la, ys, I)) * ncell, 4 * ncell);
     }
 
     __shared__ float lds[2 * 3 * NT];
@@ -882,7 +885,7 @@ static bool launch_acoustic_k(const ErtmProp* dprops, const ErtmImage* dimg,
                               const ErtmProp* hprops, const LaunchArgs& la, int zc, int img,
                               hipStream_t s) {
     int nb = 0;
-    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la.base + k], zc));
+    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la_prop(la, k)], zc));
     const dim3 g(nb, la.n);
     if (img == ERTM_FIMG_NONE)
         hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_NONE, VL>), g, dim3(NT), 0, s,
@@ -919,7 +922,7 @@ template <int K, int NT, int IMG>
 static void launch_fused_k(const ErtmProp* dprops, const ErtmImage* dimg,
                            const ErtmProp* hprops, const LaunchArgs& la, int zc, hipStream_t s) {
     int nb = 0;
-    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la.base + k], zc));
+    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la_prop(la, k)], zc));
     hipLaunchKernelGGL((k_elastic_fused<K, NT, IMG>), dim3(nb, la.n), dim3(NT), 0, s, dprops, dimg,
                        la, zc);
 }
@@ -950,14 +953,14 @@ bool launch_step_fused(const ErtmProp* dprops, const ErtmProp* hprops, const Lau
     // stay below 2^31; every state buffer below 2^30 bytes keeps ERTM_BAD out
     // of range
     for (int k = 0; k < la.n; ++k) {
-        const ErtmProp& P = hprops[la.base + k];
+        const ErtmProp& P = hprops[la_prop(la, k)];
         if (4 * P.state_floats + 128 * (int64_t)P.ld >= ((int64_t)1 << 30)) return false;
     }
     // one engine (and mechanism count) per launch
-    const int engine = hprops[la.base].engine, nmech = hprops[la.base].nmech;
+    const int engine = hprops[la_prop(la, 0)].engine, nmech = hprops[la_prop(la, 0)].nmech;
     for (int k = 0; k < la.n; ++k)
-        if (hprops[la.base + k].engine != engine ||
-            (engine == ERTM_VISCO && hprops[la.base + k].nmech != nmech))
+        if (hprops[la_prop(la, k)].engine != engine ||
+            (engine == ERTM_VISCO && hprops[la_prop(la, k)].nmech != nmech))
             return false;
     if (engine == ERTM_ACOUSTIC || engine == ERTM_VISCO) {
         switch (variant) {

@@ -489,7 +489,7 @@ k_elastic_2step(const ErtmProp* __restrict__ props, const ErtmImage* __restrict_
     const int lin = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
     const int ys = lin / gridDim.x;
     const int bid = lin - ys * gridDim.x;
-    const int slot = la.base + ys;
+    const int slot = la_prop(la, ys);
     const ErtmProp& P = props[slot];
     const int nzp = P.nzp, nxp = P.nxp;
     const int nbands = (nxp + TX - 1) / TX;
@@ -516,8 +516,9 @@ k_elastic_2step(const ErtmProp* __restrict__ props, const ErtmImage* __restrict_
     B.ld4 = 4 * P.ld;
     const int fs4 = (int)(4 * P.fstride);
     const int64_t sbytes = 4 * P.state_floats;
-    B.rS = make_rsrc(P.state[la.cur], sbytes);
-    B.rT = make_rsrc(P.state[la.cur ^ 1], sbytes);
+    const int cur = la_cur(la, ys);
+    B.rS = make_rsrc(P.state[cur], sbytes);
+    B.rT = make_rsrc(P.state[cur ^ 1], sbytes);
     // the scratch holds the CPML part of a state buffer: byte offsets in it
     // are state offsets minus psi0 (offsets below psi0 wrap out of range)
     E.psi0 = P.nfields * fs4;
@@ -576,8 +577,9 @@ k_elastic_2step(const ErtmProp* __restrict__ props, const ErtmImage* __restrict_
         B.nxph = P.nx;
         B.jcol = (E.own2 && B.c >= P.pad && B.c < P.pad + P.nx) ? B.c - P.pad : -1;
         const int64_t ncell = (int64_t)P.nz * P.nx;
-        B.rR = make_rsrc(I.rbuf + (int64_t)(t - I.t0) * ncell, 4 * ncell);
-        E.rR2 = make_rsrc(I.rbuf + (int64_t)(t + 1 - I.t0) * ncell, 4 * ncell);
+        const int t0 = la_t0(la, ys, I);
+        B.rR = make_rsrc(I.rbuf + (int64_t)(t - t0) * ncell, 4 * ncell);
+        E.rR2 = make_rsrc(I.rbuf + (int64_t)(t + 1 - t0) * ncell, 4 * ncell);
         B.sep = I.separation;
         B.swave = I.swave;
     }
@@ -609,7 +611,7 @@ template <int K, int NT, int IMG>
 static void launch2_k(const ErtmProp* dprops, const ErtmImage* dimg, const ErtmProp* hprops,
                       const LaunchArgs& la, const El2Args& a2, int zc, hipStream_t s) {
     int nb = 0;
-    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks2_for<K, NT>(hprops[la.base + k], zc));
+    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks2_for<K, NT>(hprops[la_prop(la, k)], zc));
     hipLaunchKernelGGL((k_elastic_2step<K, NT, IMG>), dim3(nb, la.n), dim3(NT), 0, s, dprops,
                        dimg, la, a2, zc);
 }
@@ -634,7 +636,7 @@ bool launch_step2_fused(const ErtmProp* dprops, const ErtmProp* hprops, const La
                         hipStream_t s) {
     if (zc < 8) return false;
     for (int k = 0; k < la.n; ++k) {
-        const ErtmProp& P = hprops[la.base + k];
+        const ErtmProp& P = hprops[la_prop(la, k)];
         if (P.engine != ERTM_ELASTIC || P.order != 8 || !a2.psi1[k]) return false;
         if (4 * P.state_floats + 128 * (int64_t)P.ld >= ((int64_t)1 << 30)) return false;
     }

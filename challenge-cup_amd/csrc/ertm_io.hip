@@ -20,12 +20,12 @@ __device__ __forceinline__ float sample_value(const ErtmProp& P, const float* S,
 
 __global__ void k_sample(const ErtmProp* __restrict__ props, const ErtmRecv* __restrict__ recv,
                          LaunchArgs la) {
-    const int slot = la.base + blockIdx.y;
+    const int slot = la_prop(la, blockIdx.y);
     const ErtmRecv& R = recv[slot];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= R.nsamp) return;
     const ErtmProp& P = props[slot];
-    const float* S = P.state[la.cur];
+    const float* S = P.state[la_cur(la, blockIdx.y)];
     const int t = la_time(la, blockIdx.y);
     const int field = R.s_field[e];
     double v = 0.0;
@@ -64,12 +64,12 @@ void launch_sample(const ErtmProp* dprops, const ErtmRecv* drecv, const LaunchAr
 // ---------------------------------------------------------------------------
 __global__ void k_inject(const ErtmProp* __restrict__ props, const ErtmRecv* __restrict__ recv,
                          LaunchArgs la) {
-    const int slot = la.base + blockIdx.y;
+    const int slot = la_prop(la, blockIdx.y);
     const ErtmRecv& R = recv[slot];
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= R.ncells) return;
     const ErtmProp& P = props[slot];
-    float* S = P.state[la.cur];
+    float* S = P.state[la_cur(la, blockIdx.y)];
     const int t = la_time(la, blockIdx.y);
     float* f = S + (int64_t)R.cell_field[u] * P.fstride + R.cell_off[u];
     float x = *f;
@@ -119,7 +119,7 @@ __device__ __forceinline__ float wave_value(const ErtmProp& P, const float* S, i
 template <int K>
 __global__ void k_image(const ErtmProp* __restrict__ props, const ErtmImage* __restrict__ imgs,
                         LaunchArgs la) {
-    const int slot = la.base + blockIdx.y;
+    const int slot = la_prop(la, blockIdx.y);
     const ErtmProp& P = props[slot];
     const ErtmImage& I = imgs[slot];
     const int64_t ncell = (int64_t)P.nz * P.nx;
@@ -127,15 +127,16 @@ __global__ void k_image(const ErtmProp* __restrict__ props, const ErtmImage* __r
     if (q >= ncell) return;
     const int i = (int)(q / P.nx), j = (int)(q - (int64_t)i * P.nx);
     const int64_t o = (int64_t)(i + P.pad) * P.ld + (j + P.pad);
-    const float* S = P.state[la.cur];
+    const float* S = P.state[la_cur(la, blockIdx.y)];
     const int t = la_time(la, blockIdx.y);
+    const int t0 = la_t0(la, blockIdx.y, I);
     switch (la.op) {
         case ERTM_IMG_EXTRACT: {
-            I.rbuf[(int64_t)(t - I.t0) * ncell + q] = wave_value<K>(P, S, I.separation, I.swave, o);
+            I.rbuf[(int64_t)(t - t0) * ncell + q] = wave_value<K>(P, S, I.separation, I.swave, o);
             break;
         }
         case ERTM_IMG_CORRELATE: {
-            const float sv = I.rbuf[(int64_t)(t - I.t0) * ncell + q];
+            const float sv = I.rbuf[(int64_t)(t - t0) * ncell + q];
             const bool cross = t >= I.start_step;
             if (I.illum & ERTM_ILLUM_SRC) {
                 float ss = sv * sv;
@@ -186,13 +187,14 @@ __global__ void k_image(const ErtmProp* __restrict__ props, const ErtmImage* __r
 // same float32 products, as `count` ERTM_IMG_CORRELATE passes.
 __global__ void k_image_flush(const ErtmProp* __restrict__ props,
                               const ErtmImage* __restrict__ imgs, LaunchArgs la) {
-    const int slot = la.base + blockIdx.y;
+    const int slot = la_prop(la, blockIdx.y);
     const ErtmProp& P = props[slot];
     const ErtmImage& I = imgs[slot];
     const int64_t ncell = (int64_t)P.nz * P.nx;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ncell) return;
     const int t_hi = la_time(la, blockIdx.y), count = la.op;
+    const int t0 = la_t0(la, blockIdx.y, I);
     const bool src = (I.illum & ERTM_ILLUM_SRC) != 0, rec = (I.illum & ERTM_ILLUM_REC) != 0;
     const int nrw = I.nrw;
     double ill = src ? I.ill[q] : 0.0;
@@ -204,7 +206,7 @@ __global__ void k_image_flush(const ErtmProp* __restrict__ props,
     const int64_t qr = qi * P.ld + (q - qi * P.nx) + P.pad;
     for (int j = 0; j < count; ++j) {
         const int t = t_hi - j;
-        const float sv = I.rbuf[(int64_t)(t - I.t0) * ncell + q];
+        const float sv = I.rbuf[(int64_t)(t - t0) * ncell + q];
         const float* ring = I.rring + (int64_t)(t % I.ring_len) * nrw * nring;
         if (src) {
             float ss = sv * sv;
@@ -380,11 +382,11 @@ void launch_stab_control(const ErtmProp*, const LaunchArgs& la, const float* pea
 
 __global__ void k_stab_scale(const ErtmProp* __restrict__ props, LaunchArgs la,
                              const float* __restrict__ scales) {
-    const int slot = la.base + blockIdx.y;
+    const int slot = la_prop(la, blockIdx.y);
     const float sc = scales[blockIdx.y];
     if (sc == 1.0f) return;
     const ErtmProp& P = props[slot];
-    float* S = P.state[la.cur];
+    float* S = P.state[la_cur(la, blockIdx.y)];
     const int64_t cells = (int64_t)P.nzp * P.ld;
     const int nf = 4 + P.nmech;  // p, vx, vz, eps, r_0..r_{L-1}
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cells;
@@ -642,10 +644,10 @@ __global__ void k_checkpoint(const ErtmProp* __restrict__ props, LaunchArgs la,
     const int k = blockIdx.y;
     const int t = la_time(la, k);
     if (t < origin || (t - origin) % interval != 0) return;
-    const ErtmProp& P = props[la.base + k];
+    const ErtmProp& P = props[la_prop(la, k)];
     const int64_t slot = (t - origin) / interval;
-    const float4* src = (const float4*)P.state[la.cur];
-    float4* dst = (float4*)(arenas[la.base + k] + slot * stride);
+    const float4* src = (const float4*)P.state[la_cur(la, k)];
+    float4* dst = (float4*)(arenas[la_prop(la, k)] + slot * stride);
     const int64_t n4 = P.state_floats / 4;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
          i += (int64_t)gridDim.x * blockDim.x)

@@ -16,6 +16,10 @@ namespace ertm {
 // after each replay; qoff the launch's step within the chunk), every
 // propagator is at time clk[1] + clk[2] * q and its amplitude is
 // ta[q * ntab + base + k] (0 without a table).
+//
+// Lanes (remap = 1, never with the device clock): member k of the launch is
+// propagator slot[k] of the table, at parity cur ^ bit k of flip, and its
+// replay slots start at time t0[k] instead of its image descriptor's t0.
 struct LaunchArgs {
     int32_t base;
     int32_t n;
@@ -27,7 +31,26 @@ struct LaunchArgs {
     const float* ta;
     int32_t ntab;
     int32_t qoff;
+    int32_t remap;
+    uint64_t flip;
+    int32_t slot[ERTM_MAX_LAUNCH];
+    int32_t t0[ERTM_MAX_LAUNCH];
 };
+
+// Table index of launch member k.
+__host__ __device__ __forceinline__ int la_prop(const LaunchArgs& la, int k) {
+    return la.remap ? la.slot[k] : la.base + k;
+}
+
+// State parity member k reads (it writes the other one).
+__host__ __device__ __forceinline__ int la_cur(const LaunchArgs& la, int k) {
+    return la.cur ^ (int)((la.flip >> k) & 1u);
+}
+
+// Time index of replay slot 0 for member k.
+__device__ __forceinline__ int la_t0(const LaunchArgs& la, int k, const ErtmImage& I) {
+    return la.remap ? la.t0[k] : I.t0;
+}
 
 __device__ __forceinline__ int la_time(const LaunchArgs& la, int k) {
     if (!la.clk) return la.t[k];

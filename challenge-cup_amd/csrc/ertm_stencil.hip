@@ -23,9 +23,9 @@ struct Cell {
 
 // Block -> (propagator, cell).  Block (64 x 4 threads) covers a 64 x 4 tile;
 // blockIdx.y is the propagator inside the launch chunk.
-__device__ __forceinline__ Cell cell_of(const ErtmProp* props, int base) {
+__device__ __forceinline__ Cell cell_of(const ErtmProp* props, const LaunchArgs& la) {
     Cell c;
-    c.slot = base + blockIdx.y;
+    c.slot = la_prop(la, blockIdx.y);
     const ErtmProp& P = props[c.slot];
     int nbx = (P.nxp + 63) >> 6;
     int bz = blockIdx.x / nbx;
@@ -42,12 +42,13 @@ __device__ __forceinline__ Cell cell_of(const ErtmProp* props, int base) {
 template <int K>
 __global__ void __launch_bounds__(256) k_velocity(const ErtmProp* __restrict__ props,
                                                   LaunchArgs la) {
-    Cell c = cell_of(props, la.base);
+    Cell c = cell_of(props, la);
     if (!c.ok) return;
     const ErtmProp& P = props[c.slot];
     const int i = c.i, j = c.j;
-    const float* __restrict__ S = P.state[la.cur];
-    float* __restrict__ T = P.state[la.cur ^ 1];
+    const int cur = la_cur(la, blockIdx.y);
+    const float* __restrict__ S = P.state[cur];
+    float* __restrict__ T = P.state[cur ^ 1];
     const int64_t ld = P.ld;
     const int64_t o = (int64_t)i * ld + j;
     const float dt = P.dt;
@@ -102,12 +103,13 @@ __global__ void __launch_bounds__(256) k_velocity(const ErtmProp* __restrict__ p
 template <int K>
 __global__ void __launch_bounds__(256) k_stress(const ErtmProp* __restrict__ props,
                                                 LaunchArgs la) {
-    Cell c = cell_of(props, la.base);
+    Cell c = cell_of(props, la);
     if (!c.ok) return;
     const ErtmProp& P = props[c.slot];
     const int i = c.i, j = c.j;
-    const float* __restrict__ S = P.state[la.cur];
-    float* __restrict__ T = P.state[la.cur ^ 1];
+    const int cur = la_cur(la, blockIdx.y);
+    const float* __restrict__ S = P.state[cur];
+    float* __restrict__ T = P.state[cur ^ 1];
     const int64_t ld = P.ld;
     const int64_t o = (int64_t)i * ld + j;
     const float dt = P.dt;
@@ -222,13 +224,13 @@ __global__ void __launch_bounds__(256) k_stress(const ErtmProp* __restrict__ pro
 
 // force_z source on vz after the stress update (ertm/propagation.py:870-871).
 __global__ void k_point_source(const ErtmProp* __restrict__ props, LaunchArgs la) {
-    const int slot = la.base + blockIdx.x;
+    const int slot = la_prop(la, blockIdx.x);
     const ErtmProp& P = props[slot];
     const float amp = la_amp(la, blockIdx.x);
     if (threadIdx.x != 0 || amp == 0.0f || P.engine != ERTM_ELASTIC ||
         P.mechanism != ERTM_SRC_FORCE_Z || P.src_iz < 0)
         return;
-    float* T = P.state[la.cur ^ 1] + (int64_t)ERTM_EL_VZ * P.fstride;
+    float* T = P.state[la_cur(la, blockIdx.x) ^ 1] + (int64_t)ERTM_EL_VZ * P.fstride;
     const int64_t o = (int64_t)P.src_iz * P.ld + P.src_ix;
     T[o] = T[o] + amp;
 }

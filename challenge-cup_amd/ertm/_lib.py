@@ -16,7 +16,7 @@ from .exceptions import ERTMError, SolverError
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libertm_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAX_LAUNCH = 64
 MAX_MECH = 8
 NOALLOC_MAX_CELLS = 1 << 21  # ERTM_NOALLOC_MAX_CELLS: image cells of ertm_postprocess_noalloc
@@ -102,6 +102,16 @@ class ErtmRun(C.Structure):
     ]
 
 
+class ErtmSeg(C.Structure):
+    _fields_ = [
+        ("lane", C.c_int32), ("prop", C.c_int32), ("phase", C.c_int32), ("t_begin", C.c_int32),
+        ("t_end", C.c_int32), ("op_after", C.c_int32), ("img_t0", C.c_int32),
+        ("ck_interval", C.c_int32), ("ck_origin", C.c_int32), ("done", C.c_int32),
+        ("save_stride", C.c_int64), ("save_arena", C.c_void_p), ("restore", C.c_void_p),
+        ("wavelet", C.c_void_p),
+    ]
+
+
 # (name, restype, argtypes) of every exported entry point; the CPU test suite
 # checks the library exports exactly these.
 EXPORTS = [
@@ -123,6 +133,8 @@ EXPORTS = [
     ("ertm_image", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_void_p]),
     ("ertm_copy_state", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     ("ertm_run", C.c_int, [C.c_void_p, C.POINTER(ErtmRun), C.POINTER(C.c_int32), C.c_void_p]),
+    ("ertm_run_lanes", C.c_int, [C.c_void_p, C.POINTER(ErtmSeg), C.c_int32, C.POINTER(C.c_int32),
+                                 C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.c_void_p]),
     ("ertm_derivative", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_void_p]),
     ("ertm_absmax", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),

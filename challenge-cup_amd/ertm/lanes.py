@@ -1,0 +1,390 @@
+"""Lanes: shot migrations on independent schedules advanced in lockstep.
+
+``RTMImager.run_batch`` advances shots that share nt and the checkpoint
+interval, one ertm_run per pass.  A lane is one shot's whole migration
+written as a list of passes (segments) of its two propagators: the
+checkpointing forward pass, then per checkpoint block, last block first, the
+replay that re-creates the source field into the replay slots and the
+backward pass that injects the record and correlates (ertm/imaging.py:
+113-162, :279-324; sub-checkpoints split a block whose replay slots exceed
+the replay budget, exactly as ``RTMImager._run_shots`` does).
+
+``LaneRunner`` keeps a set of lanes of one engine on the current stream and
+runs them with ertm_run_lanes: every tick advances each lane by one step,
+whatever its nt, dt, grid, checkpoint interval, block or pass, and the step
+launches of a tick cover every lane in the same kind of pass.  Lanes join
+between calls (continuous batching) and leave when their last pass ends.
+A lane's image equals the one ``RTMImager.migrate`` gives for its shot
+alone: each propagator runs the same kernels on the same inputs in the same
+order; only which other shots share its launches changes.
+
+Supported: acoustic and elastic engines, conditions ``zero_lag`` and
+``normalized`` (the excitation condition and the viscoacoustic stabiliser
+keep their own loops in ``RTMImager``).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from typing import Any
+
+import numpy as np
+
+from . import _lib as L
+from . import device as dev
+from .exceptions import ImagingError
+from .imaging import CheckpointStore, PreparedBatch, RTMImager, checkpoint_interval
+from .propagation import build_propagator
+from .types import ImageResult
+
+_GIB = 1024.0 ** 3
+LANE_ENGINES = ("acoustic", "elastic")
+
+
+def lanes_supported(imager: RTMImager) -> bool:
+    """Whether ``imager``'s migrations can run as lanes."""
+    return (imager.solver_config.engine in LANE_ENGINES
+            and str(imager.config.condition).lower() != "excitation")
+
+
+def lane_replay_slots(imager: RTMImager, interval: int, cells: int) -> int:
+    """Replay slots of one lane: a whole block unless over the replay budget
+    (``GpuConfig.replay_slots`` if set).  The count only sets how a block is
+    split into replay segments; the image does not depend on it."""
+    gpu = imager.gpu_config
+    if gpu.replay_slots > 0:
+        return max(1, min(interval, int(gpu.replay_slots)))
+    return max(1, min(interval, int(float(gpu.replay_budget_gb) * _GIB // (4 * max(cells, 1)))))
+
+
+class Lane:
+    """One shot's migration as propagators, buffers and a pass program.
+
+    Built from an imager and a one-shot ``PreparedBatch`` (records already
+    in HBM); ``tag`` is returned with the result."""
+
+    def __init__(self, imager: RTMImager, job: PreparedBatch, tag: Any = None) -> None:
+        import torch
+
+        if len(job.shots) != 1:
+            raise ImagingError("a lane migrates one shot")
+        if not lanes_supported(imager):
+            raise ImagingError("lanes run acoustic and elastic zero_lag/normalized migrations")
+        self.imager, self.job, self.tag = imager, job, tag
+        self.engine = imager.solver_config.engine
+        cfg = imager.config
+        shot, model, spec = job.shots[0], job.models[0], job.specs[0]
+        record = job.records[0]
+        self.condition = str(cfg.condition).lower()
+        with dev.trusted_models([model]):
+            fwd = build_propagator(model, spec, engine=self.engine,
+                                   q_options=imager.q_options(compensation=False))
+            fwd.set_source(shot.source)
+            fwd.set_receivers(shot.receivers)
+            back = build_propagator(model, spec, engine=self.engine,
+                                    q_options=imager.q_options(compensation=True))
+            back.set_source(shot.source)
+            back.set_receivers(shot.receivers)
+        if fwd._run_kwargs() or back._run_kwargs():
+            raise ImagingError("lanes do not run the viscoacoustic stabiliser")
+        names = list(record)
+        tab = dev.RecvTables()
+        tab.build_injection(back._core, shot.receivers.x, shot.receivers.z,
+                            [imager._routing(nm) for nm in names], back._k_relaxed_host())
+        tab.set_records([record[nm] for nm in names])
+        self.fwd, self.back, self.tab = fwd, back, tab
+        self.wave = np.ascontiguousarray(job.waves[0], dtype=np.float32)
+        nt = int(len(self.wave))
+        self.nt = nt
+        # the interval from the fields-only state size before the first step
+        # (ertm/imaging.py:121-124), as record_source_wavefield takes it
+        self.per_state = max(int(fwd.state_bytes()), 1)
+        interval = checkpoint_interval(nt, self.per_state, float(cfg.checkpoint_mb))
+        starts = list(range(0, nt, interval))
+        fc, bc = fwd._core, back._core
+        self.stride = int(fc.state_floats)
+        self.arena = torch.empty((len(starts), self.stride), dtype=torch.float32, device="cuda")
+        self.store = CheckpointStore(states=[(s, k) for k, s in enumerate(starts)],
+                                     interval=interval, bytes_per_state=self.per_state,
+                                     arena=self.arena, state_floats=self.stride,
+                                     forward_steps=starts[-1] + 1)
+        st = int(np.floor((float(shot.source.delay) + float(cfg.start_periods)
+                           / max(float(shot.source.f0), 1e-6)) / spec.dt))
+        self.start_step = int(np.clip(st, 0, nt))
+        rec_illum = L.ILLUM_REC if cfg.receiver_normalization else 0
+        illum = rec_illum | (L.ILLUM_SRC if self.condition == "normalized" and cfg.illumination
+                             else 0)
+        nz, nx = model.shape
+        self.slots = lane_replay_slots(imager, interval, nz * nx)
+        self.rbuf = torch.empty((self.slots, nz, nx), dtype=torch.float32, device="cuda")
+        ring = int(imager.gpu_config.image_ring)
+        self.ib = dev.ImageBuffers(nz, nx, cfg.separation, job.swave, job.rwaves, self.start_step,
+                                   illum, rbuf=self.rbuf, nslots=self.slots, ring=ring,
+                                   ring_ld=bc.ld)
+        self.fimg = dev.ImageBuffers(nz, nx, cfg.separation, job.swave, (), 0, 0, rbuf=self.rbuf,
+                                     nslots=self.slots)
+        self.subs = None
+        if self.slots < interval:
+            nsub = int(math.ceil(interval / self.slots))
+            self.subs = torch.empty((nsub, self.stride), dtype=torch.float32, device="cuda")
+        self.program = self._program()
+        self.replayed = sum(e - a for ph, a, e, *_ in self.program if ph == 1)
+        self.back_steps = sum(e - a for ph, a, e, *_ in self.program if ph == 2)
+        self.lane_steps = 0      # ticks this lane ran
+        self.shared_steps = 0    # sum over those ticks of the lanes advanced with it
+        self.started = None
+        self.finished = None
+        self.result = None
+        self._raw = None
+
+    def _program(self) -> list[tuple]:
+        """(phase, t_begin, t_end, op, img_t0, ck_interval, ck_origin, save,
+        restore) per pass, in order; save/restore are (tensor, slot) or None.
+        The same passes ``RTMImager._run_shots`` makes for this shot."""
+        st, nt, slots = self.store, self.nt, self.slots
+        prog = [(0, 0, st.forward_steps, -1, 0, st.interval, 0, (self.arena, 0), None)]
+        for blk in range(len(st.states) - 1, -1, -1):
+            s0, s1 = st.bounds(blk, nt)
+            slot = st.states[blk][1]
+            segs = [(a, min(a + slots, s1)) for a in range(s0, s1, slots)]
+            if len(segs) > 1:
+                # sub-checkpoints: the state just before w[a] of each later segment
+                prog.append((1, s0, segs[-1][0], -1, 0, slots, s0 + slots - 1, (self.subs, 0),
+                             (self.arena, slot)))
+            for k in range(len(segs) - 1, -1, -1):
+                a, e = segs[k]
+                src = (self.arena, slot) if k == 0 else (self.subs, k - 1)
+                prog.append((1, a, e, L.IMG_EXTRACT, a, 0, 0, None, src))
+                prog.append((2, a, e, L.IMG_CORRELATE, a, 0, 0, None, None))
+        return prog
+
+    def segments(self, lane_id: int, fwd_index: int, back_index: int) -> list:
+        """The program as ErtmSeg records for a batch holding the forward
+        propagator at ``fwd_index`` and the backward one at ``back_index``."""
+        out = []
+        wptr = self.wave.ctypes.data
+        for ph, a, e, op, t0, cki, cko, save, restore in self.program:
+            g = L.ErtmSeg()
+            g.lane = int(lane_id)
+            g.prop = int(back_index if ph == 2 else fwd_index)
+            g.phase, g.t_begin, g.t_end, g.op_after, g.img_t0 = ph, a, e, op, t0
+            g.ck_interval, g.ck_origin = cki, cko
+            if save is not None:
+                g.save_arena = L.ptr(save[0]) + 4 * save[1] * self.stride
+                g.save_stride = self.stride
+            if restore is not None:
+                g.restore = L.ptr(restore[0]) + 4 * restore[1] * self.stride
+            if ph != 2:
+                g.wavelet = wptr
+            out.append(g)
+        return out
+
+    @property
+    def cell_updates(self) -> float:
+        nz, nx = self.job.models[0].shape
+        return float((self.store.forward_steps + self.replayed + self.back_steps) * nz * nx)
+
+    @property
+    def mean_group(self) -> float:
+        """Lanes advanced per tick, averaged over this lane's ticks."""
+        return self.shared_steps / self.lane_steps if self.lane_steps else 1.0
+
+    def collect(self) -> None:
+        """Take the lane's accumulators once its work has completed on the
+        current stream: host copies (or, with device post-processing, the
+        finished images), after which the device buffers are released.
+        ``post`` then makes the images."""
+        import torch
+
+        im, job = self.imager, self.job
+        for p, steps in ((self.fwd, self.store.forward_steps), (self.back, self.back_steps)):
+            p.pml.touched = True
+            p.step_index += steps
+        ib = self.ib
+        use_ill = self.condition == "normalized" and bool(im.config.illumination)
+        use_rill = bool(im.config.receiver_normalization)
+        info = dict(store=self.store, start_step=self.start_step, prop=self.fwd, back=self.back,
+                    replayed_steps=self.replayed, nt=self.nt, cell_updates=self.cell_updates)
+        self._metas = [[im._meta(job.specs[0], self.engine, job.shots[0], info, self.condition, m)
+                        for m in job.modes]]
+        if im.gpu_config.device_postprocess:
+            raw = ("device", list(ib.num), ib.ill, list(ib.rill))
+            self._raw = None
+            self.result = self._post([raw])
+        else:
+            raw = ("fetch", list(ib.num), ib.ill if use_ill else None,
+                   list(ib.rill) if use_rill else None, torch.cuda.current_stream())
+            self._raw = im._fetch_host([raw])
+        self.release()
+
+    def post(self) -> dict[str, ImageResult]:
+        """The normalised, post-processed images {mode: ImageResult} (host
+        work; any thread) after ``collect``."""
+        if self._raw is not None:
+            self.result = self._post(self._raw)
+            self._raw = None
+        return self.result
+
+    def _post(self, raws) -> dict[str, ImageResult]:
+        results: list = [None]
+        self.imager._post_group(self.job, [0], raws, self._metas, self.condition, results)
+        return results[0]
+
+    def finish(self) -> dict[str, ImageResult]:
+        """``collect`` then ``post``."""
+        self.collect()
+        return self.post()
+
+    def release(self) -> None:
+        """Drop the lane's device buffers (its image has been taken)."""
+        self.arena = self.rbuf = self.subs = self.ib = self.fimg = self.tab = None
+        self.fwd = self.back = None
+        self.store.arena = None
+
+
+class LaneRunner:
+    """Lanes of one engine advanced together on the current stream.
+
+    ``add`` lanes at any time; ``run`` advances them until one finishes (or
+    ``max_ticks``) and returns the finished ones with their images."""
+
+    def __init__(self, max_lanes: int = 64, variant: tuple[int, int] | None = None) -> None:
+        if max_lanes < 1:
+            raise ValueError("max_lanes must be positive")
+        self.max_lanes = int(max_lanes)
+        self.variant = variant
+        self.lanes: list[Lane] = []
+        self._segs: dict[int, list] = {}
+        self._batch = None
+        self._curs: list[int] = []
+        self._next_id = 0
+        self.ticks = 0
+        self.lane_steps = 0
+        self.step_groups = 0
+
+    @property
+    def engine(self) -> str | None:
+        return self.lanes[0].engine if self.lanes else None
+
+    def __len__(self) -> int:
+        return len(self.lanes)
+
+    def can_take(self, engine: str) -> bool:
+        return len(self.lanes) < self.max_lanes and self.engine in (None, engine)
+
+    def add(self, lane: Lane) -> None:
+        if not self.can_take(lane.engine):
+            raise ImagingError("lane runner is full or runs another engine")
+        lane.lane_id = self._next_id
+        self._next_id += 1
+        self.lanes.append(lane)
+        self._close_batch()
+
+    def _close_batch(self) -> None:
+        if self._batch is not None:
+            for k, c in enumerate(self._batch.cores):
+                c.cur = self._curs[k]
+            self._batch.close()
+            self._batch = None
+
+    def _open_batch(self):
+        cores, recvs, imgs = [], [], []
+        for ln in self.lanes:
+            cores += [ln.fwd._core, ln.back._core]
+            recvs += [dev.RecvTables(), ln.tab]
+            imgs += [ln.fimg, ln.ib]
+        self._batch = dev.Batch(cores, recvs=recvs, imgs=imgs, variant=self.variant)
+        self._curs = [int(c.cur) for c in cores]
+        self._segs = {}
+        for k, ln in enumerate(self.lanes):
+            old = getattr(ln, "_segs", None)
+            segs = ln.segments(ln.lane_id, 2 * k, 2 * k + 1)
+            if old is not None:
+                for g, o in zip(segs, old):
+                    g.done = o.done
+            ln._segs = segs
+        flat = [g for ln in self.lanes for g in ln._segs]
+        self._arr = (L.ErtmSeg * len(flat))(*flat)
+        # the array holds copies: progress is read back from it after each call
+        self._flat_index = []
+        i = 0
+        for ln in self.lanes:
+            self._flat_index.append((ln, i, len(ln._segs)))
+            i += len(ln._segs)
+
+    def run(self, max_ticks: int = 0, post: bool = True) -> list[Lane]:
+        """Advance every lane until one finishes (or ``max_ticks`` ticks);
+        returns the finished lanes (removed from the runner) after their
+        work has completed and their accumulators are collected, images in
+        ``lane.result`` (with ``post`` False, ``lane.post()`` makes them)."""
+        import torch
+
+        if not self.lanes:
+            return []
+        if self._batch is None:
+            self._open_batch()
+        now = time.perf_counter()
+        for ln in self.lanes:
+            if ln.started is None:
+                ln.started = now
+        nlanes = len(self.lanes)
+        ticks, steps, groups = self._batch.run_lanes(self._arr, self._curs, max_ticks=max_ticks,
+                                                     stop_at_lane_end=True)
+        self.ticks += ticks
+        self.lane_steps += steps
+        self.step_groups += groups
+        done = []
+        for ln, i0, cnt in self._flat_index:
+            for j in range(cnt):
+                ln._segs[j].done = self._arr[i0 + j].done
+            # the call ends at the tick a lane finishes: every lane ran every tick
+            ln.lane_steps += ticks
+            ln.shared_steps += ticks * nlanes
+            if all(g.done >= g.t_end - g.t_begin for g in ln._segs):
+                done.append(ln)
+        if not done:
+            return []
+        # the finished lanes' results are complete once the stream is
+        torch.cuda.current_stream().synchronize()
+        self._close_batch()
+        self.lanes = [ln for ln in self.lanes if ln not in done]
+        now = time.perf_counter()
+        for ln in done:
+            ln.finished = now
+            ln.collect()
+            if post:
+                ln.post()
+        return done
+
+    def run_all(self) -> list[Lane]:
+        """Run until every lane has finished; lanes in finishing order."""
+        out = []
+        while self.lanes:
+            out += self.run()
+        return out
+
+    def close(self) -> None:
+        self._close_batch()
+        self.lanes = []
+
+
+def migrate_lanes(items: list[tuple[RTMImager, PreparedBatch]], *, max_lanes: int = 64
+                  ) -> list[dict[str, ImageResult]]:
+    """Migrate one-shot prepared jobs (each with its own imager: nt, dt,
+    grid and configs may all differ) as lanes; one runner per engine, run
+    one after the other.  Returns {mode: ImageResult} per item, in order."""
+    out: list = [None] * len(items)
+    by_engine: dict[str, list[int]] = {}
+    for i, (im, _job) in enumerate(items):
+        by_engine.setdefault(im.solver_config.engine, []).append(i)
+    for idx in by_engine.values():
+        for c0 in range(0, len(idx), max_lanes):
+            runner = LaneRunner(max_lanes=max_lanes)
+            for i in idx[c0:c0 + max_lanes]:
+                im, job = items[i]
+                runner.add(Lane(im, job, tag=i))
+            for ln in runner.run_all():
+                out[ln.tag] = ln.result
+            runner.close()
+    return out

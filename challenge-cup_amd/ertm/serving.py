@@ -10,27 +10,32 @@ runs a stream of independent migration requests on one device:
   device) and admitted only while the reserved total stays within the
   budget.  Admission is first-come first-served: the oldest waiting request
   is admitted first, and a request larger than the whole budget is rejected;
-* **continuous batching**: when a request is admitted, waiting requests that
-  can share its launches (same engine, configs, nt and checkpoint interval --
-  see ``RTMImager.migrate_batch``) and that also fit join it, up to
-  ``max_group`` shots advanced by one launch per time step;
-* **concurrency**: several launch groups run at once, each driven by its own
-  host thread on its own HIP stream; HBM freed by a finished group admits the
+* **continuous batching**: acoustic and elastic zero_lag/normalized requests
+  run as lanes (``ertm.lanes``): each worker advances up to ``max_group``
+  lanes of one engine with shared launches per time step, whatever each
+  request's nt, dt, grid, checkpoint interval or phase; an admitted request
+  joins the worker with the fewest lanes of its engine within
+  ``lane_ticks`` steps, and a finished one leaves at once.  Other requests
+  run in launch groups: waiting requests that can share the oldest one's
+  launches (same engine, configs, nt and checkpoint interval -- see
+  ``RTMImager.migrate_batch``) and that also fit join it;
+* **concurrency**: several workers run at once, each driven by its own host
+  thread on its own HIP stream; HBM freed by a finished request admits the
   next requests at once.  Wavefields, checkpoints and partial images stay on
   the device for a request's whole life;
-* **isolation**: a request that fails validation fails alone; if a launch
-  group fails while running, its members are re-run one by one, so a bad
-  request fails only itself.  A shot's image does not depend on what else
-  is running (each image equals the one ``migrate`` gives for that shot
-  alone);
+* **isolation**: a request that fails validation fails alone; if a worker's
+  shared run fails, its requests are re-run one by one, so a bad request
+  fails only itself.  A shot's image does not depend on what else is running
+  (each image equals the one ``migrate`` gives for that shot alone);
 * **stacking as shots finish**: requests that name a stack (``stack``,
   ``stack_index``) are folded into that stack's float64 sum as they finish
   (``open_stack``), in stack order through a reorder buffer, so the stacked
   image equals ``stack_images`` over the shots in stack order
   (ertm/pipeline.py:133-161).
 
-Results carry arrival, start and finish times; ``stats`` reports shots/s and
-the p50/p99 latency.
+Results carry arrival, start and finish times and how many shots advanced
+with them per step; ``stats`` reports shots/s, the p50/p99 latency, the mean
+of that group size and the lanes sharing each step launch.
 """
 
 from __future__ import annotations
@@ -40,7 +45,7 @@ import math
 import threading
 import time
 from collections import deque
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Any
 
@@ -51,6 +56,7 @@ from ._lib import EngineUnavailable
 from .config import GpuConfig, ImagingConfig, QConfig, SolverConfig
 from .exceptions import ERTMError, ImagingError, PipelineError
 from .imaging import RTMImager, checkpoint_interval
+from .lanes import Lane, LaneRunner, lanes_supported
 from .pipeline import StackAccumulator
 from .propagation import PropagationSpec
 from .types import ElasticModel, ImageResult, ShotRecord
@@ -81,9 +87,14 @@ class ShotResult:
     arrival: float
     start: float = math.nan
     finish: float = math.nan
-    group_size: int = 0
+    group_size: float = 0      # shots advanced with it per step (mean over its steps)
     hbm_bytes: int = 0
     rerun_alone: bool = False   # its launch group failed; it was re-run by itself
+    released: float = math.nan  # when its HBM reservation was returned
+
+    def __post_init__(self) -> None:
+        if math.isnan(self.released):
+            self.released = self.finish
 
     @property
     def latency(self) -> float:
@@ -142,73 +153,128 @@ def _key(req: ShotRequest, imager_key) -> tuple:
 
 
 class Admission:
-    """First-come first-served admission by bytes, with same-key companions.
+    """First-come first-served admission by bytes onto the loop's workers.
 
-    Items are (request, job, bytes, key, arrival, future) tuples; only bytes
-    and key are read here.  ``take`` returns the next launch group to start,
-    or None when the oldest waiting item does not fit or no stream is idle.
+    Items are (request, job, bytes, key, arrival, future, lane) tuples; only
+    bytes, key and lane are read here.  ``lane`` is the engine name of a
+    request that runs as a lane (``ertm.lanes``), None for one that runs in a
+    launch group.  A worker is idle, runs one launch group, or runs lanes of
+    one engine (up to ``max_group``).  ``take`` returns the next assignment
+    -- ("group", worker, [items]) with the oldest waiting item and its
+    same-key companions, or ("lane", worker, [item]) onto the worker whose
+    lanes of that engine are fewest, else an idle one -- or None when the
+    oldest waiting item does not fit in the budget or has nowhere to go.
+    Nothing overtakes the oldest waiting item.
     """
 
     def __init__(self, budget: int, streams: int, max_group: int) -> None:
         self.budget = int(budget)
-        self.idle = int(streams)
         self.max_group = int(max_group)
         self.pending: deque = deque()
         self.reserved = 0
         self.peak = 0
+        self.kind: list = [None] * int(streams)   # None | "group" | engine of its lanes
+        self.count = [0] * int(streams)
+
+    @property
+    def idle(self) -> int:
+        return sum(k is None for k in self.kind)
 
     def add(self, item) -> None:
         self.pending.append(item)
 
+    def _lane_worker(self, engine: str):
+        best = None
+        for w, k in enumerate(self.kind):
+            if k == engine and self.count[w] < self.max_group and (
+                    best is None or self.count[w] < self.count[best]):
+                best = w
+        if best is None:
+            best = next((w for w, k in enumerate(self.kind) if k is None), None)
+        return best
+
+    def _reserve(self, nbytes: int) -> None:
+        self.reserved += nbytes
+        self.peak = max(self.peak, self.reserved)
+
     def take(self):
-        if not self.pending or self.idle <= 0:
+        if not self.pending:
             return None
         head = self.pending[0]
         if self.reserved + head[2] > self.budget:
+            return None
+        lane = head[6]
+        if lane is not None:
+            w = self._lane_worker(lane)
+            if w is None:
+                return None
+            self.pending.popleft()
+            self._reserve(head[2])
+            self.kind[w] = lane
+            self.count[w] += 1
+            return ("lane", w, [head])
+        w = next((w for w, k in enumerate(self.kind) if k is None), None)
+        if w is None:
             return None
         group = [head]
         used = head[2]
         for item in list(self.pending)[1:]:
             if len(group) >= self.max_group:
                 break
-            if item[3] == head[3] and self.reserved + used + item[2] <= self.budget:
+            if item[6] is None and item[3] == head[3] and self.reserved + used + item[2] <= self.budget:
                 group.append(item)
                 used += item[2]
         taken = {id(item) for item in group}
         self.pending = deque(item for item in self.pending if id(item) not in taken)
-        self.reserved += used
-        self.peak = max(self.peak, self.reserved)
-        self.idle -= 1
-        return group
+        self._reserve(used)
+        self.kind[w] = "group"
+        self.count[w] = len(group)
+        return ("group", w, group)
 
-    def release(self, group) -> None:
-        self.reserved -= sum(item[2] for item in group)
-        self.idle += 1
+    def release(self, worker: int, items) -> None:
+        """Items assigned to ``worker`` have finished: free their bytes."""
+        self.reserved -= sum(item[2] for item in items)
+        self.count[worker] -= len(items)
+        if self.count[worker] <= 0:
+            self.kind[worker] = None
+            self.count[worker] = 0
 
 
 class ServingLoop:
     """Admit, batch and run migration requests on the current GPU.
 
-    ``submit`` returns a Future of a ``ShotResult``.  ``streams`` launch
-    groups run concurrently; ``hbm_budget_gb`` caps the HBM reserved by
-    admitted requests; ``max_group`` caps the shots per launch group.
+    ``submit`` returns a Future of a ``ShotResult``.  ``streams`` workers run
+    concurrently, each on its own stream; ``hbm_budget_gb`` caps the HBM
+    reserved by admitted requests; ``max_group`` caps the shots one worker
+    advances together.  With ``lanes`` (the default), acoustic and elastic
+    zero_lag/normalized requests run as lanes (``ertm.lanes``): a worker
+    advances all its lanes of one engine in shared launches whatever their
+    nt, dt, grid and phase, and a new request joins a running worker's lanes
+    within ``lane_ticks`` time steps.  Other requests (and all of them
+    without ``lanes``) run in launch groups of requests with one launch key.
     """
 
     def __init__(self, hbm_budget_gb: float = 200.0, streams: int = 4, max_group: int = 64,
-                 gpu_config: GpuConfig | None = None) -> None:
-        if streams < 1 or max_group < 1 or hbm_budget_gb <= 0:
-            raise ValueError("streams, max_group and hbm_budget_gb must be positive")
+                 gpu_config: GpuConfig | None = None, lanes: bool = True,
+                 lane_ticks: int = 256) -> None:
+        if streams < 1 or max_group < 1 or hbm_budget_gb <= 0 or lane_ticks < 1:
+            raise ValueError("streams, max_group, lane_ticks and hbm_budget_gb must be positive")
         self.budget = int(float(hbm_budget_gb) * _GIB)
         self.gpu_config = gpu_config or GpuConfig()
+        self.lanes = bool(lanes)
+        self.lane_ticks = int(lane_ticks)
         self._lock = threading.Lock()
         self._work = threading.Condition(self._lock)
         self._adm = Admission(self.budget, streams, max_group)
-        self._groups: deque = deque()
+        self._inbox: list[list] = [[] for _ in range(int(streams))]
         self._open = True
         self._results: list[ShotResult] = []
         self._outstanding = 0
         self._stacks: dict[Any, dict[str, Any]] = {}
-        self._threads = [threading.Thread(target=self._worker, name=f"ertm-serve-{k}",
+        self._launch_stats = [0, 0]   # lane steps, step-launch groups
+        # host post-processing of finished lanes, beside the workers' steps
+        self._post_pool = ThreadPoolExecutor(max_workers=4, thread_name_prefix="ertm-post")
+        self._threads = [threading.Thread(target=self._worker, args=(k,), name=f"ertm-serve-{k}",
                                           daemon=True) for k in range(int(streams))]
         for t in self._threads:
             t.start()
@@ -266,11 +332,12 @@ class ServingLoop:
                                          finish=time.perf_counter(), hbm_bytes=nbytes), req)
             return fut
         key = _key(req, imager._batch_key(req.model, job.specs[0]))
+        lane = req.solver.engine if self.lanes and lanes_supported(imager) else None
         with self._lock:
             if not self._open:
                 raise RuntimeError("serving loop is closed")
             self._outstanding += 1
-            self._adm.add((req, job, nbytes, key, arrival, fut))
+            self._adm.add((req, job, nbytes, key, arrival, fut, lane))
             self._admit()
         return fut
 
@@ -291,6 +358,7 @@ class ServingLoop:
             self._work.notify_all()
         for t in self._threads:
             t.join()
+        self._post_pool.shutdown(wait=True)
 
     def __enter__(self) -> "ServingLoop":
         return self
@@ -314,6 +382,10 @@ class ServingLoop:
                         "p50_s": float(np.percentile(lat, 50)),
                         "p99_s": float(np.percentile(lat, 99)),
                         "mean_group": float(np.mean([r.group_size for r in done]))})
+        steps, groups = self._launch_stats
+        if groups:
+            # lanes advanced per step launch (over every lane tick so far)
+            out["lanes_per_launch"] = steps / groups
         return out
 
     # ------------------------------------------------------------ internals
@@ -324,12 +396,14 @@ class ServingLoop:
         return RTMImager(req.imaging, req.solver, req.q, gpu)
 
     def _admit(self) -> None:
-        """Start every group that can start now (called with the lock held)."""
+        """Hand every assignment that can start now to its worker (called
+        with the lock held)."""
         while True:
-            group = self._adm.take()
-            if group is None:
+            got = self._adm.take()
+            if got is None:
                 return
-            self._groups.append(group)
+            kind, w, items = got
+            self._inbox[w].append((kind, items))
             self._work.notify_all()
 
     def _finish(self, fut: Future, res: ShotResult, req: ShotRequest | None = None) -> None:
@@ -363,27 +437,92 @@ class ServingLoop:
         else:
             fut.set_result({m: acc.result() for m, acc in st["accs"].items()})
 
-    def _worker(self) -> None:
+    def _worker(self, w: int) -> None:
         import torch
 
         stream = torch.cuda.Stream()
+        runner = LaneRunner(max_lanes=self._adm.max_group,
+                            variant=self.gpu_config.kernel_variant)
         while True:
             with self._lock:
-                while not self._groups and self._open:
+                while not self._inbox[w] and not runner.lanes and self._open:
                     self._work.wait()
-                if not self._groups:
+                if not self._inbox[w] and not runner.lanes:
                     return
-                group = self._groups.popleft()
+                inbox, self._inbox[w] = self._inbox[w], []
             with torch.cuda.stream(stream):
-                outcome = self._run(group)
-            with self._lock:
-                self._adm.release(group)
-            for item, res in zip(group, outcome):
-                self._finish(item[5], res, item[0])
-            with self._lock:
-                self._outstanding -= len(group)
-                self._admit()
-                self._work.notify_all()
+                for kind, items in inbox:
+                    if kind == "group":
+                        self._done(w, items, self._run(items))
+                    else:
+                        self._add_lane(w, runner, items[0])
+                if runner.lanes:
+                    self._run_lanes(w, runner)
+
+    def _done(self, w: int, items, outcome) -> None:
+        """Finish items assigned to worker ``w`` and admit what now fits."""
+        self._release(w, items)
+        self._complete(items, outcome)
+
+    def _release(self, w: int, items) -> None:
+        """The items' HBM is free: admit what now fits."""
+        with self._lock:
+            self._adm.release(w, items)
+            self._admit()
+            self._work.notify_all()
+
+    def _complete(self, items, outcome) -> None:
+        for item, res in zip(items, outcome):
+            self._finish(item[5], res, item[0])
+        with self._lock:
+            self._outstanding -= len(items)
+            self._work.notify_all()
+
+    def _add_lane(self, w: int, runner: "LaneRunner", item) -> None:
+        req, job = item[0], item[1]
+        try:
+            lane = Lane(self._imager(req), job, tag=item)
+        except Exception:  # noqa: BLE001 - set-up failed: run it by itself
+            self._done(w, [item], self._run([item]))
+            return
+        runner.add(lane)
+
+    def _run_lanes(self, w: int, runner: "LaneRunner") -> None:
+        """Advance the worker's lanes for up to ``lane_ticks`` steps; finished
+        lanes return their results.  If the shared run fails, each lane's
+        request is run again by itself, so a bad request fails only itself."""
+        steps0, groups0 = runner.lane_steps, runner.step_groups
+        try:
+            finished = runner.run(max_ticks=self.lane_ticks, post=False)
+        except Exception:  # noqa: BLE001 - isolate per request below
+            items = [ln.tag for ln in runner.lanes]
+            runner.close()
+            for item in items:
+                r = self._run([item])[0]
+                r.rerun_alone = True
+                self._done(w, [item], [r])
+            return
+        with self._lock:
+            self._launch_stats[0] += runner.lane_steps - steps0
+            self._launch_stats[1] += runner.step_groups - groups0
+        for ln in finished:
+            # device buffers are released: admit now, post-process beside
+            # the worker's next steps
+            ln.released = time.perf_counter()
+            self._release(w, [ln.tag])
+            self._post_pool.submit(self._post_lane, ln)
+
+    def _post_lane(self, ln: "Lane") -> None:
+        item = ln.tag
+        try:
+            images = ln.post()
+            bad = [m for m, r in images.items() if not np.isfinite(r.image).all()]
+            err = "ImagingError: non-finite image" if bad else None
+        except Exception as exc:  # noqa: BLE001 - fails this request only
+            images, err = None, f"{type(exc).__name__}: {exc}"
+        res = ShotResult(item[0].rid, None if err else images, err, item[4], ln.started,
+                         time.perf_counter(), ln.mean_group, item[2], released=ln.released)
+        self._complete([item], [res])
 
     def _run(self, group) -> list[ShotResult]:
         start = time.perf_counter()

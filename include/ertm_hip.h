@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define ERTM_ABI_VERSION 4
+#define ERTM_ABI_VERSION 5
 
 enum { ERTM_OK = 0, ERTM_EINVAL = 1, ERTM_EHIP = 2, ERTM_ENOMEM = 3, ERTM_ESTATE = 4 };
 
@@ -213,9 +213,10 @@ typedef struct ertm_batch_s* ertm_batch_t;
 /* ---- library ---------------------------------------------------------- */
 int ertm_abi_version(void);
 /* sizeof(ErtmProp), sizeof(ErtmRecv), sizeof(ErtmImage), sizeof(ErtmRun),
- * offsetof(ErtmProp, state), offsetof(ErtmRun, ctl): lets a binding check
- * its structure layout without a GPU. */
-int ertm_struct_sizes(int64_t* out6);
+ * offsetof(ErtmProp, state), offsetof(ErtmRun, ctl), sizeof(ErtmSeg),
+ * offsetof(ErtmSeg, wavelet): lets a binding check its structure layout
+ * without a GPU. */
+int ertm_struct_sizes(int64_t* out8);
 const char* ertm_last_error(void);
 int ertm_device_info(int32_t* device, int32_t* n_cu, char* name, size_t name_len);
 
@@ -328,6 +329,58 @@ typedef struct ErtmRun {
     double* ctl;               /* device, 4 doubles per propagator */
 } ErtmRun;
 int ertm_run(ertm_batch_t b, const ErtmRun* run, int32_t* cur_out, void* stream);
+
+/*
+ * Lanes: the time loops of several independent migrations, each on its own
+ * schedule, advanced in lockstep.  No counterpart in the reference, which
+ * migrates one shot per call (ertm/imaging.py:205-349); it is what lets a
+ * server batch shots whose nt, checkpoint interval, replay segments and
+ * phase all differ.
+ *
+ * A segment is one ertm_run-like pass of one propagator.  Segments with the
+ * same `lane` run one after another in list order; the lanes run side by
+ * side: each tick advances the current segment of every lane by one step,
+ * and the step launches of a tick cover every lane in the same kind of pass
+ * (forward, extracting replay, backward), so shots at different times,
+ * phases and checkpoint blocks share one launch.  A propagator belongs to
+ * one lane.  Per step a segment does what ertm_run does for its phase:
+ *   phase 0: step(amp = wavelet[t]); image(op_after, t); checkpoint
+ *   phase 1: as phase 0, after copying `restore` into the state before the
+ *            segment's first step
+ *   phase 2: inject(t); step(0); image(op_after, t), for t = t_end-1 down
+ * op_after: -1, ERTM_IMG_EXTRACT (phases 0, 1) or ERTM_IMG_CORRELATE (phase
+ * 2), with replay slot t - img_t0 of the propagator's image buffers (slots
+ * [t_begin - img_t0, t_end - img_t0) must exist).  Checkpoints (phases 0, 1):
+ * after step t with t >= ck_origin and (t - ck_origin) % ck_interval == 0 the
+ * state goes to save_arena + ((t - ck_origin) / ck_interval) * save_stride.
+ * cur (n entries, in/out) holds each propagator's state parity.  `done`
+ * (in/out) counts the steps of a segment already run, so a call can resume
+ * where an earlier one stopped.  The call returns when every segment is
+ * done, after max_ticks ticks (> 0), or, with stop_at_lane_end, after the
+ * tick in which a lane ran out of segments.  stats3 (optional): ticks run,
+ * lane steps (one per lane per tick) and step-launch groups (the launches
+ * of one pass kind in one tick; each covers up to ERTM_MAX_LAUNCH lanes per
+ * HIP launch), so lane steps / groups = lanes sharing a launch.  Each
+ * propagator's results equal those of the same passes run by ertm_run.
+ * No capture, stabiliser, sampling or two-step launches.
+ */
+typedef struct ErtmSeg {
+    int32_t lane;
+    int32_t prop;              /* propagator of the batch */
+    int32_t phase;             /* 0 forward, 1 replay, 2 backward */
+    int32_t t_begin, t_end;
+    int32_t op_after;          /* -1, ERTM_IMG_EXTRACT or ERTM_IMG_CORRELATE */
+    int32_t img_t0;            /* time of replay slot 0 */
+    int32_t ck_interval;       /* 0 = no checkpoints */
+    int32_t ck_origin;
+    int32_t done;              /* in/out: steps already run */
+    int64_t save_stride;       /* floats between saved slots */
+    float* save_arena;
+    const float* restore;      /* phase 1: state to start from */
+    const float* wavelet;      /* phases 0, 1: host amplitudes, >= t_end entries */
+} ErtmSeg;
+int ertm_run_lanes(ertm_batch_t b, ErtmSeg* segs, int32_t nseg, int32_t* cur, int32_t max_ticks,
+                   int32_t stop_at_lane_end, int64_t* stats3, void* stream);
 
 /* ---- utilities ----------------------------------------------------------- */
 /* Staggered derivative of a dense (n0, n1) array (dplus/dminus): axis 0 or

@@ -38,11 +38,12 @@ def test_struct_layouts_match_c():
     from ertm import _lib
 
     lib = _lib.load_library()
-    out = (C.c_int64 * 6)()
+    out = (C.c_int64 * 8)()
     assert lib.ertm_struct_sizes(out) == 0
     assert list(out) == [C.sizeof(_lib.ErtmProp), C.sizeof(_lib.ErtmRecv),
                          C.sizeof(_lib.ErtmImage), C.sizeof(_lib.ErtmRun),
-                         _lib.ErtmProp.state.offset, _lib.ErtmRun.ctl.offset]
+                         _lib.ErtmProp.state.offset, _lib.ErtmRun.ctl.offset,
+                         C.sizeof(_lib.ErtmSeg), _lib.ErtmSeg.wavelet.offset]
 
 
 def test_abi_version_and_error_path_without_gpu():

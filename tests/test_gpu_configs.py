@@ -172,6 +172,8 @@ def test_serving_workload_at_its_sizes_alone_equals_mix_and_oracle():
         stats = loop.stats()
     _say("stats", stats)
     assert all(r.ok for r in res), [r.error for r in res if not r.ok]
+    # requests of different nt, dt and grids shared launches (as lanes)
+    assert stats["mean_group"] > 1.0 and stats["lanes_per_launch"] > 1.0, stats
     # arrivals overlapped: some requests started before others had finished
     assert any(a.start < b.finish and b.start < a.finish
                for i, a in enumerate(res) for b in res[i + 1:])

@@ -33,8 +33,9 @@ def _alone(req):
     return RTMImager(req.imaging, req.solver, req.q).migrate(req.shot, req.model)
 
 
-def test_results_identical_alone_and_in_the_mix():
-    _say("serving mix vs alone")
+@pytest.mark.parametrize("lanes", [True, False])
+def test_results_identical_alone_and_in_the_mix(lanes):
+    _say("serving mix vs alone, lanes", lanes)
     from ertm.serving import ServingLoop
 
     reqs = [make_request(100 + k, rid=k) for k in range(5)]
@@ -44,7 +45,7 @@ def test_results_identical_alone_and_in_the_mix():
         src = dataclasses.replace(base.shot.source, x=base.shot.source.x + 20.0 * k)
         reqs.append(dataclasses.replace(base, shot=dataclasses.replace(base.shot, source=src),
                                         rid=f"s{k}"))
-    with ServingLoop(hbm_budget_gb=16.0, streams=3) as loop:
+    with ServingLoop(hbm_budget_gb=16.0, streams=3, lanes=lanes) as loop:
         futs = [loop.submit(r) for r in reqs]
         res = [f.result(timeout=600) for f in futs]
         stats = loop.stats()
@@ -58,8 +59,9 @@ def test_results_identical_alone_and_in_the_mix():
     assert stats["completed"] == len(reqs) and stats["p99_s"] >= stats["p50_s"] > 0
 
 
-def test_a_failing_request_fails_alone():
-    _say("serving isolation")
+@pytest.mark.parametrize("lanes", [True, False])
+def test_a_failing_request_fails_alone(lanes):
+    _say("serving isolation, lanes", lanes)
     from ertm.serving import ServingLoop
 
     good = make_request(11, engine="elastic", nz=48, nx=64, nt=420, rid="good")
@@ -72,12 +74,17 @@ def test_a_failing_request_fails_alone():
     invalid = dataclasses.replace(good, rid="invalid",
                                   solver=dataclasses.replace(good.solver, dt=1.0))
     blocker = make_request(12, nz=64, nx=96, nt=460, rid="blocker")
-    with ServingLoop(hbm_budget_gb=8.0, streams=1) as loop:
+    with ServingLoop(hbm_budget_gb=8.0, streams=1, lanes=lanes) as loop:
         # one stream: "good" and "bad" wait behind "blocker" and start together
+        # (as a launch group, or as lanes of one worker)
         futs = {r.rid: loop.submit(r) for r in (blocker, good, bad, invalid)}
         res = {k: f.result(timeout=600) for k, f in futs.items()}
     assert res["blocker"].ok and res["good"].ok
-    assert res["bad"].rerun_alone and res["good"].rerun_alone  # their group broke
+    if not lanes:
+        assert res["bad"].rerun_alone and res["good"].rerun_alone  # their group broke
+    else:
+        # lanes keep their own buffers: the bad one fails, the others run on
+        assert res["good"].group_size > 1 and not res["good"].rerun_alone
     assert not res["bad"].ok and not res["invalid"].ok
     assert res["invalid"].error.startswith("rejected")
     assert np.array_equal(res["good"].images["pp"].image, _alone(good).image)
@@ -99,10 +106,10 @@ def test_admission_keeps_reserved_hbm_within_budget():
     assert all(r.ok for r in res)
     assert not rej.ok and "HBM" in rej.error
     assert stats["peak_reserved_gib"] <= budget_gb + 1e-9
-    # at every moment the requests running together fit the budget
+    # at every moment the requests holding HBM together fit the budget
     for r in res:
         t = r.start
-        live = sum(q.hbm_bytes for q in res if q.start <= t < q.finish)
+        live = sum(q.hbm_bytes for q in res if q.start <= t < q.released)
         assert live <= budget_gb * 1024 ** 3
 
 

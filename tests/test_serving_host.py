@@ -10,8 +10,12 @@ from ertm.serving import Admission, ShotRequest, request_bytes
 from ertm.types import ElasticModel, GridSpec, ReceiverSpec, ShotRecord, SourceSpec
 
 
-def _item(name, nbytes, key):
-    return (name, None, nbytes, key, 0.0, None)
+def _item(name, nbytes, key, lane=None):
+    return (name, None, nbytes, key, 0.0, None, lane)
+
+
+def _names(got):
+    return [i[0] for i in got[2]]
 
 
 def test_admission_is_first_come_first_served_within_budget():
@@ -20,14 +24,14 @@ def test_admission_is_first_come_first_served_within_budget():
         adm.add(_item(name, nb, key))
     g1 = adm.take()
     # "a" starts with its same-key companion "c" (60 + 30 <= 100)
-    assert [i[0] for i in g1] == ["a", "c"]
+    assert g1[0] == "group" and _names(g1) == ["a", "c"]
     # "b" (50) does not fit next to 90 reserved; "d" must not overtake it
     assert adm.take() is None
-    adm.release(g1)
+    adm.release(g1[1], g1[2])
     g2 = adm.take()
-    assert [i[0] for i in g2] == ["b"]
+    assert _names(g2) == ["b"]
     g3 = adm.take()
-    assert [i[0] for i in g3] == ["d"]
+    assert _names(g3) == ["d"]
     assert adm.peak <= 100
 
 
@@ -36,10 +40,31 @@ def test_admission_respects_streams_and_group_cap():
     for k in range(5):
         adm.add(_item(k, 1, "same"))
     g1, g2 = adm.take(), adm.take()
-    assert [i[0] for i in g1] == [0, 1] and [i[0] for i in g2] == [2, 3]
+    assert _names(g1) == [0, 1] and _names(g2) == [2, 3] and g1[1] != g2[1]
     assert adm.take() is None          # no idle stream
-    adm.release(g1)
-    assert [i[0] for i in adm.take()] == [4]
+    adm.release(g1[1], g1[2])
+    assert _names(adm.take()) == [4]
+
+
+def test_lanes_go_to_the_worker_with_fewest_lanes_of_their_engine():
+    adm = Admission(budget=100, streams=3, max_group=2)
+    for name, nb, lane in [("e0", 10, "elastic"), ("a0", 10, "acoustic"), ("e1", 10, "elastic"),
+                           ("e2", 10, "elastic"), ("g", 10, None), ("e3", 10, "elastic")]:
+        adm.add(_item(name, nb, "k" + name, lane))
+    got = [adm.take() for _ in range(4)]
+    assert [(g[0], g[1], _names(g)) for g in got] == [
+        ("lane", 0, ["e0"]), ("lane", 1, ["a0"]), ("lane", 0, ["e1"]), ("lane", 2, ["e2"])]
+    # every worker busy: the group request waits, and the lane behind it too
+    assert adm.take() is None and adm.kind == ["elastic", "acoustic", "elastic"]
+    adm.release(1, got[1][2])          # the acoustic worker is idle again
+    g = adm.take()
+    assert g[0] == "group" and g[1] == 1 and _names(g) == ["g"]
+    # e3 joins the elastic worker with fewer lanes (worker 0 is full)
+    e3 = adm.take()
+    assert (e3[0], e3[1], _names(e3)) == ("lane", 2, ["e3"]) and adm.count == [2, 1, 2]
+    adm.release(2, got[3][2])
+    adm.release(2, e3[2])
+    assert adm.kind[2] is None and adm.reserved == 30 and adm.peak <= 100
 
 
 def _request(nz, nx, nt, engine="elastic", checkpoint_mb=256.0):

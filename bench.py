@@ -194,14 +194,14 @@ def serving_leg(n, scale, load, gcells_per_s, seed=2024):
             "errors": sorted({r.error for r in res if not r.ok})[:3]}
 
 
-def _serve(reqs, arrivals, lanes, t0=None):
+def _serve(reqs, arrivals, lanes, streams=4):
     """Submit ``reqs`` at ``arrivals`` (seconds after the start; None = all at
     once) to one ServingLoop; returns its stats plus the makespan."""
     from ertm.config import GpuConfig
     from ertm.serving import ServingLoop
 
-    with ServingLoop(hbm_budget_gb=200.0, streams=4, gpu_config=GpuConfig(replay_budget_gb=4.0),
-                     lanes=lanes) as loop:
+    with ServingLoop(hbm_budget_gb=200.0, streams=streams,
+                     gpu_config=GpuConfig(replay_budget_gb=4.0), lanes=lanes) as loop:
         t0 = time.perf_counter()
         futs = []
         for k, req in enumerate(reqs):
@@ -216,7 +216,7 @@ def _serve(reqs, arrivals, lanes, t0=None):
     return stats
 
 
-def serving_sweep(n, scale, loads, lanes=True, seed=2024):
+def serving_sweep(n, scale, loads, lanes=True, streams=4, seed=2024):
     """The mixed request stream's capacity and latency under load: first a
     closed loop (all ``n`` requests submitted at once; capacity = n / the
     makespan), then open-loop Poisson arrivals at each ``load`` x that
@@ -227,7 +227,8 @@ def serving_sweep(n, scale, loads, lanes=True, seed=2024):
     mode = "lanes" if lanes else "groups"
 
     def line(kind, load, rate, st):
-        out = {"metric": "serving_sweep", "mode": mode, "kind": kind, "load": load,
+        out = {"metric": "serving_sweep", "mode": mode, "streams": streams, "kind": kind,
+               "load": load,
                "requests": n, "scale": scale, "offered_per_s": None if rate is None else
                round(rate, 4), "mean_cell_updates": mean_cells,
                "completed": st["completed"], "failed": st["failed"],
@@ -241,13 +242,13 @@ def serving_sweep(n, scale, loads, lanes=True, seed=2024):
                "peak_reserved_gib": st.get("peak_reserved_gib"), "errors": st["errors"]}
         print(json.dumps(out), flush=True)
 
-    st = _serve(reqs, None, lanes)
+    st = _serve(reqs, None, lanes, streams)
     capacity = st["completed"] / st["makespan_s"]
     line("closed_loop", None, None, st)
     for load in loads:
         rate = load * capacity
         gaps = np.random.default_rng([seed, int(load * 1000)]).exponential(1.0 / rate, n)
-        line("open_loop", load, rate, _serve(reqs, np.cumsum(gaps), lanes))
+        line("open_loop", load, rate, _serve(reqs, np.cumsum(gaps), lanes, streams))
 
 
 def c5_model(seed=5):
@@ -551,6 +552,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-loads", default="0.25,0.5,1,2,4")
     ap.add_argument("--serve-groups", action="store_true",
                     help="serving: launch groups of one launch key only (no lanes)")
+    ap.add_argument("--serve-streams", type=int, default=4,
+                    help="--serve-sweep: serving workers (one stream each)")
     ap.add_argument("--c5-sweep", action="store_true",
                     help="instead of the batched benchmark: the large single elastic shot "
                          "(2000 x 6000, nt 10000), excitation vs zero_lag at checkpoint "
@@ -620,7 +623,7 @@ def main(argv=None) -> int:
     if args.serve_sweep:
         serving_sweep(args.serve, args.serve_scale,
                       [float(v) for v in args.serve_loads.split(",")],
-                      lanes=not args.serve_groups)
+                      lanes=not args.serve_groups, streams=args.serve_streams)
         return 0
 
     if args.copy_probe_mib > 0:

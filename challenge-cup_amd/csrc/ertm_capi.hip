@@ -68,10 +68,40 @@ size_t pool_class(size_t bytes) {
     return c;
 }
 
+// Blocks released while launches that read them may still be queued: each
+// returns to the pool once the events recorded after those launches (one per
+// stream that used it) have completed.
+struct Deferred {
+    void* p;
+    size_t cls;
+    std::vector<hipEvent_t> ev;
+};
+std::vector<Deferred> g_deferred;
+
+// Move deferred blocks whose events have completed into the pool (lock
+// held).  The blocks still waiting move to a new list: compacting in place
+// would move an entry onto itself, and a self-moved std::vector is left
+// empty, which would free its block with its events forgotten.
+void pool_reap() {
+    std::vector<Deferred> waiting;
+    for (Deferred& d : g_deferred) {
+        bool done = true;
+        for (hipEvent_t e : d.ev) done = done && hipEventQuery(e) == hipSuccess;
+        if (done) {
+            for (hipEvent_t e : d.ev) (void)hipEventDestroy(e);
+            g_pool.emplace(d.cls, d.p);
+        } else {
+            waiting.push_back(std::move(d));
+        }
+    }
+    g_deferred.swap(waiting);
+}
+
 void* pool_get(size_t bytes) {
     const size_t c = pool_class(bytes);
     {
         std::lock_guard<std::mutex> lk(g_pool_m);
+        pool_reap();
         auto it = g_pool.find(c);
         if (it != g_pool.end()) {
             void* p = it->second;
@@ -88,6 +118,28 @@ void pool_put(void* p, size_t bytes) {
     if (!p) return;
     std::lock_guard<std::mutex> lk(g_pool_m);
     g_pool.emplace(pool_class(bytes), p);
+}
+
+// pool_put once the work now queued on `streams` has run.  Returns false
+// (and releases nothing) if an event cannot be recorded.
+bool pool_put_after(void* p, size_t bytes, const std::vector<hipStream_t>& streams) {
+    if (!p) return true;
+    Deferred d{p, pool_class(bytes), {}};
+    for (hipStream_t s : streams) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            for (hipEvent_t x : d.ev) (void)hipEventDestroy(x);
+            return false;
+        }
+        d.ev.push_back(e);
+        if (hipEventRecord(e, s) != hipSuccess) {
+            for (hipEvent_t x : d.ev) (void)hipEventDestroy(x);
+            return false;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_pool_m);
+    g_deferred.push_back(std::move(d));
+    return true;
 }
 
 }  // namespace
@@ -319,7 +371,24 @@ int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img, void* stream) 
 
 int ertm_batch_destroy(ertm_batch_t b) {
     if (!b) return ERTM_OK;
-    for (hipStream_t t : b->streams) (void)hipStreamSynchronize(t);
+    // A batch with captured graphs, FFT plans or a capture stream waits for
+    // its streams; otherwise its device blocks (descriptor tables, clock,
+    // tables, scratch) return to the pool once the launches already queued
+    // on its streams have run, and the caller's streams go on undisturbed.
+    bool wait = !b->graphs.empty() || !b->fft.empty() || b->cap;
+    if (!wait) {
+        void** blocks[] = {&b->block, (void**)&b->dclk, (void**)&b->damp, (void**)&b->dark,
+                           &b->psi1};
+        const size_t sizes[] = {b->block_bytes, 64, b->damp_bytes, sizeof(float*) * b->props.size(),
+                                b->psi1_bytes};
+        for (int i = 0; i < 5 && !wait; ++i) {
+            if (!*blocks[i]) continue;
+            if (pool_put_after(*blocks[i], sizes[i], b->streams)) *blocks[i] = nullptr;
+            else wait = true;
+        }
+    }
+    if (wait)
+        for (hipStream_t t : b->streams) (void)hipStreamSynchronize(t);
     delete b;
     return ERTM_OK;
 }
@@ -687,9 +756,12 @@ static int step_kind(bool fused, int kind) {
 
 // One step; with img_op >= 0 and *imaged set on return, the step launches
 // also did that imaging op (t = time index per propagator).
+// shape: launch shape (variant, zc) instead of the batch's, or nullptr.
 static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
                    int img_op = -1, const int32_t* t = nullptr, bool* imaged = nullptr,
-                   const DevClock* dc = nullptr, const Members* m = nullptr) {
+                   const DevClock* dc = nullptr, const Members* m = nullptr,
+                   const int* shape = nullptr) {
+    const int variant = shape ? shape[0] : b->variant, zc = shape ? shape[1] : b->zc;
     const int n = count_of(b, m);
     const bool fuse_img = img_op >= 0 && t &&
                           (img_op == ERTM_IMG_RING_STORE ? ring_ok(b, m) : fused_image_ok(b, img_op, m));
@@ -704,8 +776,8 @@ static int do_step(ertm_batch_t b, int cur, const float* amp, hipStream_t s,
         const int kind = fuse_img ? img_kind(b, img_op, la) : 0;
         Timed tm(s, dc != nullptr);
         if (tm.status() != ERTM_OK) return tm.status();
-        bool fused = b->variant != 0 &&
-                     launch_step_fused(b->dprops, b->props.data(), la, b->order, b->variant, b->zc,
+        bool fused = variant != 0 &&
+                     launch_step_fused(b->dprops, b->props.data(), la, b->order, variant, zc,
                                        b->dimg, kind, s);
         if (!fused) {
             if (fuse_img) return fail(ERTM_ESTATE, "step: fused imaging launch refused");
@@ -1325,6 +1397,61 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
 
 // Lanes (ertm_run_lanes): every lane's current segment advances one step per
 // tick; per tick the launches group the lanes by pass kind.
+
+// Compute units of the current device (cached per device).
+static int cu_count() {
+    static std::mutex m;
+    static std::map<int, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lk(m);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+        n = 256;
+    cache[dev] = n;
+    return n;
+}
+
+// Launch shape of one lane launch group.  A fused block streams the rows of
+// its chunk one after another, so a launch of few blocks lasts as long as
+// one chunk's chain of row loads: lanes take the longest chunks that still
+// give 16 blocks per CU (the batch tuner's rule), over the blocks their
+// grids really have; 8-row chunks while they give 4 blocks per CU; below
+// that the two-launch kernels (variant 0), whose 64 x 4 tiles have no chain
+// (a 208 x 529 elastic step: 38 us fused with 8-row chunks, 30 us as two
+// launches and an imaging pass).  Every shape gives bit-identical results.
+static void lane_shape(ertm_batch_t b, const Members& m, int* shape) {
+    shape[0] = b->variant;
+    shape[1] = b->zc;
+    if (b->variant == 0) return;
+    auto blocks = [&](int threads, int zc) {
+        const int tx = threads - 16;
+        int64_t nb = 0;
+        for (int i = 0; i < m.n; ++i) {
+            const ErtmProp& P = b->props[m.idx[i]];
+            nb += (int64_t)((P.nxp + tx - 1) / tx) * ((P.nzp + zc - 1) / zc);
+        }
+        return nb;
+    };
+    const int64_t target = 16 * (int64_t)cu_count();
+    if (blocks(192, 48) >= target) {
+        shape[0] = 4, shape[1] = 48;
+        return;
+    }
+    for (int zc : {48, 40, 32, 24, 16, 12}) {
+        if (blocks(128, zc) >= target) {
+            shape[0] = 2, shape[1] = zc;
+            return;
+        }
+    }
+    if (blocks(128, 8) >= target / 4) {
+        shape[0] = 2, shape[1] = 8;
+        return;
+    }
+    shape[0] = 0;
+}
 static int check_seg(ertm_batch_t b, const ErtmSeg& g, int i) {
     const int n = (int)b->props.size();
     if (g.prop < 0 || g.prop >= n) return fail(ERTM_EINVAL, "lanes: segment %d: bad propagator %d", i, g.prop);
@@ -1385,8 +1512,8 @@ int ertm_run_lanes(ertm_batch_t b, ErtmSeg* segs, int32_t nseg, int32_t* cur, in
             std::find(ring_members.begin(), ring_members.end(), g.prop) == ring_members.end())
             ring_members.push_back(g.prop);
     }
-    // one decision for every backward lane, so a propagator's accumulation
-    // order never switches between the ring and direct correlation
+    // the ring is used when every backward lane has one, in the ticks whose
+    // backward launch is fused
     const Members all_back{ring_members.data(), (int)ring_members.size(), cur, nullptr};
     const bool use_ring = !ring_members.empty() && ring_ok(b, &all_back);
     const int ring_len = use_ring ? b->img[ring_members[0]].ring_len : 0;
@@ -1454,31 +1581,51 @@ int ertm_run_lanes(ertm_batch_t b, ErtmSeg* segs, int32_t nseg, int32_t* cur, in
             if ((rc = do_inject(b, 0, tt.data(), s, nullptr, &m)) != ERTM_OK) return rc;
         }
         bool ext_done = false, back_done = false;
+        int shape[2];
         if (!fplain.empty()) {
             const Members m = mem(fplain);
-            if ((rc = do_step(b, 0, amp.data(), s, -1, nullptr, nullptr, nullptr, &m)) != ERTM_OK) return rc;
+            lane_shape(b, m, shape);
+            if ((rc = do_step(b, 0, amp.data(), s, -1, nullptr, nullptr, nullptr, &m, shape)) != ERTM_OK)
+                return rc;
         }
+        // extracting replays: fused extraction, or a plain step and k_image
         if (!fext.empty()) {
             const Members m = mem(fext);
-            if ((rc = do_step(b, 0, amp.data(), s, ERTM_IMG_EXTRACT, tt.data(), &ext_done, nullptr, &m)) !=
-                ERTM_OK)
+            lane_shape(b, m, shape);
+            if ((rc = do_step(b, 0, amp.data(), s, shape[0] ? ERTM_IMG_EXTRACT : -1, tt.data(),
+                              &ext_done, nullptr, &m, shape)) != ERTM_OK)
                 return rc;
         }
+        // backward passes: ring stores (folded later by k_image_flush), fused
+        // correlation, or a plain step and k_image.  Per cell the ring folds
+        // steps in the order direct correlation adds them, so a propagator may
+        // switch between the two after folding its pending ring steps.
+        bool ring_tick = false;
         if (!bimg.empty()) {
             const Members m = mem(bimg);
-            const int op = use_ring ? ERTM_IMG_RING_STORE : ERTM_IMG_CORRELATE;
-            if ((rc = do_step(b, 0, amp.data(), s, op, tt.data(), &back_done, nullptr, &m)) != ERTM_OK)
+            lane_shape(b, m, shape);
+            ring_tick = use_ring && shape[0] != 0;
+            if (use_ring && !ring_tick) {
+                for (int p : bimg)
+                    if (pend[p] > 0) due.push_back(p);
+                if (!due.empty() && (rc = flush(due)) != ERTM_OK) return rc;
+            }
+            const int op = ring_tick ? ERTM_IMG_RING_STORE : (shape[0] ? ERTM_IMG_CORRELATE : -1);
+            if ((rc = do_step(b, 0, amp.data(), s, op, tt.data(), &back_done, nullptr, &m, shape)) !=
+                ERTM_OK)
                 return rc;
-            if (use_ring && !back_done) return fail(ERTM_ESTATE, "lanes: ring imaging launch refused");
+            if (ring_tick && !back_done) return fail(ERTM_ESTATE, "lanes: ring imaging launch refused");
         }
         if (!bplain.empty()) {
             const Members m = mem(bplain);
-            if ((rc = do_step(b, 0, amp.data(), s, -1, nullptr, nullptr, nullptr, &m)) != ERTM_OK) return rc;
+            lane_shape(b, m, shape);
+            if ((rc = do_step(b, 0, amp.data(), s, -1, nullptr, nullptr, nullptr, &m, shape)) != ERTM_OK)
+                return rc;
         }
         for (int i : act) cur[segs[i].prop] ^= 1;
         lane_steps += (int64_t)act.size();
         for (auto* v : {&fplain, &fext, &bimg, &bplain}) step_groups += v->empty() ? 0 : 1;
-        if (use_ring) {
+        if (ring_tick) {
             for (int i : act) {
                 const ErtmSeg& g = segs[i];
                 if (g.phase != 2 || g.op_after < 0) continue;

@@ -136,7 +136,7 @@ class Lane:
         self.started = None
         self.finished = None
         self.result = None
-        self._raw = None
+        self._event = None
 
     def _program(self) -> list[tuple]:
         """(phase, t_begin, t_end, op, img_t0, ck_interval, ck_origin, save,
@@ -190,40 +190,53 @@ class Lane:
         """Lanes advanced per tick, averaged over this lane's ticks."""
         return self.shared_steps / self.lane_steps if self.lane_steps else 1.0
 
-    def collect(self) -> None:
-        """Take the lane's accumulators once its work has completed on the
-        current stream: host copies (or, with device post-processing, the
-        finished images), after which the device buffers are released.
-        ``post`` then makes the images."""
+    def collect(self, done_event=None) -> None:
+        """Record that the lane's passes are all enqueued on the current
+        stream (``done_event`` recorded after them, or a new one): the
+        metadata is taken now; ``post`` waits for the event, copies the
+        accumulators to the host, releases the device buffers and makes the
+        images.  With device post-processing the images are made here, on
+        the current stream."""
         import torch
 
         im, job = self.imager, self.job
         for p, steps in ((self.fwd, self.store.forward_steps), (self.back, self.back_steps)):
             p.pml.touched = True
             p.step_index += steps
-        ib = self.ib
-        use_ill = self.condition == "normalized" and bool(im.config.illumination)
-        use_rill = bool(im.config.receiver_normalization)
         info = dict(store=self.store, start_step=self.start_step, prop=self.fwd, back=self.back,
                     replayed_steps=self.replayed, nt=self.nt, cell_updates=self.cell_updates)
         self._metas = [[im._meta(job.specs[0], self.engine, job.shots[0], info, self.condition, m)
                         for m in job.modes]]
         if im.gpu_config.device_postprocess:
-            raw = ("device", list(ib.num), ib.ill, list(ib.rill))
-            self._raw = None
-            self.result = self._post([raw])
-        else:
-            raw = ("fetch", list(ib.num), ib.ill if use_ill else None,
-                   list(ib.rill) if use_rill else None, torch.cuda.current_stream())
-            self._raw = im._fetch_host([raw])
-        self.release()
+            ib = self.ib
+            self.result = self._post([("device", list(ib.num), ib.ill, list(ib.rill))])
+            torch.cuda.current_stream().synchronize()
+            self.release()
+            self._event = None
+            return
+        if done_event is None:
+            done_event = torch.cuda.Event()
+            done_event.record()
+        self._event = done_event
 
     def post(self) -> dict[str, ImageResult]:
-        """The normalised, post-processed images {mode: ImageResult} (host
-        work; any thread) after ``collect``."""
-        if self._raw is not None:
-            self.result = self._post(self._raw)
-            self._raw = None
+        """The normalised, post-processed images {mode: ImageResult} after
+        ``collect``; any thread.  The device buffers are released here."""
+        if self.result is not None:
+            return self.result
+        import torch
+
+        im, ib = self.imager, self.ib
+        self._event.synchronize()
+        use_ill = self.condition == "normalized" and bool(im.config.illumination)
+        use_rill = bool(im.config.receiver_normalization)
+        # the copies run on this thread's side stream: the lane's work is done
+        raw = ("fetch", list(ib.num), ib.ill if use_ill else None,
+               list(ib.rill) if use_rill else None, torch.cuda.default_stream())
+        raws = im._fetch_host([raw])
+        self.release()
+        self._event = None
+        self.result = self._post(raws)
         return self.result
 
     def _post(self, raws) -> dict[str, ImageResult]:
@@ -282,6 +295,8 @@ class LaneRunner:
         self._close_batch()
 
     def _close_batch(self) -> None:
+        """Close the current batch (its device tables return to the library's
+        pool once the launches already queued have run: no wait here)."""
         if self._batch is not None:
             for k, c in enumerate(self._batch.cores):
                 c.cur = self._curs[k]
@@ -294,7 +309,15 @@ class LaneRunner:
             cores += [ln.fwd._core, ln.back._core]
             recvs += [dev.RecvTables(), ln.tab]
             imgs += [ln.fimg, ln.ib]
-        self._batch = dev.Batch(cores, recvs=recvs, imgs=imgs, variant=self.variant)
+        variant = self.variant
+        if variant is None:
+            from . import tuning
+
+            c0 = cores[0]
+            # a fused shape: ertm_run_lanes sizes each launch's chunks itself
+            if tuning.fused_supported(c0.engine, c0.order, c0.nmech):
+                variant = (2, 48)
+        self._batch = dev.Batch(cores, recvs=recvs, imgs=imgs, variant=variant)
         self._curs = [int(c.cur) for c in cores]
         self._segs = {}
         for k, ln in enumerate(self.lanes):
@@ -315,9 +338,10 @@ class LaneRunner:
 
     def run(self, max_ticks: int = 0, post: bool = True) -> list[Lane]:
         """Advance every lane until one finishes (or ``max_ticks`` ticks);
-        returns the finished lanes (removed from the runner) after their
-        work has completed and their accumulators are collected, images in
-        ``lane.result`` (with ``post`` False, ``lane.post()`` makes them)."""
+        returns the finished lanes (removed from the runner), images in
+        ``lane.result``.  With ``post`` False the call does not wait for
+        them: ``lane.post()`` (any thread) waits, takes the images and
+        releases the lane's device memory, while the other lanes go on."""
         import torch
 
         if not self.lanes:
@@ -345,14 +369,14 @@ class LaneRunner:
                 done.append(ln)
         if not done:
             return []
-        # the finished lanes' results are complete once the stream is
-        torch.cuda.current_stream().synchronize()
+        ev = torch.cuda.Event()
+        ev.record()
         self._close_batch()
         self.lanes = [ln for ln in self.lanes if ln not in done]
         now = time.perf_counter()
         for ln in done:
             ln.finished = now
-            ln.collect()
+            ln.collect(ev)
             if post:
                 ln.post()
         return done

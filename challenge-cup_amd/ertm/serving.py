@@ -14,8 +14,9 @@ runs a stream of independent migration requests on one device:
   run as lanes (``ertm.lanes``): each worker advances up to ``max_group``
   lanes of one engine with shared launches per time step, whatever each
   request's nt, dt, grid, checkpoint interval or phase; an admitted request
-  joins the worker with the fewest lanes of its engine within
-  ``lane_ticks`` steps, and a finished one leaves at once.  Other requests
+  starts on an idle worker or, when none is idle, joins the worker with the
+  fewest lanes of its engine within ``lane_ticks`` steps, and a finished one
+  leaves at once.  Other requests
   run in launch groups: waiting requests that can share the oldest one's
   launches (same engine, configs, nt and checkpoint interval -- see
   ``RTMImager.migrate_batch``) and that also fit join it;
@@ -161,8 +162,8 @@ class Admission:
     launch group.  A worker is idle, runs one launch group, or runs lanes of
     one engine (up to ``max_group``).  ``take`` returns the next assignment
     -- ("group", worker, [items]) with the oldest waiting item and its
-    same-key companions, or ("lane", worker, [item]) onto the worker whose
-    lanes of that engine are fewest, else an idle one -- or None when the
+    same-key companions, or ("lane", worker, [item]) onto an idle worker,
+    else the worker whose lanes of that engine are fewest -- or None when the
     oldest waiting item does not fit in the budget or has nowhere to go.
     Nothing overtakes the oldest waiting item.
     """
@@ -184,13 +185,15 @@ class Admission:
         self.pending.append(item)
 
     def _lane_worker(self, engine: str):
-        best = None
+        # an idle worker first (its own stream: runs beside the others), else
+        # the worker with the fewest lanes of this engine
+        best = next((w for w, k in enumerate(self.kind) if k is None), None)
+        if best is not None:
+            return best
         for w, k in enumerate(self.kind):
             if k == engine and self.count[w] < self.max_group and (
                     best is None or self.count[w] < self.count[best]):
                 best = w
-        if best is None:
-            best = next((w for w, k in enumerate(self.kind) if k is None), None)
         return best
 
     def _reserve(self, nbytes: int) -> None:
@@ -506,13 +509,11 @@ class ServingLoop:
             self._launch_stats[0] += runner.lane_steps - steps0
             self._launch_stats[1] += runner.step_groups - groups0
         for ln in finished:
-            # device buffers are released: admit now, post-process beside
-            # the worker's next steps
-            ln.released = time.perf_counter()
-            self._release(w, [ln.tag])
-            self._post_pool.submit(self._post_lane, ln)
+            # the worker goes on with its other lanes; a post thread waits
+            # for this one, frees its HBM and makes its images
+            self._post_pool.submit(self._post_lane, w, ln)
 
-    def _post_lane(self, ln: "Lane") -> None:
+    def _post_lane(self, w: int, ln: "Lane") -> None:
         item = ln.tag
         try:
             images = ln.post()
@@ -520,8 +521,11 @@ class ServingLoop:
             err = "ImagingError: non-finite image" if bad else None
         except Exception as exc:  # noqa: BLE001 - fails this request only
             images, err = None, f"{type(exc).__name__}: {exc}"
+            ln.release()
+        released = time.perf_counter()
+        self._release(w, [item])
         res = ShotResult(item[0].rid, None if err else images, err, item[4], ln.started,
-                         time.perf_counter(), ln.mean_group, item[2], released=ln.released)
+                         time.perf_counter(), ln.mean_group, item[2], released=released)
         self._complete([item], [res])
 
     def _run(self, group) -> list[ShotResult]:

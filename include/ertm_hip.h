@@ -228,7 +228,11 @@ int ertm_batch_create(const ErtmProp* props, const ErtmRecv* recv, const ErtmIma
 /* Replace the imaging descriptors.  Waits for the work already enqueued with
  * this batch (on any stream), then uploads on `stream` and waits for it. */
 int ertm_batch_update_image(ertm_batch_t b, const ErtmImage* img, void* stream);
-/* Waits for the work enqueued with this batch, then releases it. */
+/* Releases the batch.  Its device tables return to the library's pool only
+ * after the work already enqueued with it has run (events on its streams);
+ * the call waits for that work only when the batch holds captured graphs or
+ * FFT plans.  The caller's buffers named in the descriptors stay the
+ * caller's to keep alive until that work has run. */
 int ertm_batch_destroy(ertm_batch_t b);
 /* Stencil kernel variant: 0 = two launches per step (velocity, then
  * pressure/stress; every engine and order); 1, 2, 3, 4 = single fused launch
@@ -362,7 +366,11 @@ int ertm_run(ertm_batch_t b, const ErtmRun* run, int32_t* cur_out, void* stream)
  * of one pass kind in one tick; each covers up to ERTM_MAX_LAUNCH lanes per
  * HIP launch), so lane steps / groups = lanes sharing a launch.  Each
  * propagator's results equal those of the same passes run by ertm_run.
- * No capture, stabiliser, sampling or two-step launches.
+ * With a fused variant set on the batch, each launch picks its own chunk
+ * length from the blocks its lanes' grids give (long chunks when the chip
+ * is full, short ones when few blocks would leave each launch waiting on one
+ * chunk's chain of rows).  No capture, stabiliser, sampling or two-step
+ * launches.
  */
 typedef struct ErtmSeg {
     int32_t lane;

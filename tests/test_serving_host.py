@@ -46,25 +46,27 @@ def test_admission_respects_streams_and_group_cap():
     assert _names(adm.take()) == [4]
 
 
-def test_lanes_go_to_the_worker_with_fewest_lanes_of_their_engine():
+def test_lanes_start_on_idle_workers_then_join_the_fewest_of_their_engine():
     adm = Admission(budget=100, streams=3, max_group=2)
     for name, nb, lane in [("e0", 10, "elastic"), ("a0", 10, "acoustic"), ("e1", 10, "elastic"),
-                           ("e2", 10, "elastic"), ("g", 10, None), ("e3", 10, "elastic")]:
+                           ("e2", 10, "elastic"), ("e3", 10, "elastic"), ("g", 10, None),
+                           ("e4", 10, "elastic")]:
         adm.add(_item(name, nb, "k" + name, lane))
-    got = [adm.take() for _ in range(4)]
+    got = [adm.take() for _ in range(5)]
     assert [(g[0], g[1], _names(g)) for g in got] == [
-        ("lane", 0, ["e0"]), ("lane", 1, ["a0"]), ("lane", 0, ["e1"]), ("lane", 2, ["e2"])]
-    # every worker busy: the group request waits, and the lane behind it too
+        ("lane", 0, ["e0"]), ("lane", 1, ["a0"]), ("lane", 2, ["e1"]), ("lane", 0, ["e2"]),
+        ("lane", 2, ["e3"])]
+    # no idle worker for the group request, and the lane behind it waits too
     assert adm.take() is None and adm.kind == ["elastic", "acoustic", "elastic"]
     adm.release(1, got[1][2])          # the acoustic worker is idle again
     g = adm.take()
     assert g[0] == "group" and g[1] == 1 and _names(g) == ["g"]
-    # e3 joins the elastic worker with fewer lanes (worker 0 is full)
-    e3 = adm.take()
-    assert (e3[0], e3[1], _names(e3)) == ("lane", 2, ["e3"]) and adm.count == [2, 1, 2]
-    adm.release(2, got[3][2])
-    adm.release(2, e3[2])
-    assert adm.kind[2] is None and adm.reserved == 30 and adm.peak <= 100
+    # elastic workers are full (max_group 2): e4 waits until a lane leaves
+    assert adm.take() is None
+    adm.release(2, got[2][2])
+    e4 = adm.take()
+    assert (e4[0], e4[1], _names(e4)) == ("lane", 2, ["e4"]) and adm.count == [2, 1, 2]
+    assert adm.reserved == 50 and adm.peak <= 100
 
 
 def _request(nz, nx, nt, engine="elastic", checkpoint_mb=256.0):

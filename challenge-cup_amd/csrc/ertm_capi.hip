@@ -1066,6 +1066,20 @@ int ertm_rotate_components(float* vx, float* vz, int64_t n, double cos_a, double
     return ERTM_OK;
 }
 
+int ertm_resample_record(const float* in, int32_t n_in, int32_t nrec, const double* w,
+                         const int32_t* off, int32_t n_out, int32_t taps, float* out,
+                         int32_t out_rows, void* stream) {
+    if (!in || !w || !off || !out || n_in < 1 || nrec < 1 || n_out < 0 || out_rows < 0 ||
+        taps < 1 || taps > 64)
+        return fail(ERTM_EINVAL, "resample_record: bad arguments");
+    if ((int64_t)out_rows * nrec >= ((int64_t)1 << 40))
+        return fail(ERTM_EINVAL, "resample_record: record too large");
+    if (out_rows == 0) return ERTM_OK;
+    launch_resample(in, n_in, nrec, w, off, n_out, taps, out, out_rows, (hipStream_t)stream);
+    LAUNCHCHK("resample");
+    return ERTM_OK;
+}
+
 int ertm_step_timing(int32_t enable) {
     std::lock_guard<std::mutex> lk(g_timer.m);
     if (enable) {

@@ -620,6 +620,39 @@ void launch_prep_mute_taper(float* rec, int nt, int nrec, const double* t, const
                        nrec, t, t_mute, taper, w_end);
 }
 
+// Sinc resampling of a record (ertm/numerics.py:337-392) followed by the
+// length fit of ertm/imaging.py:416-423: out row k < n_out is
+// sum_q w[k][q] * in[clip(off[k][q], 0, n_in - 1)] over the taps q in order,
+// in float64 from zero with every product and sum rounded once (the order
+// NumPy's einsum takes for records of two or more traces), then rounded to
+// float32; rows n_out .. out_rows-1 are zero.  Thread = (row, trace), traces
+// fastest: coalesced gathers of input rows.
+__global__ void k_resample(const float* __restrict__ in, int n_in, int nrec,
+                           const double* __restrict__ w, const int32_t* __restrict__ off,
+                           int n_out, int taps, float* __restrict__ out, int out_rows) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)out_rows * nrec) return;
+    const int k = (int)(i / nrec);
+    const int r = (int)(i - (int64_t)k * nrec);
+    float v = 0.0f;
+    if (k < n_out) {
+        double acc = 0.0;
+        for (int q = 0; q < taps; ++q) {
+            const int j = min(max(off[(int64_t)k * taps + q], 0), n_in - 1);
+            const double p = w[(int64_t)k * taps + q] * (double)in[(int64_t)j * nrec + r];
+            acc = acc + p;
+        }
+        v = (float)acc;
+    }
+    out[i] = v;
+}
+
+void launch_resample(const float* in, int n_in, int nrec, const double* w, const int32_t* off,
+                     int n_out, int taps, float* out, int out_rows, hipStream_t s) {
+    hipLaunchKernelGGL(k_resample, grid_of((int64_t)out_rows * nrec), dim3(256), 0, s, in, n_in,
+                       nrec, w, off, n_out, taps, out, out_rows);
+}
+
 void launch_prep_rotate(float* vx, float* vz, int64_t n, double c, double s_, hipStream_t s) {
     hipLaunchKernelGGL(k_prep_rotate, grid_of(n), dim3(256), 0, s, vx, vz, n, c, s_);
 }

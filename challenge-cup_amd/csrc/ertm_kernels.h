@@ -102,6 +102,8 @@ void launch_post_finish(const double* src, int from_complex, int nz, int nx, int
 void launch_prep_mute_taper(float* rec, int nt, int nrec, const double* t, const double* t_mute,
                             double taper, int w_end, hipStream_t s);
 void launch_prep_rotate(float* vx, float* vz, int64_t n, double c, double s_, hipStream_t s);
+void launch_resample(const float* in, int n_in, int nrec, const double* w, const int32_t* off,
+                     int n_out, int taps, float* out, int out_rows, hipStream_t s);
 void launch_image_flush(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la,
                         int max_cells, hipStream_t s);
 void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,

@@ -145,6 +145,8 @@ EXPORTS = [
                                       C.c_double, C.c_int32, C.c_void_p]),
     ("ertm_rotate_components", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
                                          C.c_double, C.c_void_p]),
+    ("ertm_resample_record", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                       C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
     ("ertm_step_timing", C.c_int, [C.c_int32]),
     ("ertm_step_timing_read", C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),

@@ -187,6 +187,12 @@ class GpuConfig:
                    (0, 0) = one step per launch; None = the tuner's choice
                    (one step: two per launch measured slower on MI355X).
                    Same results.
+    device_resample  resample records whose dt differs from the solver's on
+                   the device (ertm_resample_record: weights from the host,
+                   the tap sums on the GPU, fused with the length fit) when
+                   they are float32 with two or more traces and no
+                   preprocess callable is given; same results as the host
+                   path (whose einsum sums the taps in the same order).
     """
 
     strict_cfl: bool = True
@@ -199,6 +205,7 @@ class GpuConfig:
     device_postprocess: bool = False
     graph_steps: int = 0
     two_step: tuple | None = None
+    device_resample: bool = True
 
     def validate(self) -> None:
         if self.device_mem_gb <= 0:

@@ -669,18 +669,18 @@ class Batch:
                                        C.c_void_p(L.ptr(ctl)), L.stream_handle()),
                 "ertm_stabilize")
 
-    def run_lanes(self, segs, curs: list, *, max_ticks: int = 0,
+    def run_lanes(self, segs, nseg: int, curs: list, *, max_ticks: int = 0,
                   stop_at_lane_end: bool = False) -> tuple[int, int, int]:
-        """One ertm_run_lanes call over ``segs`` (an ErtmSeg array whose
-        ``done`` counters advance in place); ``curs`` holds each
-        propagator's parity and is updated.  The host wavelets the segments
-        point to must stay alive for the call.  Returns (ticks, lane steps,
-        step-launch groups)."""
+        """One ertm_run_lanes call over ``nseg`` ErtmSeg records at ``segs``
+        (a ctypes pointer; their ``done`` counters advance in place);
+        ``curs`` holds each propagator's parity and is updated.  The host
+        wavelets the segments point to must stay alive for the call.
+        Returns (ticks, lane steps, step-launch groups)."""
         if len(curs) != self.n:
             raise ValueError("one parity per propagator expected")
         cur = (C.c_int32 * self.n)(*[int(c) for c in curs])
         stats = (C.c_int64 * 3)()
-        L.check(L.lib().ertm_run_lanes(self.h, segs, len(segs), cur, int(max_ticks),
+        L.check(L.lib().ertm_run_lanes(self.h, segs, int(nseg), cur, int(max_ticks),
                                        int(bool(stop_at_lane_end)), stats, L.stream_handle()),
                 "ertm_run_lanes")
         curs[:] = list(cur)

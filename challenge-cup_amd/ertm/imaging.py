@@ -411,7 +411,8 @@ class RTMImager:
             if not record:
                 raise ImagingError("empty shot record: nothing to back-propagate")
             specs.append(spec)
-            records.append({nm: dev.to_device(np.asarray(v, dtype=np.float32))
+            records.append({nm: v if not isinstance(v, np.ndarray)
+                            else dev.to_device(np.asarray(v, dtype=np.float32))
                             for nm, v in record.items()})
             waves.append(self._prepare_wavelet(shot, spec,
                                                None if wavelets is None else wavelets[i]))
@@ -733,12 +734,25 @@ class RTMImager:
                         preprocess: Callable[[np.ndarray, float], np.ndarray] | None
                         ) -> dict[str, np.ndarray]:
         """Align the record to (nt, dt): sinc resampling, optional preprocess,
-        truncate / zero-pad (ertm/imaging.py:396-423)."""
+        truncate / zero-pad (ertm/imaging.py:396-423).  Host float64 arrays,
+        or, for float32 records of two or more traces resampled without a
+        preprocess callable (``GpuConfig.device_resample``), (nt, nrec)
+        float32 device tensors resampled and fitted on the GPU."""
         comps = dict(shot.components) or {"primary": shot.data}
         out: dict[str, np.ndarray] = {}
         for name, arr in comps.items():
+            resample = abs(float(shot.dt) - spec.dt) > 1e-12 and np.shape(arr)[0] > 1
+            if (resample and preprocess is None and self.gpu_config.device_resample
+                    and getattr(arr, "dtype", None) == np.float32 and np.ndim(arr) == 2
+                    and np.shape(arr)[1] >= 2 and np.shape(arr)[0] >= 3):
+                # on the device: a (nt, nrec) float32 tensor, already fitted
+                from .preprocess import resample_record_on_device
+
+                out[name] = resample_record_on_device(arr, float(shot.dt), spec.dt,
+                                                      int(spec.nt))
+                continue
             data = np.asarray(arr, dtype=np.float64)
-            if abs(float(shot.dt) - spec.dt) > 1e-12 and data.shape[0] > 1:
+            if resample:
                 data = num.resample_sinc(data, float(shot.dt), spec.dt)
             if preprocess is not None:
                 data = np.asarray(preprocess(data, spec.dt), dtype=np.float64)

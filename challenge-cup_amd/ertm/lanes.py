@@ -25,6 +25,7 @@ keep their own loops in ``RTMImager``).
 
 from __future__ import annotations
 
+import ctypes as C
 import math
 import time
 from typing import Any
@@ -40,6 +41,7 @@ from .types import ImageResult
 
 _GIB = 1024.0 ** 3
 LANE_ENGINES = ("acoustic", "elastic")
+SEG_DTYPE = np.dtype(L.ErtmSeg)   # the C layout of ErtmSeg
 
 
 def lanes_supported(imager: RTMImager) -> bool:
@@ -129,6 +131,7 @@ class Lane:
             nsub = int(math.ceil(interval / self.slots))
             self.subs = torch.empty((nsub, self.stride), dtype=torch.float32, device="cuda")
         self.program = self._program()
+        self.segs = self.segments()
         self.replayed = sum(e - a for ph, a, e, *_ in self.program if ph == 1)
         self.back_steps = sum(e - a for ph, a, e, *_ in self.program if ph == 2)
         self.lane_steps = 0      # ticks this lane ran
@@ -159,26 +162,31 @@ class Lane:
                 prog.append((2, a, e, L.IMG_CORRELATE, a, 0, 0, None, None))
         return prog
 
-    def segments(self, lane_id: int, fwd_index: int, back_index: int) -> list:
-        """The program as ErtmSeg records for a batch holding the forward
-        propagator at ``fwd_index`` and the backward one at ``back_index``."""
-        out = []
-        wptr = self.wave.ctypes.data
-        for ph, a, e, op, t0, cki, cko, save, restore in self.program:
-            g = L.ErtmSeg()
-            g.lane = int(lane_id)
-            g.prop = int(back_index if ph == 2 else fwd_index)
-            g.phase, g.t_begin, g.t_end, g.op_after, g.img_t0 = ph, a, e, op, t0
-            g.ck_interval, g.ck_origin = cki, cko
-            if save is not None:
-                g.save_arena = L.ptr(save[0]) + 4 * save[1] * self.stride
-                g.save_stride = self.stride
-            if restore is not None:
-                g.restore = L.ptr(restore[0]) + 4 * restore[1] * self.stride
-            if ph != 2:
-                g.wavelet = wptr
-            out.append(g)
-        return out
+    def segments(self) -> np.ndarray:
+        """The program as ErtmSeg records (a NumPy structured array of the
+        C layout) with ``prop`` 0 for the forward propagator and 1 for the
+        backward one; the runner offsets them to the batch's indices."""
+        prog = self.program
+        segs = np.zeros(len(prog), dtype=SEG_DTYPE)
+        segs["phase"] = [g[0] for g in prog]
+        segs["prop"] = (segs["phase"] == 2).astype(np.int32)
+        segs["t_begin"] = [g[1] for g in prog]
+        segs["t_end"] = [g[2] for g in prog]
+        segs["op_after"] = [g[3] for g in prog]
+        segs["img_t0"] = [g[4] for g in prog]
+        segs["ck_interval"] = [g[5] for g in prog]
+        segs["ck_origin"] = [g[6] for g in prog]
+        row = 4 * self.stride   # bytes per saved state
+        segs["save_arena"] = [0 if g[7] is None else L.ptr(g[7][0]) + row * g[7][1] for g in prog]
+        segs["save_stride"] = [0 if g[7] is None else self.stride for g in prog]
+        segs["restore"] = [0 if g[8] is None else L.ptr(g[8][0]) + row * g[8][1] for g in prog]
+        segs["wavelet"] = np.where(segs["phase"] != 2, self.wave.ctypes.data, 0)
+        return segs
+
+    @property
+    def done(self) -> bool:
+        last = self.segs[-1]
+        return bool(last["done"] >= last["t_end"] - last["t_begin"])
 
     @property
     def cell_updates(self) -> float:
@@ -268,7 +276,6 @@ class LaneRunner:
         self.max_lanes = int(max_lanes)
         self.variant = variant
         self.lanes: list[Lane] = []
-        self._segs: dict[int, list] = {}
         self._batch = None
         self._curs: list[int] = []
         self._next_id = 0
@@ -319,22 +326,13 @@ class LaneRunner:
                 variant = (2, 48)
         self._batch = dev.Batch(cores, recvs=recvs, imgs=imgs, variant=variant)
         self._curs = [int(c.cur) for c in cores]
-        self._segs = {}
-        for k, ln in enumerate(self.lanes):
-            old = getattr(ln, "_segs", None)
-            segs = ln.segments(ln.lane_id, 2 * k, 2 * k + 1)
-            if old is not None:
-                for g, o in zip(segs, old):
-                    g.done = o.done
-            ln._segs = segs
-        flat = [g for ln in self.lanes for g in ln._segs]
-        self._arr = (L.ErtmSeg * len(flat))(*flat)
-        # the array holds copies: progress is read back from it after each call
-        self._flat_index = []
-        i = 0
-        for ln in self.lanes:
-            self._flat_index.append((ln, i, len(ln._segs)))
-            i += len(ln._segs)
+        # every lane's program in one array: lane k's propagators are 2k, 2k+1
+        counts = np.array([len(ln.segs) for ln in self.lanes])
+        self._arr = np.concatenate([ln.segs for ln in self.lanes])
+        k = np.repeat(np.arange(len(self.lanes), dtype=np.int32), counts)
+        self._arr["prop"] += 2 * k
+        self._arr["lane"] = np.repeat(np.array([ln.lane_id for ln in self.lanes], np.int32), counts)
+        self._bounds = np.concatenate([[0], np.cumsum(counts)])
 
     def run(self, max_ticks: int = 0, post: bool = True) -> list[Lane]:
         """Advance every lane until one finishes (or ``max_ticks`` ticks);
@@ -353,19 +351,19 @@ class LaneRunner:
             if ln.started is None:
                 ln.started = now
         nlanes = len(self.lanes)
-        ticks, steps, groups = self._batch.run_lanes(self._arr, self._curs, max_ticks=max_ticks,
-                                                     stop_at_lane_end=True)
+        segs = self._arr.ctypes.data_as(C.POINTER(L.ErtmSeg))
+        ticks, steps, groups = self._batch.run_lanes(segs, len(self._arr), self._curs,
+                                                     max_ticks=max_ticks, stop_at_lane_end=True)
         self.ticks += ticks
         self.lane_steps += steps
         self.step_groups += groups
         done = []
-        for ln, i0, cnt in self._flat_index:
-            for j in range(cnt):
-                ln._segs[j].done = self._arr[i0 + j].done
+        for k, ln in enumerate(self.lanes):
+            ln.segs["done"] = self._arr["done"][self._bounds[k]:self._bounds[k + 1]]
             # the call ends at the tick a lane finishes: every lane ran every tick
             ln.lane_steps += ticks
             ln.shared_steps += ticks * nlanes
-            if all(g.done >= g.t_end - g.t_begin for g in ln._segs):
+            if ln.done:
                 done.append(ln)
         if not done:
             return []

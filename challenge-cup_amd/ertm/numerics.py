@@ -140,6 +140,27 @@ def gaussian_pulse(t: np.ndarray, f0: float, t0: float = 0.0) -> np.ndarray:
     return np.exp(-0.5 * u ** 2).astype(np.float32)
 
 
+def sinc_taps(n_in: int, dt_in: float, dt_out: float, *, half_length: int = 8,
+              beta: float = 8.6) -> tuple[np.ndarray, np.ndarray]:
+    """Weights (n_out, 2*half_length) float64 and tap rows (unclipped) of
+    ``resample_sinc`` (ertm/numerics.py:370-383): output sample k sits at
+    p = k*dt_out/dt_in, taps floor(p)-half_length+1 .. floor(p)+half_length,
+    Kaiser-windowed sinc weights normalised to unit sum (NumPy's row sum)."""
+    if dt_in <= 0 or dt_out <= 0:
+        raise ValueError("dt_in and dt_out must be positive")
+    if n_in < 3:
+        raise ValueError(f"too few samples ({n_in}) to interpolate")
+    n_out = int(np.floor((n_in - 1) * dt_in / dt_out + 1e-9)) + 1
+    pos = np.arange(n_out, dtype=np.float64) * (dt_out / dt_in)
+    taps = np.arange(-half_length + 1, half_length + 1)
+    idx = np.floor(pos).astype(np.int64)[:, None] + taps[None, :]
+    w = np.sinc(pos[:, None] - idx) * np.kaiser(2 * half_length, beta)[None, :]
+    norm = w.sum(axis=1, keepdims=True)
+    norm[norm == 0] = 1.0
+    w /= norm
+    return w, idx
+
+
 def resample_sinc(data: np.ndarray, dt_in: float, dt_out: float, *, axis: int = 0,
                   half_length: int = 8, beta: float = 8.6) -> np.ndarray:
     """Kaiser-windowed sinc interpolation along time (ertm/numerics.py:329-392).
@@ -158,16 +179,7 @@ def resample_sinc(data: np.ndarray, dt_in: float, dt_out: float, *, axis: int = 
     if axis == 1:
         x = x.T
     n_in = x.shape[0]
-    if n_in < 3:
-        raise ValueError(f"too few samples ({n_in}) to interpolate")
-    n_out = int(np.floor((n_in - 1) * dt_in / dt_out + 1e-9)) + 1
-    pos = np.arange(n_out, dtype=np.float64) * (dt_out / dt_in)
-    taps = np.arange(-half_length + 1, half_length + 1)
-    idx = np.floor(pos).astype(np.int64)[:, None] + taps[None, :]
-    w = np.sinc(pos[:, None] - idx) * np.kaiser(2 * half_length, beta)[None, :]
-    norm = w.sum(axis=1, keepdims=True)
-    norm[norm == 0] = 1.0
-    w /= norm
+    w, idx = sinc_taps(n_in, dt_in, dt_out, half_length=half_length, beta=beta)
     out = np.einsum("ot,ots->os", w, x[np.clip(idx, 0, n_in - 1)])
     out = np.ascontiguousarray(out)
     if squeeze:

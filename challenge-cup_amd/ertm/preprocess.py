@@ -169,6 +169,43 @@ def build_shot_sequence(base_shot: ShotRecord, source_positions: Sequence[float]
                     meta={**base_shot.meta, "shot_x": float(x)}) for x in source_positions]
 
 
+def resample_record_on_device(record: np.ndarray, dt_in: float, dt_out: float, nt: int):
+    """``resample_sinc`` (ertm/numerics.py:337-392) of a (n_in, nrec) float32
+    record onto ``dt_out``, then cut or zero-padded to ``nt`` rows as
+    ``RTMImager._prepare_record`` does (ertm/imaging.py:396-423): a (nt, nrec)
+    float32 device tensor.  The weights come from ``numerics.sinc_taps`` on
+    the host; the GPU sums the taps in order in float64, the order NumPy's
+    einsum takes for two or more traces, so the result equals the host
+    path's.  The record is complete when the call returns."""
+    import ctypes as C
+
+    import torch
+
+    from . import _lib as L
+    from . import device as dev
+
+    rec = np.ascontiguousarray(record, dtype=np.float32)
+    if rec.ndim != 2:
+        raise DataError("resample_record_on_device takes a (nt, nrec) record")
+    n_in, nrec = rec.shape
+    w, idx = num.sinc_taps(n_in, float(dt_in), float(dt_out))
+    n_out, taps = w.shape
+    if idx.max(initial=0) >= 2 ** 31:
+        raise DataError("record too long to resample on the device")
+    src = dev.to_device(rec)
+    wd = torch.from_numpy(np.ascontiguousarray(w)).to("cuda")
+    od = torch.from_numpy(np.ascontiguousarray(idx, dtype=np.int32)).to("cuda")
+    out = torch.empty((int(nt), nrec), dtype=torch.float32, device="cuda")
+    L.check(L.lib().ertm_resample_record(C.c_void_p(L.ptr(src)), n_in, nrec,
+                                         C.c_void_p(L.ptr(wd)), C.c_void_p(L.ptr(od)), n_out,
+                                         taps, C.c_void_p(L.ptr(out)), int(nt),
+                                         L.stream_handle()), "ertm_resample_record")
+    # complete on return, like an uploaded record: callers (the serving
+    # loop) may read it from other streams
+    torch.cuda.current_stream().synchronize()
+    return out
+
+
 def prepare_records_on_device(records: dict, shot: ShotRecord, *, model: ElasticModel,
                               rotation_deg: float = 0.0, mute: bool = True,
                               v_reference: float | None = None,

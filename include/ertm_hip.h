@@ -450,6 +450,17 @@ int ertm_prepare_record(float* rec, int32_t nt, int32_t nrec, const double* t,
  * ertm/preprocess.py:36-63), in place on n floats each. */
 int ertm_rotate_components(float* vx, float* vz, int64_t n, double cos_a, double sin_a,
                            void* stream);
+/* Kaiser-windowed sinc resampling of a record onto the solver's dt
+ * (resample_sinc, ertm/numerics.py:337-392) fused with the length fit of
+ * RTMImager._prepare_record (ertm/imaging.py:396-423).  in: (n_in, nrec)
+ * float32 device record; w, off: (n_out, taps) float64 weights (normalised as
+ * the reference normalises them) and int32 tap rows, on the device, computed
+ * on the host; out: (out_rows, nrec) float32 device record.  Row k < n_out is
+ * the float64 sum over the taps in order of w * in[clip(off, 0, n_in - 1)],
+ * rounded to float32; later rows are zero. */
+int ertm_resample_record(const float* in, int32_t n_in, int32_t nrec, const double* w,
+                         const int32_t* off, int32_t n_out, int32_t taps, float* out,
+                         int32_t out_rows, void* stream);
 
 /* ---- measurement (no counterpart in the reference) ------------------------ */
 /* Launch timing.  While enabled, every launch that ertm_step / ertm_run /

@@ -55,6 +55,10 @@ def test_abi_version_and_error_path_without_gpu():
     assert lib.ertm_batch_create(None, None, None, 0, None) == 1
     assert b"bad arguments" in lib.ertm_last_error()
     assert lib.ertm_derivative(None, None, 4, 4, 0, 1, 8, 1.0, 0, None) == 1
+    segs = (_lib.ErtmSeg * 1)()
+    cur = (C.c_int32 * 1)()
+    assert lib.ertm_run_lanes(None, segs, 1, cur, 0, 0, None, None) == 1
+    assert b"lanes: bad arguments" in lib.ertm_last_error()
 
 
 def test_noalloc_postprocess_rejects_oversize_and_lowpass_without_gpu():

@@ -223,6 +223,34 @@ def test_device_record_preparation_matches_host(rotation):
         assert np.mean(got == want) > 0.99, name
 
 
+@pytest.mark.parametrize("shape,dt_in,dt_out,nt", [
+    ((500, 37), 3.6e-5, 2.5e-5, 700),     # upsampled, then cut
+    ((500, 37), 3.6e-5, 2.5e-5, 900),     # upsampled, then zero-padded
+    ((801, 2), 4.0e-5, 5.3e-5, 640),      # downsampled, two traces
+    ((64, 300), 1.0e-4, 3.3e-5, 190),     # few samples, many traces
+])
+def test_device_resampling_equals_host_and_oracle(shape, dt_in, dt_out, nt):
+    """ertm_resample_record (sinc taps on the GPU, weights from the host)
+    fused with the length fit equals the host path -- resample_sinc then
+    _fit_length, rounded to float32 -- bit for bit, and the oracle's
+    resample_sinc (ertm/numerics.py:337-392 restated) as well."""
+    import torch
+    from ertm import numerics as num
+    from ertm.imaging import RTMImager
+    from ertm.preprocess import resample_record_on_device
+    from oracle import numerics as onum
+
+    rng = np.random.default_rng(shape[0] + shape[1])
+    rec = (rng.standard_normal(shape) * np.linspace(1, 3, shape[1])).astype(np.float32)
+    got = resample_record_on_device(rec, dt_in, dt_out, nt).cpu().numpy()
+    host = RTMImager._fit_length(num.resample_sinc(rec, dt_in, dt_out), nt).astype(np.float32)
+    oracle = RTMImager._fit_length(onum.resample_sinc(rec, dt_in, dt_out), nt).astype(np.float32)
+    assert got.shape == (nt, shape[1]) and got.dtype == np.float32
+    assert np.array_equal(got, host) and np.array_equal(got, oracle)
+    assert np.abs(got).max() > 0
+    torch.cuda.synchronize()
+
+
 # ---------------------------------------------------------------------------
 # Viscoacoustic engine (tests/test_rtm.py:391-558): measured Q, round-trip
 # gain of attenuation then compensation, the band limit and gain budget of

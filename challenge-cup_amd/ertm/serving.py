@@ -154,18 +154,24 @@ def _key(req: ShotRequest, imager_key) -> tuple:
 
 
 class Admission:
-    """First-come first-served admission by bytes onto the loop's workers.
+    """Admission by bytes onto the loop's workers.
 
     Items are (request, job, bytes, key, arrival, future, lane) tuples; only
     bytes, key and lane are read here.  ``lane`` is the engine name of a
     request that runs as a lane (``ertm.lanes``), None for one that runs in a
     launch group.  A worker is idle, runs one launch group, or runs lanes of
-    one engine (up to ``max_group``).  ``take`` returns the next assignment
-    -- ("group", worker, [items]) with the oldest waiting item and its
-    same-key companions, or ("lane", worker, [item]) onto an idle worker,
-    else the worker whose lanes of that engine are fewest -- or None when the
-    oldest waiting item does not fit in the budget or has nowhere to go.
-    Nothing overtakes the oldest waiting item.
+    one engine (up to ``max_group``).  ``take`` returns the next assignment:
+    ("group", worker, [items]) -- an item and its same-key companions on an
+    idle worker -- or ("lane", worker, [item]) onto an idle worker, else the
+    worker whose lanes of that engine are fewest; or None.
+
+    Order: the bytes are first-come first-served (nothing overtakes an item
+    that waits for HBM).  An item that fits but has no worker to go to (every
+    worker busy with the other engine, or no idle one for a launch group)
+    lets later items of other kinds start, so one engine never blocks the
+    other; and so that it does not wait for ever, the busy worker with the
+    fewest lanes is marked draining: it takes no new lanes and goes idle
+    when its own finish.
     """
 
     def __init__(self, budget: int, streams: int, max_group: int) -> None:
@@ -176,6 +182,7 @@ class Admission:
         self.peak = 0
         self.kind: list = [None] * int(streams)   # None | "group" | engine of its lanes
         self.count = [0] * int(streams)
+        self.draining: set = set()
 
     @property
     def idle(self) -> int:
@@ -184,55 +191,76 @@ class Admission:
     def add(self, item) -> None:
         self.pending.append(item)
 
+    def _idle_worker(self):
+        return next((w for w, k in enumerate(self.kind) if k is None), None)
+
     def _lane_worker(self, engine: str):
         # an idle worker first (its own stream: runs beside the others), else
         # the worker with the fewest lanes of this engine
-        best = next((w for w, k in enumerate(self.kind) if k is None), None)
+        best = self._idle_worker()
         if best is not None:
             return best
         for w, k in enumerate(self.kind):
-            if k == engine and self.count[w] < self.max_group and (
-                    best is None or self.count[w] < self.count[best]):
+            if (k == engine and w not in self.draining and self.count[w] < self.max_group
+                    and (best is None or self.count[w] < self.count[best])):
                 best = w
         return best
+
+    def _drain_one(self, engine) -> None:
+        """No worker can take an item of ``engine`` (None: a launch group):
+        unless one already drains, stop feeding the busy worker with the
+        fewest items of another kind, so it comes free."""
+        if self.draining:
+            return
+        cands = [w for w, k in enumerate(self.kind) if k is not None and k != engine]
+        if cands:
+            self.draining.add(min(cands, key=lambda w: self.count[w]))
 
     def _reserve(self, nbytes: int) -> None:
         self.reserved += nbytes
         self.peak = max(self.peak, self.reserved)
 
     def take(self):
-        if not self.pending:
-            return None
-        head = self.pending[0]
-        if self.reserved + head[2] > self.budget:
-            return None
-        lane = head[6]
-        if lane is not None:
-            w = self._lane_worker(lane)
-            if w is None:
+        blocked: set = set()   # kinds whose oldest waiting item has no worker
+        for item in list(self.pending):
+            if self.reserved + item[2] > self.budget:
                 return None
-            self.pending.popleft()
-            self._reserve(head[2])
-            self.kind[w] = lane
-            self.count[w] += 1
-            return ("lane", w, [head])
-        w = next((w for w, k in enumerate(self.kind) if k is None), None)
-        if w is None:
-            return None
-        group = [head]
-        used = head[2]
-        for item in list(self.pending)[1:]:
-            if len(group) >= self.max_group:
-                break
-            if item[6] is None and item[3] == head[3] and self.reserved + used + item[2] <= self.budget:
-                group.append(item)
-                used += item[2]
-        taken = {id(item) for item in group}
-        self.pending = deque(item for item in self.pending if id(item) not in taken)
-        self._reserve(used)
-        self.kind[w] = "group"
-        self.count[w] = len(group)
-        return ("group", w, group)
+            lane = item[6]
+            kind = lane if lane is not None else ("group", item[3])
+            if kind in blocked:
+                continue
+            if lane is not None:
+                w = self._lane_worker(lane)
+                if w is None:
+                    blocked.add(kind)
+                    self._drain_one(lane)
+                    continue
+                self.pending.remove(item)
+                self._reserve(item[2])
+                self.kind[w] = lane
+                self.count[w] += 1
+                return ("lane", w, [item])
+            w = self._idle_worker()
+            if w is None:
+                blocked.add(kind)
+                self._drain_one(None)
+                continue
+            group = [item]
+            used = item[2]
+            for other in self.pending:
+                if len(group) >= self.max_group:
+                    break
+                if (other is not item and other[6] is None and other[3] == item[3]
+                        and self.reserved + used + other[2] <= self.budget):
+                    group.append(other)
+                    used += other[2]
+            taken = {id(x) for x in group}
+            self.pending = deque(x for x in self.pending if id(x) not in taken)
+            self._reserve(used)
+            self.kind[w] = "group"
+            self.count[w] = len(group)
+            return ("group", w, group)
+        return None
 
     def release(self, worker: int, items) -> None:
         """Items assigned to ``worker`` have finished: free their bytes."""
@@ -241,6 +269,7 @@ class Admission:
         if self.count[worker] <= 0:
             self.kind[worker] = None
             self.count[worker] = 0
+            self.draining.discard(worker)
 
 
 class ServingLoop:

@@ -100,3 +100,23 @@ def test_request_price_covers_the_large_parts_and_grows_with_the_job():
     assert request_bytes(r, GpuConfig(replay_slots=8)) < request_bytes(r)
     # acoustic requests are cheaper than elastic ones of the same shape
     assert request_bytes(_request(600, 1200, 4000, "acoustic")) < request_bytes(r)
+
+
+def test_one_engine_does_not_block_the_other():
+    """An acoustic request with no worker to go to lets later elastic ones
+    start, and the busy worker with the fewest lanes drains so that the
+    acoustic one gets a worker."""
+    adm = Admission(budget=100, streams=2, max_group=8)
+    for name, lane in [("e0", "elastic"), ("e1", "elastic"), ("a0", "acoustic"),
+                       ("e2", "elastic"), ("e3", "elastic")]:
+        adm.add(_item(name, 10, "k" + name, lane))
+    got = [adm.take() for _ in range(3)]
+    assert [(g[1], _names(g)) for g in got] == [(0, ["e0"]), (1, ["e1"]), (1, ["e2"])]
+    # a0 waits; worker 0 (fewest lanes) drains: e3 joins worker 1 instead
+    assert adm.draining == {0}
+    e3 = adm.take()
+    assert (e3[1], _names(e3)) == (1, ["e3"]) and adm.take() is None
+    adm.release(0, got[0][2])
+    a0 = adm.take()
+    assert (a0[0], a0[1], _names(a0)) == ("lane", 0, ["a0"]) and adm.kind == ["acoustic", "elastic"]
+    assert not adm.draining and not adm.pending

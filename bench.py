@@ -400,6 +400,64 @@ def engine_sweep(nshots, steps, warmup=3):
     return out
 
 
+def step_shapes(shapes, steps, nshots, large=False, warmup=3):
+    """Plain elastic step time per launch shape: 64 shots of 600 x 1200 (the
+    layered model, free surface, 20-cell CPML; ``nshots`` shots) or, with
+    ``large``, one 2000 x 6000 shot of the large single-shot model.  One JSON
+    line per (variant, zc): microseconds per launch from HIP events on the
+    launch stream, the fraction of the 8 TB/s roofline at 52 algorithmic bytes
+    per cell-update, the launch's block count and the rounds of resident
+    blocks it makes (20 waves per CU on 256 CUs)."""
+    import math
+
+    import torch
+    from ertm import device as dev
+    from ertm.propagation import PropagationSpec, build_propagator
+
+    if large:
+        model, dt, n, label = c5_model(), 2.0e-4, 1, "elastic step, one 2000x6000 shot"
+    else:
+        model, dt, n = layered_model(), DT, nshots
+        label = f"elastic step, {nshots} shots of {NZ}x{NX}"
+    nz, nx = model.shape
+    spec = PropagationSpec(dt=dt, nt=steps + warmup, order=8, npml=20, boundary="free")
+    props = [build_propagator(model, spec, engine="elastic") for _ in range(n)]
+    cores = [p._core for p in props]
+    for c in cores:
+        c.state[c.cur].normal_(0.0, 1e-3)
+    b = dev.Batch(cores)
+    amps = [0.0] * n
+    out = []
+    for v, zc in shapes:
+        b.set_variant(v, zc)
+        for _ in range(warmup):
+            b.step(cores[0].cur, amps)
+            for c in cores:
+                c.cur ^= 1
+        torch.cuda.synchronize()
+        with dev.StepTimer():
+            for _ in range(steps):
+                b.step(cores[0].cur, amps)
+                for c in cores:
+                    c.cur ^= 1
+            torch.cuda.synchronize()
+        kt = dev.StepTimer.read()
+        us = kt["total_ms"] / max(kt["launches"], 1) * 1e3
+        lanes = {1: 256, 2: 128, 3: 64, 4: 192}[v]
+        c0 = cores[0]
+        nb = n * math.ceil(c0.nxp / (lanes - 16)) * math.ceil(c0.nzp / zc)
+        per_round = 256 * (20 // (lanes // 64))
+        line = {"workload": label, "launch_shape": [v, zc], "launches": kt["launches"],
+                "us_per_launch": round(us, 2),
+                "roofline_frac": round(n * nz * nx * BYTES_PER_CELL / (us * 1e-6)
+                                       / (HBM_PEAK_GBS * 1e9), 4),
+                "blocks": nb, "rounds": round(nb / per_round, 3)}
+        print(json.dumps(line), flush=True)
+        out.append(line)
+    b.close()
+    return out
+
+
 def step_roofline(kinds, steps, elapsed):
     """The step kernel over every launch of the timed batches, from HIP events
     on the launch stream: all its variants together (the forward pass's plain
@@ -567,6 +625,11 @@ def main(argv=None) -> int:
                     help="instead of the RTM bench: plain step throughput of each engine's "
                          "single-launch kernel on the 64-shot grid")
     ap.add_argument("--sweep-steps", type=int, default=200)
+    ap.add_argument("--step-shapes", default=None,
+                    help="instead of the RTM bench: plain elastic step time for each launch "
+                         "shape 'v,zc;v,zc;...' on the 64-shot grid (--step-large: on one "
+                         "2000 x 6000 shot), --sweep-steps steps each")
+    ap.add_argument("--step-large", action="store_true")
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="also time this many shots migrated one at a time (the plain per-shot "
                          "path) for the batched/plain ratio; 0 = skip")
@@ -594,7 +657,8 @@ def main(argv=None) -> int:
 
     pool_baseline = None
     if (not args.no_cpu_baseline and not args.c5_sweep and not args.engine_sweep
-            and not args.serve_sweep and torch.cuda.device_count() > 0):
+            and not args.serve_sweep and not args.step_shapes
+            and torch.cuda.device_count() > 0):
         # before any HIP call: the pool forks
         print(f"# cpu baseline (process pool): oracle, nt={args.cpu_sample_nt}",
               file=sys.stderr, flush=True)
@@ -619,6 +683,10 @@ def main(argv=None) -> int:
         return 0
     if args.engine_sweep:
         engine_sweep(args.shots, args.sweep_steps)
+        return 0
+    if args.step_shapes:
+        shapes = [tuple(int(v) for v in t.split(",")) for t in args.step_shapes.split(";")]
+        step_shapes(shapes, args.sweep_steps, args.shots, large=args.step_large)
         return 0
     if args.serve_sweep:
         serving_sweep(args.serve, args.serve_scale,

@@ -141,6 +141,29 @@ def autotune(batch, steps: int = 10, cands=None) -> tuple[tuple[int, int], dict]
     return best, times
 
 
+def round_filling(nzp: int, nxp: int, nbatch: int, n_cu: int = 256, max_rounds: int = 4,
+                  waves_per_cu: int = 20) -> list[tuple[int, int]]:
+    """Launch shapes whose blocks fill whole rounds of the resident blocks.
+
+    A launch of equal blocks takes about ceil(blocks / resident) block
+    durations: a few blocks past a round boundary cost a whole extra round.
+    For 256-, 128- and 192-lane blocks (resident: 20 waves per CU) and r = 1
+    .. max_rounds rounds, the three shortest chunk heights (16 to 128 rows)
+    whose block count stays within r rounds.  Measured on the 2000 x 6000
+    shot (one launch of 0.96-1.0 rounds per step): 128 lanes, 42-row chunks
+    (1.01 rounds) 249 us per step against 216 us with 43-row chunks (0.99);
+    256 lanes, 43-row chunks 198 us against 205 us for the shape the
+    candidates above chose."""
+    out = []
+    for v in (1, 2, 4):
+        per_round = n_cu * (waves_per_cu // (threads(v) // 64))
+        for r in range(1, max_rounds + 1):
+            fits = [zc for zc in range(16, 129) if blocks(v, zc, nzp, nxp, nbatch) <= r * per_round]
+            heights = sorted({math.ceil(nzp / math.ceil(nzp / zc)) for zc in fits})[:3]
+            out += [(v, zc) for zc in heights if (v, zc) not in out]
+    return out
+
+
 # A run of at least this many cell-updates (cells x steps x shots) is worth a
 # short tuning pass on its first batch: about 60 ms of steps against seconds.
 TUNE_MIN_WORK = 2.0e10
@@ -168,6 +191,7 @@ def maybe_autotune(batch, work: float, n_cu: int = 256) -> tuple[int, int] | Non
     # 11 % faster with (4, 24) than with (4, 48)
     cands = [(v, zc) for v in (4, 2) for zc in (48, 32, 64, 24, 16)
              if blocks(v, zc, nzp, nxp, batch.n) >= 4 * n_cu]
+    cands += [c for c in round_filling(nzp, nxp, batch.n, n_cu) if c not in cands]
     if len(cands) < 2:
         return None
     saved = [(c.state.clone(), c.cur) for c in batch.cores]

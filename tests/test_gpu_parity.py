@@ -587,3 +587,102 @@ def test_fused_imaging_on_random_small_grids_matches_two_launch(seed):
         for md in modes:
             assert np.array_equal(got[md].image, ref[md].image), (kv, ring, md)
         assert np.abs(ref[modes[0]].image).max() > 0.0
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_fused_on_mid_size_grids_match_two_launch(seed):
+    """Grids of 150-260 x 200-420 cells, where most blocks of a fused launch
+    stream chunks away from every edge, beside blocks at CPML strips, the
+    free surface, grid edges and the source: the whole state after 60 steps,
+    the state and checkpoints of a checkpointing forward pass two steps per
+    launch, and migrated images with imaging inside the step (one or two
+    steps per launch) equal the two-launch kernels' bit for bit for every
+    launch shape, boundary, source mechanism, mode pair and separation."""
+    import torch
+    from ertm import device as dev
+    from ertm.config import GpuConfig, ImagingConfig, SolverConfig
+    from ertm.imaging import RTMImager
+    from ertm.propagation import PropagationSpec, build_propagator, forward_shot
+    from ertm.types import ElasticModel, GridSpec, ReceiverSpec, SourceSpec
+
+    rng = np.random.default_rng(300 + seed)
+    boundary = ("none", "free", "cpml")[seed % 3]
+    mech = ("explosive", "shear", "force_z")[(seed // 3 + seed) % 3]
+    nz, nx = int(rng.integers(150, 260)), int(rng.integers(200, 420))
+    _say("mid-size grid", seed, boundary, mech, nz, nx)
+    vp = rng.uniform(2000, 3600, (nz, nx)).astype(np.float32)
+    vp[nz // 2:] += 700.0
+    m = ElasticModel(GridSpec(nx=nx, nz=nz, dx=5.0, dz=4.0), vp=vp,
+                     vs=(vp / rng.uniform(1.6, 2.0, (nz, nx))).astype(np.float32),
+                     rho=rng.uniform(1800, 2600, (nz, nx)).astype(np.float32),
+                     qp=np.full_like(vp, 100.0), qs=np.full_like(vp, 50.0))
+    spec = PropagationSpec(dt=3e-4, nt=60, order=8, npml=int(rng.integers(6, 14)),
+                           boundary=boundary)
+    src = SourceSpec(x=float(rng.uniform(0.1, 0.9) * (nx - 1) * 5.0),
+                     z=float(rng.uniform(0.1, 0.9) * (nz - 1) * 4.0), f0=30.0, delay=0.01,
+                     mechanism=mech)
+    w = (np.sin(np.arange(60) * 0.4) * 1e5).astype(np.float32)
+    shapes = [(0, 64), (1, 16), (2, 8), (2, 48), (3, 20), (4, 48), (4, 11)]
+    outs = []
+    for v in shapes:
+        p = build_propagator(m, spec, engine="elastic")
+        p.set_source(src)
+        p._batches["step"] = dev.Batch([p._core], variant=v)
+        for s in range(60):
+            p.step(float(w[s]))
+        torch.cuda.synchronize()
+        outs.append(p._core.state[p._core.cur].clone())
+    assert float(outs[0].abs().max()) > 0.0
+    for v, o in zip(shapes[1:], outs[1:]):
+        assert torch.equal(o, outs[0]), v
+
+    # checkpointing forward pass two steps per launch (every built shape)
+    # against one step per launch: final state and every checkpoint
+    interval = 7
+    ref = None
+    for ts in [(0, 0), (2, 16), (2, 24), (2, 48), (2, 37)]:
+        p = build_propagator(m, spec, engine="elastic")
+        p.set_source(src)
+        c = p._core
+        arena = torch.zeros(((60 + interval - 1) // interval, c.state_floats),
+                            dtype=torch.float32, device="cuda")
+        b = dev.Batch([c], variant=(2, 48), two_step=ts)
+        with dev.StepTimer():
+            cur = b.run(0, 0, 60, c.cur, wavelets=[w], ck_interval=interval, ck_origin=0,
+                        save=[arena], save_stride=c.state_floats)
+            torch.cuda.synchronize()
+        kinds = dev.StepTimer.read_kinds()
+        b.close()
+        assert (kinds["plain2"]["launches"] > 0) == (ts[0] != 0), (ts, kinds["plain2"])
+        out = (c.state[cur].clone(), arena)
+        if ref is None:
+            ref = out
+            assert torch.equal(out[0], outs[0])
+            continue
+        assert torch.equal(out[0], ref[0]), (ts, "state")
+        assert torch.equal(out[1], ref[1]), (ts, "checkpoints")
+
+    # migration: replay extraction and receiver ring inside the step
+    modes = (("pp", "ps"), ("sp", "ss"), ("pp", "ps"))[seed % 3]
+    separation = ("divcurl", "divcurl", "stress")[(seed + 1) % 3]
+    bnd = "free" if boundary == "none" else boundary
+    solver = SolverConfig(nt=300, dt=3e-4, order=8, boundary=bnd, engine="elastic", npml=10)
+    ssrc = SourceSpec(x=float(rng.uniform(0.3, 0.7) * (nx - 1) * 5.0), z=12.0, f0=30.0,
+                      delay=0.035)
+    x = np.linspace(0.0, (nx - 1) * 5.0, nx // 4)
+    rec = ReceiverSpec(x=x, z=np.full_like(x, 12.0), component="vz")
+    shot = forward_shot(m, ssrc, rec, PropagationSpec.from_config(solver, m), engine="elastic",
+                        components=("vx", "vz"))
+    per_state = 5 * (nz + 20) * (nx + 20) * 4
+    imaging = ImagingConfig(mode=modes[0], condition="normalized", separation=separation,
+                            checkpoint_mb=3 * per_state / 2**20)
+    ref = RTMImager(imaging, solver, None, GpuConfig(kernel_variant=(0, 64))).migrate_batch(
+        [shot], m, modes=modes)[0]
+    for kv, ts in [((2, 16), (0, 0)), ((4, 48), (0, 0)), ((1, 24), (0, 0)),
+                   ((2, 48), (2, 16)), ((2, 48), (2, 40)), ((4, 24), (2, 48))]:
+        got = RTMImager(imaging, solver, None,
+                        GpuConfig(kernel_variant=kv, image_ring=int(rng.choice([1, 7, 32])),
+                                  two_step=ts)).migrate_batch([shot], m, modes=modes)[0]
+        for md in modes:
+            assert np.array_equal(got[md].image, ref[md].image), (kv, ts, md)
+    assert np.abs(ref[modes[0]].image).max() > 0.0

@@ -47,9 +47,16 @@ __host__ __device__ __forceinline__ int la_cur(const LaunchArgs& la, int k) {
     return la.cur ^ (int)((la.flip >> k) & 1u);
 }
 
-// Time index of replay slot 0 for member k.
+// Time index of replay slot 0 for member k.  Both candidates are read and
+// the values selected, and the result is made wave-uniform: selecting
+// between the two addresses (a kernel argument and the image table) gives a
+// flat load, whose result the compiler treats as per-lane, and the replay
+// slot's buffer descriptor derived from it then went through a waterfall
+// loop around every image store of the fused step.
 __device__ __forceinline__ int la_t0(const LaunchArgs& la, int k, const ErtmImage& I) {
-    return la.remap ? la.t0[k] : I.t0;
+    const int from_lanes = la.t0[k];
+    const int from_image = I.t0;
+    return __builtin_amdgcn_readfirstlane(la.remap ? from_lanes : from_image);
 }
 
 __device__ __forceinline__ int la_time(const LaunchArgs& la, int k) {

@@ -285,6 +285,121 @@ def c5_model(seed=5):
                         qs=f32(np.full((nz, nx), 50.0)))
 
 
+C3_DT, C3_NT, C3_F0 = 3.5e-4, 6000, 30.0
+C3_RECORD_DT = 36e-6
+
+
+def c3_model():
+    """Borehole viscoacoustic workload: 800 x 1600 at 4 m, four flat layers
+    (tops 0/800/1600/2400 m) with vp 2500/3333/4167/5000 m/s and Q
+    30/60/120/200, Gardner density, vs = vp/1.8 (unused by the engine)."""
+    from ertm.types import ElasticModel, GridSpec
+
+    nz, nx, dx = 800, 1600, 4.0
+    z = np.arange(nz)[:, None] * dx * np.ones((1, nx))
+    vp = np.full((nz, nx), 2500.0)
+    qp = np.full((nz, nx), 30.0)
+    for top, v, q in ((800.0, 3333.0, 60.0), (1600.0, 4167.0, 120.0), (2400.0, 5000.0, 200.0)):
+        vp[z >= top] = v
+        qp[z >= top] = q
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    return ElasticModel(grid=GridSpec(nx=nx, nz=nz, dx=dx, dz=dx), vp=f32(vp),
+                        vs=f32(vp / 1.8), rho=f32(310.0 * vp ** 0.25), qp=f32(qp),
+                        qs=f32(np.full((nz, nx), 50.0)))
+
+
+def c3_configs(nt=C3_NT):
+    """Solver (viscoacoustic, CPML) and GSLS Q options of the borehole
+    workload: 3 mechanisms fitted over 0.67-1.5 x 30 Hz, compensation with
+    alpha_c = 1, a 60 Hz band limit, a 40 dB gain budget, amplitude cap 50
+    and a wavenumber low-pass every 20 steps."""
+    from ertm.config import QConfig, SolverConfig
+
+    solver = SolverConfig(nt=nt, dt=C3_DT, order=8, boundary="cpml", engine="viscoacoustic",
+                          npml=20)
+    q = QConfig(enabled=True, model="gsls", n_mechanisms=3, f_ref=C3_F0, band_ratio=1.5,
+                alpha_c=1.0, f_max=60.0, gain_budget_db=40.0, amplitude_cap=50.0,
+                lowpass_interval=20)
+    return solver, q
+
+
+def c3_acquisition(n=32):
+    """Sources every 200 m from x = 100 m at z = 8 m (Ricker 30 Hz, delay
+    1.2/f0); a vertical well at x = 3200 m with 200 receivers from z = 1000 m
+    every 4 m; one sensor azimuth per shot, uniform in [0, 360) from seed 3,
+    mapped to [-180, 180)."""
+    from ertm.types import ReceiverSpec, SourceSpec
+
+    sources = [SourceSpec(x=100.0 + 200.0 * i, z=8.0, f0=C3_F0, delay=1.2 / C3_F0)
+               for i in range(n)]
+    zr = 1000.0 + 4.0 * np.arange(200)
+    receivers = ReceiverSpec(x=np.full(200, 3200.0), z=zr)
+    az = np.random.default_rng(3).uniform(0.0, 360.0, n)
+    az = ((az + 180.0) % 360.0) - 180.0
+    return sources, receivers, [float(a) for a in az]
+
+
+def c3_records(model, sources, receivers, azimuths, nt=C3_NT):
+    """Records (p, vx, vz) modelled on the GPU in one batched viscoacoustic
+    forward pass (no compensation), then per shot as the workload's field
+    data arrive: resampled to 36 us, the horizontals rotated by the sensor
+    azimuth, the direct arrival muted (v_ref = 25th percentile of vp) and the
+    ends tapered (``ertm.preprocess.prepare_shot``)."""
+    from ertm.config import GpuConfig
+    from ertm.preprocess import prepare_shot
+    from ertm.solver import build_solver
+
+    solver, q = c3_configs(nt)
+    fwd = build_solver(solver, q, gpu_config=GpuConfig(strict_cfl=False))
+    shots = fwd.forward_batch(model, sources, receivers, components=("p", "vx", "vz"))
+    vref = float(np.percentile(model.vp, 25))
+    return [prepare_shot(s, model=model, record_dt=C3_RECORD_DT, rotation_deg=a, mute=True,
+                         v_reference=vref) for s, a in zip(shots, azimuths)]
+
+
+def c3_bench(nshots, nt, batches=2):
+    """Borehole viscoacoustic RTM: ``nshots`` shots migrated in one batch
+    (normalized condition, Q compensation in the backward pass, records
+    aligned from 36 us to the solver's dt on the device); one JSON line with
+    shots/s, RTM cell-updates/s (3 nt nz nx per shot) and the fraction of the
+    8 TB/s roofline at 68 algorithmic bytes per viscoacoustic cell-update
+    (L = 3).  Records are modelled and prepared before timing."""
+    import torch
+    from ertm.config import GpuConfig, ImagingConfig
+    from ertm.imaging import RTMImager
+
+    model = c3_model()
+    sources, receivers, az = c3_acquisition(nshots)
+    t0 = time.perf_counter()
+    shots = c3_records(model, sources, receivers, az, nt)
+    print(f"# c3: {nshots} records modelled and prepared in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    solver, q = c3_configs(nt)
+    imager = RTMImager(ImagingConfig(mode="pp", condition="normalized"), solver, q,
+                       GpuConfig(strict_cfl=False, replay_budget_gb=64.0))
+    imager.migrate_batch(shots[:2], model)   # warm-up: tuning, allocations
+    times = []
+    for _ in range(batches):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = imager.migrate_batch(shots, model)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    secs = min(times)
+    nz, nx = model.shape
+    cells = 3.0 * nt * nz * nx * nshots
+    line = {"workload": "viscoacoustic borehole RTM (GSLS L=3, Q compensation), batched",
+            "grid": [nz, nx], "dx": 4.0, "nt": nt, "dt": C3_DT, "shots": nshots,
+            "record_dt": C3_RECORD_DT, "seconds_per_batch": round(secs, 3),
+            "all_batches_s": [round(t, 3) for t in times],
+            "shots_per_s": round(nshots / secs, 4),
+            "rtm_gcell_updates_per_s": round(cells / secs / 1e9, 2),
+            "roofline_frac": round(cells * 68 / secs / (HBM_PEAK_GBS * 1e9), 4),
+            "finite": bool(all(np.isfinite(o["pp"].image).all() for o in out))}
+    print(json.dumps(line), flush=True)
+    return line
+
+
 def c5_sweep(budgets_mb, nt, kernel_variant=None, conditions=("zero_lag", "excitation"),
              segment_check=True, two_step=None):
     """Excitation vs zero_lag imaging of the large shot at several checkpoint
@@ -616,6 +731,12 @@ def main(argv=None) -> int:
                     help="instead of the batched benchmark: the large single elastic shot "
                          "(2000 x 6000, nt 10000), excitation vs zero_lag at checkpoint "
                          "budgets 1/4/16/64 GB, one JSON line per run")
+    ap.add_argument("--c3", action="store_true",
+                    help="instead of the batched benchmark: the borehole viscoacoustic workload "
+                         "(32 shots of 800 x 1600, nt 6000, GSLS L=3 with Q compensation), one "
+                         "JSON line")
+    ap.add_argument("--c3-shots", type=int, default=32)
+    ap.add_argument("--c3-nt", type=int, default=C3_NT)
     ap.add_argument("--c5-nt", type=int, default=10000)
     ap.add_argument("--c5-budgets", default="1024,4096,16384,65536",
                     help="--c5-sweep: checkpoint budgets (MB)")
@@ -657,7 +778,7 @@ def main(argv=None) -> int:
 
     pool_baseline = None
     if (not args.no_cpu_baseline and not args.c5_sweep and not args.engine_sweep
-            and not args.serve_sweep and not args.step_shapes
+            and not args.serve_sweep and not args.step_shapes and not args.c3
             and torch.cuda.device_count() > 0):
         # before any HIP call: the pool forks
         print(f"# cpu baseline (process pool): oracle, nt={args.cpu_sample_nt}",
@@ -683,6 +804,9 @@ def main(argv=None) -> int:
         return 0
     if args.engine_sweep:
         engine_sweep(args.shots, args.sweep_steps)
+        return 0
+    if args.c3:
+        c3_bench(args.c3_shots, args.c3_nt)
         return 0
     if args.step_shapes:
         shapes = [tuple(int(v) for v in t.split(",")) for t in args.step_shapes.split(";")]

@@ -879,16 +879,14 @@ static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int
     if (n > ERTM_MAX_LAUNCH) return fail(ERTM_EINVAL, "stabilize: batch larger than %d", ERTM_MAX_LAUNCH);
     float* peaks = b->scratch;
     float* scales = b->scratch + n;
-    for (int k = 0; k < n; ++k) {
-        const ErtmProp& P = b->props[k];
-        launch_absmax(P.state[cur] + (int64_t)ERTM_VI_P * P.fstride, (int64_t)P.nzp * P.ld,
-                      peaks + k, s);
-        LAUNCHCHK("absmax");
-    }
     LaunchArgs la{};
     la.base = 0;
     la.n = n;
     la.cur = cur;
+    int64_t max_n = 0;
+    for (const ErtmProp& P : b->props) max_n = std::max<int64_t>(max_n, (int64_t)P.nzp * P.ld);
+    launch_absmax_batch(b->dprops, la, ERTM_VI_P, max_n, peaks, s);
+    LAUNCHCHK("absmax");
     const bool lowpass_now = step_index >= 30 && interval > 1 && step_index % interval == 0;
     launch_stab_control(b->dprops, la, peaks, ctl, step_index, cap, lowpass_now ? 1 : 0, scales, s);
     LAUNCHCHK("stab_control");

@@ -298,6 +298,55 @@ void launch_absmax(const float* x, int64_t n, float* out, hipStream_t s) {
                        (unsigned int*)out);
 }
 
+// max |field| of every member of a launch in one launch: member k's padded
+// field slab (nzp rows of ld floats, as k_absmax reads it) reduces into
+// out[k] (float bits, which order as unsigned integers for |x| and put NaN
+// above infinity); out[0 .. la.n) is zeroed by k_zero_n first.
+__global__ void k_zero_n(unsigned int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void k_absmax_batch(const ErtmProp* __restrict__ props, LaunchArgs la, int field,
+                               unsigned int* out) {
+    const int k = blockIdx.y;
+    const ErtmProp& P = props[la_prop(la, k)];
+    const float* x = P.state[la_cur(la, k)] + (int64_t)field * P.fstride;
+    const int64_t n = (int64_t)P.nzp * P.ld;
+    float m = 0.0f;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        float v = fabsf(x[q]);
+        m = (v > m || v != v) ? v : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        float o = __shfl_down(m, off, 64);
+        m = (o > m || o != o) ? o : m;
+    }
+    __shared__ float red[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = (red[w] > r || red[w] != red[w]) ? red[w] : r;
+        atomicMax(out + k, __float_as_uint(r));
+    }
+}
+
+void launch_absmax_batch(const ErtmProp* dprops, const LaunchArgs& la, int field, int64_t max_n,
+                         float* out, hipStream_t s) {
+    if (la.n <= 0) return;
+    hipLaunchKernelGGL(k_zero_n, dim3((la.n + 255) / 256), dim3(256), 0, s, (unsigned int*)out,
+                       la.n);
+    // about 2048 blocks over the launch, at least 8 per member
+    int64_t per = (max_n + 1023) / 1024;
+    per = std::min<int64_t>(per, std::max<int64_t>(8, 2048 / la.n));
+    if (per < 1) per = 1;
+    hipLaunchKernelGGL(k_absmax_batch, dim3((unsigned)per, la.n), dim3(1024), 0, s, dprops, la,
+                       field, (unsigned int*)out);
+}
+
 // ---------------------------------------------------------------------------
 // standalone staggered derivative of a dense array (ertm/numerics.py:68-137)
 // ---------------------------------------------------------------------------

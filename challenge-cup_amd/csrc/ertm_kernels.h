@@ -116,6 +116,9 @@ void launch_image_flush(const ErtmProp* dprops, const ErtmImage* dimg, const Lau
 void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,
                   int max_cells, hipStream_t s);
 void launch_absmax(const float* x, int64_t n, float* out, hipStream_t s);
+// out[k] = max |field slab| of launch member k (nzp * ld floats), one launch
+void launch_absmax_batch(const ErtmProp* dprops, const LaunchArgs& la, int field, int64_t max_n,
+                         float* out, hipStream_t s);
 // device clock: advance clk[0] by `steps` (after a replayed chunk)
 void launch_tick(int32_t* clk, int steps, hipStream_t s);
 // checkpoint save inside a captured loop: state[cur] of every propagator of

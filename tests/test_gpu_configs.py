@@ -12,6 +12,18 @@
   checkpoint blocks, against the oracle bit for bit; and a 400-step
   migration whose blocks are replayed in segments from sub-checkpoints
   equals whole-block replay bit for bit.
+* Borehole viscoacoustic workload (800 x 1600 at 4 m, four layers with Q
+  30-200, GSLS L = 3), nt truncated to 400: the uncompensated forward
+  wavefield of one of its shots equals the oracle's bit for bit; two shots'
+  records (p, vx, vz), modelled on the GPU plus seeded noise (the first
+  arrivals reach the well only after ~1200 steps, and the direct-wave mute
+  zeroes the earlier samples), resampled to 36 us, rotated by the sensor
+  azimuths and muted, are migrated as one batch with Q compensation in the
+  backward pass (normalized condition, records aligned back to the solver's
+  dt on the device, several checkpoint blocks); the first image agrees with
+  the oracle's within the compensated viscoacoustic tolerance (relative L2
+  1e-4: the stabiliser's float64 FFT low-pass is hipFFT here, pocketfft in
+  NumPy).
 * Serving workload at its real grid bounds (256 x 256 to 2000 x 4000, dx
   2.5-10 m, acoustic and elastic, nt capped at 1500, the bottom of its
   range): requests submitted together to the serving loop; each image equals
@@ -95,6 +107,85 @@ def test_c2_shape_batched_pp_ps_matches_oracle():
             assert np.abs(ref).max() > 0
             assert got.meta["checkpoints"]["interval"] == 75
             assert np.array_equal(got.image, ref), (k, mode)
+
+
+# compensated viscoacoustic images (hipFFT against pocketfft in the stabiliser)
+VISCO_COMP_TOL_IMAGE = 1e-4
+
+
+def test_borehole_visco_shape_matches_oracle():
+    import bench
+    from ertm.config import GpuConfig, ImagingConfig
+    from ertm.imaging import RTMImager
+    from ertm.preprocess import prepare_shot
+    from ertm.propagation import PropagationSpec, build_propagator
+    from ertm.solver import build_solver
+    from oracle import rtm as ortm
+    from product_cases import rel_l2
+
+    nt = 400
+    _say("borehole viscoacoustic workload: 2 shots, nt", nt)
+    model = bench.c3_model()
+    sources, receivers, az = bench.c3_acquisition(32)
+    sources, az = [sources[15], sources[16]], [az[15], az[16]]
+    solver, q = bench.c3_configs(nt)
+    g = model.grid
+    om = ortm.Model(g.nz, g.nx, g.dx, g.dz, model.vp, model.vs, model.rho, model.qp, g.x0, g.z0)
+    qs = ortm.QSpec(f_ref=q.f_ref, n=q.n_mechanisms, band_ratio=q.band_ratio,
+                    alpha_c=q.alpha_c, amplitude_cap=q.amplitude_cap,
+                    lowpass_interval=q.lowpass_interval, f_max=q.f_max,
+                    gain_budget_db=q.gain_budget_db)
+    # forward modelling (no compensation): the whole wavefield after nt steps
+    # against the oracle's, bit for bit (the wave has not reached the well yet)
+    fwd = build_solver(solver, q, gpu_config=GpuConfig(strict_cfl=False))
+    spec = PropagationSpec(dt=solver.dt, nt=nt, order=8, npml=solver.npml, boundary="cpml")
+    prop = build_propagator(model, spec, engine="viscoacoustic", q_options=fwd.q_options())
+    prop.set_source(sources[0])
+    op = ortm.build("viscoacoustic", om, solver.dt, nt, 8, solver.npml, "cpml", q=qs)
+    op.set_source(sources[0].x, sources[0].z)
+    w = ortm.wavelet("ricker", sources[0].f0, sources[0].delay, 1.0, solver.dt, nt)
+    for k in range(nt):
+        prop.step(float(w[k]))
+        op.step(float(w[k]))
+    for f in ("p", "vx", "vz"):
+        assert np.array_equal(getattr(prop, f), getattr(op, f)), f
+    assert float(np.abs(op.p).max()) > 0
+    # migration inputs: the modelled records plus seeded noise (so that data
+    # survive the direct-wave mute), resampled to 36 us, rotated by the
+    # sensor azimuths, muted and tapered
+    raw = fwd.forward_batch(model, sources, receivers, components=("p", "vx", "vz"))
+    rng = np.random.default_rng(33)
+    for r in raw:
+        for c in r.components:
+            r.components[c] = (r.components[c]
+                               + rng.normal(0.0, 1e-6, r.components[c].shape)).astype(np.float32)
+    vref = float(np.percentile(model.vp, 25))
+    shots = [prepare_shot(s, model=model, record_dt=bench.C3_RECORD_DT, rotation_deg=a,
+                          mute=True, v_reference=vref) for s, a in zip(raw, az)]
+    imaging = ImagingConfig(mode="pp", condition="normalized")
+    out = RTMImager(imaging, solver, q, GpuConfig(strict_cfl=False)).migrate_batch(shots, model)
+    cfg = ortm.Imaging(mode="pp", condition="normalized", separation=imaging.separation,
+                       illumination=imaging.illumination,
+                       receiver_normalization=imaging.receiver_normalization,
+                       laplacian=imaging.laplacian, lowpass_fraction=imaging.lowpass_fraction,
+                       checkpoint_mb=imaging.checkpoint_mb, eps=imaging.eps,
+                       start_periods=imaging.start_periods)
+    shot = shots[0]
+    s = shot.source
+    src = dict(x=s.x, z=s.z, f0=s.f0, delay=s.delay, amplitude=s.amplitude, kind=s.kind,
+               mechanism=s.mechanism)
+    ref, info = ortm.migrate("viscoacoustic", om, solver.dt, nt, 8, "cpml", cfg,
+                             dict(shot.components), src, shot.receivers.x, shot.receivers.z,
+                             q=qs, q_enabled=True, rec_dt=shot.dt)
+    got = out[0]["pp"]
+    err = rel_l2(got.image, ref)
+    _say("azimuth", round(az[0], 2), "blocks", info["n_states"], "max|image|",
+         float(np.abs(ref).max()), "rel L2", err)
+    assert info["n_states"] >= 2
+    assert np.abs(ref).max() > 0
+    assert got.meta["checkpoints"]["interval"] == info["interval"]
+    assert err <= VISCO_COMP_TOL_IMAGE, err
+    assert np.isfinite(out[1]["pp"].image).all() and np.abs(out[1]["pp"].image).max() > 0
 
 
 def _c5_shot(nt, delay):

@@ -50,9 +50,12 @@ int fail(int code, const char* fmt, ...) {
             return fail(ERTM_EHIP, "%s launch failed: %s", what, hipGetErrorString(e_)); \
     } while (0)
 
+// Batched real-to-complex plans and buffers of the stabiliser's low-pass,
+// one per (nzp, nxp, members of the group).
 struct FftCache {
-    hipfftHandle plan = 0;
-    void* buf = nullptr;
+    hipfftHandle fwd = 0, inv = 0;
+    double* real = nullptr;
+    void* cplx = nullptr;
 };
 
 // Descriptor tables come from a process-wide pool of device blocks that are
@@ -170,7 +173,7 @@ struct ertm_batch_s {
     void* block = nullptr;     // pool block holding dprops, drecv, dimg, scratch
     size_t block_bytes = 0;
     std::vector<hipStream_t> streams;  // streams this batch has enqueued work on
-    std::map<std::pair<int, int>, FftCache> fft;
+    std::map<std::tuple<int, int, int>, FftCache> fft;
     // captured time loops (ertm_run with graph_steps > 0): device clock
     // {step, t0, dir}, amplitude table, checkpoint arena pointers, and the
     // instantiated graphs keyed by everything their launches bake in
@@ -194,8 +197,10 @@ struct ertm_batch_s {
         pool_put(block, block_bytes);
         pool_put(psi1, psi1_bytes);
         for (auto& kv : fft) {
-            if (kv.second.plan) hipfftDestroy(kv.second.plan);
-            if (kv.second.buf) (void)hipFree(kv.second.buf);
+            if (kv.second.fwd) hipfftDestroy(kv.second.fwd);
+            if (kv.second.inv) hipfftDestroy(kv.second.inv);
+            if (kv.second.real) (void)hipFree(kv.second.real);
+            if (kv.second.cplx) (void)hipFree(kv.second.cplx);
         }
     }
 };
@@ -894,33 +899,61 @@ static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int
     launch_stab_scale(b->dprops, la, scales, b->max_tiles, s);
     LAUNCHCHK("stab_scale");
     if (lowpass_now) {
-        for (int k = 0; k < n; ++k) {
-            const ErtmProp& P = b->props[k];
-            auto key = std::make_pair(P.nzp, P.nxp);
-            FftCache& fc = b->fft[key];
-            if (!fc.plan) {
-                if (hipfftPlan2d(&fc.plan, P.nzp, P.nxp, HIPFFT_Z2Z) != HIPFFT_SUCCESS)
-                    return fail(ERTM_EHIP, "hipfftPlan2d(%d,%d) failed", P.nzp, P.nxp);
-                HIPCHK(hipMalloc(&fc.buf, sizeof(double) * 2 * (size_t)P.nzp * P.nxp));
-            }
-            if (hipfftSetStream(fc.plan, s) != HIPFFT_SUCCESS)
-                return fail(ERTM_EHIP, "hipfftSetStream failed");
-            const double time = step_index * P.dt64;
-            double ratio = 0.9 - 0.4 * std::min(time / 1.0, 1.0);
-            ratio = std::max(ratio, 0.35);
-            const int fields[2] = {ERTM_VI_P, ERTM_VI_EPS};
-            for (int f : fields) {
-                float* x = P.state[cur] + (int64_t)f * P.fstride;
-                launch_lowpass_prep(x, P.nzp, P.nxp, P.ld, fc.buf, s);
+        // members grouped by padded shape, in groups of at most ERTM_LP_GROUP;
+        // each group is one batched D2Z / Z2D pair over its p and eps fields
+        std::map<std::pair<int, int>, std::vector<int>> shapes;
+        for (int k = 0; k < n; ++k) shapes[{b->props[k].nzp, b->props[k].nxp}].push_back(k);
+        for (const auto& sh : shapes) {
+            const int nzp = sh.first.first, nxp = sh.first.second, nh = nxp / 2 + 1;
+            const std::vector<int>& ks = sh.second;
+            for (size_t g0 = 0; g0 < ks.size(); g0 += ERTM_LP_GROUP) {
+                const int m = (int)std::min<size_t>(ERTM_LP_GROUP, ks.size() - g0);
+                FftCache& fc = b->fft[std::make_tuple(nzp, nxp, m)];
+                if (!fc.inv) {
+                    if ((int64_t)nzp * nxp > INT_MAX)
+                        return fail(ERTM_EINVAL, "lowpass: padded grid %dx%d too large", nzp, nxp);
+                    int dims[2] = {nzp, nxp}, remb[2] = {nzp, nxp}, cemb[2] = {nzp, nh};
+                    const int rdist = nzp * nxp, cdist = nzp * nh;
+                    if (!fc.real) HIPCHK(hipMalloc(&fc.real, sizeof(double) * 2 * (size_t)m * rdist));
+                    if (!fc.cplx)
+                        HIPCHK(hipMalloc(&fc.cplx, 2 * sizeof(double) * 2 * (size_t)m * cdist));
+                    if (!fc.fwd && hipfftPlanMany(&fc.fwd, 2, dims, remb, 1, rdist, cemb, 1, cdist,
+                                                  HIPFFT_D2Z, 2 * m) != HIPFFT_SUCCESS) {
+                        fc.fwd = 0;
+                        return fail(ERTM_EHIP, "hipfftPlanMany D2Z (%d,%d) x %d failed", nzp, nxp,
+                                    2 * m);
+                    }
+                    if (hipfftPlanMany(&fc.inv, 2, dims, cemb, 1, cdist, remb, 1, rdist,
+                                       HIPFFT_Z2D, 2 * m) != HIPFFT_SUCCESS) {
+                        fc.inv = 0;
+                        return fail(ERTM_EHIP, "hipfftPlanMany Z2D (%d,%d) x %d failed", nzp, nxp,
+                                    2 * m);
+                    }
+                }
+                if (hipfftSetStream(fc.fwd, s) != HIPFFT_SUCCESS ||
+                    hipfftSetStream(fc.inv, s) != HIPFFT_SUCCESS)
+                    return fail(ERTM_EHIP, "hipfftSetStream failed");
+                LpArgs a{};
+                a.n = m;
+                a.nzp = nzp;
+                a.nxp = nxp;
+                a.cur = cur;
+                for (int i = 0; i < m; ++i) {
+                    const int k = ks[g0 + i];
+                    const double time = step_index * b->props[k].dt64;
+                    double ratio = 0.9 - 0.4 * std::min(time / 1.0, 1.0);
+                    a.prop[i] = k;
+                    a.ratio[i] = std::max(ratio, 0.35);
+                }
+                launch_lowpass_prep(b->dprops, a, fc.real, s);
                 LAUNCHCHK("lowpass_prep");
-                auto* c = (hipfftDoubleComplex*)fc.buf;
-                if (hipfftExecZ2Z(fc.plan, c, c, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
-                    return fail(ERTM_EHIP, "hipfftExecZ2Z forward failed");
-                launch_lowpass_response(fc.buf, P.nzp, P.nxp, (double)P.dx, (double)P.dz, ratio, s);
+                if (hipfftExecD2Z(fc.fwd, fc.real, (hipfftDoubleComplex*)fc.cplx) != HIPFFT_SUCCESS)
+                    return fail(ERTM_EHIP, "hipfftExecD2Z failed");
+                launch_lowpass_response(b->dprops, a, fc.cplx, s);
                 LAUNCHCHK("lowpass_response");
-                if (hipfftExecZ2Z(fc.plan, c, c, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
-                    return fail(ERTM_EHIP, "hipfftExecZ2Z backward failed");
-                launch_lowpass_finish(x, P.nzp, P.nxp, P.ld, fc.buf, s);
+                if (hipfftExecZ2D(fc.inv, (hipfftDoubleComplex*)fc.cplx, fc.real) != HIPFFT_SUCCESS)
+                    return fail(ERTM_EHIP, "hipfftExecZ2D failed");
+                launch_lowpass_finish(b->dprops, a, fc.real, s);
                 LAUNCHCHK("lowpass_finish");
             }
         }

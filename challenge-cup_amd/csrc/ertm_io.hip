@@ -453,15 +453,30 @@ void launch_stab_scale(const ErtmProp* dprops, const LaunchArgs& la, const float
     hipLaunchKernelGGL(k_stab_scale, dim3(512, la.n), dim3(256), 0, s, dprops, la, scales);
 }
 
-// k-space low-pass of a padded field (ertm/numerics.py:424-442): the field is
-// widened to complex float64, transformed with hipFFT by the caller, weighted
-// by the radial ramp, transformed back and narrowed with the 1/N factor.
-__global__ void k_lp_prep(const float* __restrict__ f, int nzp, int nxp, int64_t ld,
-                          double2* __restrict__ c) {
+// k-space low-pass of padded fields (ertm/numerics.py:424-442), batched over
+// the p and eps fields of every member of an LpArgs group (transform y =
+// 2 * member + field, fields of one shape).  The input is real and the radial
+// ramp is even in kz and kx, so the filtered spectrum is Hermitian and the
+// real part of the full inverse transform equals the real-to-complex round
+// trip: each field is widened to float64, transformed by the caller with a
+// D2Z plan, weighted on the half spectrum (nxp / 2 + 1 columns), transformed
+// back with a Z2D plan and narrowed with the 1/N factor.
+__device__ __forceinline__ const ErtmProp& lp_prop(const ErtmProp* props, const LpArgs& a, int y) {
+    return props[a.prop[y >> 1]];
+}
+
+__device__ __forceinline__ float* lp_field(const ErtmProp& P, const LpArgs& a, int y) {
+    return P.state[a.cur] + (int64_t)((y & 1) ? ERTM_VI_EPS : ERTM_VI_P) * P.fstride;
+}
+
+__global__ void k_lp_prep(const ErtmProp* __restrict__ props, LpArgs a, double* __restrict__ r) {
+    const int y = blockIdx.y;
+    const int64_t cells = (int64_t)a.nzp * a.nxp;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)nzp * nxp) return;
-    const int i = (int)(q / nxp), j = (int)(q - (int64_t)i * nxp);
-    c[q] = make_double2((double)f[(int64_t)i * ld + j], 0.0);
+    if (q >= cells) return;
+    const ErtmProp& P = lp_prop(props, a, y);
+    const int i = (int)(q / a.nxp), j = (int)(q - (int64_t)i * a.nxp);
+    r[(int64_t)y * cells + q] = (double)lp_field(P, a, y)[(int64_t)i * P.ld + j];
 }
 
 __device__ __forceinline__ double fftfreq(int k, int n, double d) {
@@ -470,50 +485,59 @@ __device__ __forceinline__ double fftfreq(int k, int n, double d) {
     return (k < half ? (double)k : (double)(k - n)) * v;
 }
 
-__global__ void k_lp_response(double2* __restrict__ c, int nzp, int nxp, double dx, double dz,
-                              double ratio) {
+__global__ void k_lp_response(const ErtmProp* __restrict__ props, LpArgs a,
+                              double2* __restrict__ c) {
+    const int y = blockIdx.y;
+    const int nh = a.nxp / 2 + 1;
+    const int64_t cells = (int64_t)a.nzp * nh;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)nzp * nxp) return;
-    const int i = (int)(q / nxp), j = (int)(q - (int64_t)i * nxp);
+    if (q >= cells) return;
+    const ErtmProp& P = lp_prop(props, a, y);
+    const int i = (int)(q / nh), j = (int)(q - (int64_t)i * nh);
+    const double dx = (double)P.dx, dz = (double)P.dz;
     double kzm = 0.5 / dz, kxm = 0.5 / dx;
     kzm = kzm > 1e-12 ? kzm : 1e-12;
     kxm = kxm > 1e-12 ? kxm : 1e-12;
-    const double kz = fftfreq(i, nzp, dz) / kzm;
-    const double kx = fftfreq(j, nxp, dx) / kxm;
-    const double r = sqrt(kz * kz + kx * kx);
-    double w = 1.0 - (r - ratio) / 0.15;
+    // column j of the half spectrum is frequency index j of the full one
+    const double kz = fftfreq(i, a.nzp, dz) / kzm;
+    const double kx = fftfreq(j, a.nxp, dx) / kxm;
+    const double rr = sqrt(kz * kz + kx * kx);
+    double w = 1.0 - (rr - a.ratio[y >> 1]) / 0.15;
     w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
-    c[q].x *= w;
-    c[q].y *= w;
+    double2* v = c + (int64_t)y * cells + q;
+    v->x *= w;
+    v->y *= w;
 }
 
-__global__ void k_lp_finish(float* __restrict__ f, int nzp, int nxp, int64_t ld,
-                            const double2* __restrict__ c) {
+__global__ void k_lp_finish(const ErtmProp* __restrict__ props, LpArgs a,
+                            const double* __restrict__ r) {
+    const int y = blockIdx.y;
+    const int64_t cells = (int64_t)a.nzp * a.nxp;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)nzp * nxp) return;
-    const int i = (int)(q / nxp), j = (int)(q - (int64_t)i * nxp);
-    f[(int64_t)i * ld + j] = (float)(c[q].x / ((double)nzp * (double)nxp));
+    if (q >= cells) return;
+    const ErtmProp& P = lp_prop(props, a, y);
+    const int i = (int)(q / a.nxp), j = (int)(q - (int64_t)i * a.nxp);
+    lp_field(P, a, y)[(int64_t)i * P.ld + j] =
+        (float)(r[(int64_t)y * cells + q] / ((double)a.nzp * (double)a.nxp));
 }
 
-void launch_lowpass_prep(const float* field, int nzp, int nxp, int64_t ld, void* cbuf,
-                         hipStream_t s) {
-    const int64_t n = (int64_t)nzp * nxp;
-    hipLaunchKernelGGL(k_lp_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, field, nzp,
-                       nxp, ld, (double2*)cbuf);
+void launch_lowpass_prep(const ErtmProp* dprops, const LpArgs& a, double* real, hipStream_t s) {
+    const int64_t n = (int64_t)a.nzp * a.nxp;
+    hipLaunchKernelGGL(k_lp_prep, dim3((unsigned)((n + 255) / 256), 2 * a.n), dim3(256), 0, s,
+                       dprops, a, real);
 }
 
-void launch_lowpass_response(void* cbuf, int nzp, int nxp, double dx, double dz, double ratio,
-                             hipStream_t s) {
-    const int64_t n = (int64_t)nzp * nxp;
-    hipLaunchKernelGGL(k_lp_response, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       (double2*)cbuf, nzp, nxp, dx, dz, ratio);
+void launch_lowpass_response(const ErtmProp* dprops, const LpArgs& a, void* cplx, hipStream_t s) {
+    const int64_t n = (int64_t)a.nzp * (a.nxp / 2 + 1);
+    hipLaunchKernelGGL(k_lp_response, dim3((unsigned)((n + 255) / 256), 2 * a.n), dim3(256), 0, s,
+                       dprops, a, (double2*)cplx);
 }
 
-void launch_lowpass_finish(float* field, int nzp, int nxp, int64_t ld, const void* cbuf,
+void launch_lowpass_finish(const ErtmProp* dprops, const LpArgs& a, const double* real,
                            hipStream_t s) {
-    const int64_t n = (int64_t)nzp * nxp;
-    hipLaunchKernelGGL(k_lp_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, field, nzp,
-                       nxp, ld, (const double2*)cbuf);
+    const int64_t n = (int64_t)a.nzp * a.nxp;
+    hipLaunchKernelGGL(k_lp_finish, dim3((unsigned)((n + 255) / 256), 2 * a.n), dim3(256), 0, s,
+                       dprops, a, real);
 }
 
 // ---------------------------------------------------------------------------

@@ -144,11 +144,20 @@ void launch_stab_control(const ErtmProp* dprops, const LaunchArgs& la, const flo
                          hipStream_t s);
 void launch_stab_scale(const ErtmProp* dprops, const LaunchArgs& la, const float* scales,
                        int max_tiles, hipStream_t s);
-void launch_lowpass_prep(const float* field, int nzp, int nxp, int64_t ld, void* cbuf,
-                         hipStream_t s);
-void launch_lowpass_response(void* cbuf, int nzp, int nxp, double dx, double dz, double ratio,
-                             hipStream_t s);
-void launch_lowpass_finish(float* field, int nzp, int nxp, int64_t ld, const void* cbuf,
+// A group of the stabiliser's low-pass: members k < n are propagators prop[k]
+// of the batch table, all of padded shape (nzp, nxp), filtered at cutoff
+// ratio[k]; their p and eps fields in state[cur] are transforms 2k and 2k + 1
+// of the batched plans.  Real buffer: 2n * nzp * nxp doubles; complex buffer:
+// 2n * nzp * (nxp / 2 + 1) double2.
+constexpr int ERTM_LP_GROUP = 16;
+struct LpArgs {
+    int32_t n, nzp, nxp, cur;
+    int32_t prop[ERTM_LP_GROUP];
+    double ratio[ERTM_LP_GROUP];
+};
+void launch_lowpass_prep(const ErtmProp* dprops, const LpArgs& a, double* real, hipStream_t s);
+void launch_lowpass_response(const ErtmProp* dprops, const LpArgs& a, void* cplx, hipStream_t s);
+void launch_lowpass_finish(const ErtmProp* dprops, const LpArgs& a, const double* real,
                            hipStream_t s);
 
 }  // namespace ertm

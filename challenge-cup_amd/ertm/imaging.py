@@ -252,6 +252,40 @@ def side_stream(role: str, launch=None):
     return s
 
 
+class _HostTask:
+    """``fn()`` on a host thread, on the caller's device; ``join()`` waits
+    and re-raises what it raised.
+
+    For set-up that overlaps a pass: a pass is one library call, which
+    releases the GIL while it queues its launches, and that call returns
+    only once the GPU's queue has drained to a few hundred launches, so
+    work started after it overlaps only that tail."""
+
+    def __init__(self, fn: Callable[[], None]):
+        import torch
+
+        device = torch.cuda.current_device()
+        self._exc: BaseException | None = None
+
+        def run():
+            try:
+                torch.cuda.set_device(device)
+                fn()
+            except BaseException as e:  # re-raised by join()
+                self._exc = e
+
+        self._thread = threading.Thread(target=run, name="ertm-setup", daemon=True)
+        self._thread.start()
+
+    def wait(self) -> None:
+        self._thread.join()
+
+    def join(self) -> None:
+        self._thread.join()
+        if self._exc is not None:
+            raise self._exc
+
+
 def _host_workers(n_items: int) -> int:
     return max(1, min(n_items, 16, os.cpu_count() or 1))
 
@@ -572,9 +606,20 @@ class RTMImager:
                     backs.append(back)
                     tabs.append(tab)
 
-        stores = record_source_wavefields(fwds, waves, budget_mb=float(self.config.checkpoint_mb),
-                                          variant=kv, graph_steps=gsteps, during=build_backs,
-                                          to_last_checkpoint=True, two_step=ts)
+        setup = None
+        if side.cuda_stream != torch.cuda.current_stream().cuda_stream:
+            # the set-up has a stream of its own: run it beside the whole
+            # forward pass, from before its launch
+            setup = _HostTask(build_backs)
+        try:
+            stores = record_source_wavefields(
+                fwds, waves, budget_mb=float(self.config.checkpoint_mb), variant=kv,
+                graph_steps=gsteps, during=setup.join if setup else build_backs,
+                to_last_checkpoint=True, two_step=ts)
+        except BaseException:
+            if setup is not None:
+                setup.wait()
+            raise
         torch.cuda.current_stream().wait_stream(side)
         starts = []
         for shot, spec in zip(shots, specs):

@@ -143,6 +143,29 @@ def checkpoint_interval(nt: int, per_state: int, budget_mb: float) -> int:
     return max(1, int(np.ceil(nt / max_states)))
 
 
+def replay_segments(s0: int, s1: int, slots: int) -> list[tuple[int, int]]:
+    """Block [s0, s1) split into replay segments of at most ``slots`` steps,
+    the short one (if any) first.
+
+    A block longer than its replay slots is replayed segment by segment,
+    last segment first, each from a sub-checkpoint that one pass from s0 to
+    the start of the last segment saves.  That pass is the overhead: with
+    the short segment first it runs s1 - s0 - slots steps, with the short
+    segment last up to s1 - s0 - 1.  Later segments all hold ``slots``
+    steps, so their sub-checkpoints are ``slots`` steps apart, the first
+    just before the second segment (``sub_checkpoint_origin``)."""
+    slots = max(1, int(slots))
+    first = (s1 - s0) % slots or slots
+    bounds = [s0] + list(range(s0 + first, s1 + 1, slots))
+    return list(zip(bounds[:-1], bounds[1:]))
+
+
+def sub_checkpoint_origin(segs: list[tuple[int, int]]) -> int:
+    """The step whose state the first sub-checkpoint holds (the state just
+    before the second segment's first step)."""
+    return segs[1][0] - 1
+
+
 def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
                             budget_mb: float) -> CheckpointStore:
     """Forward-propagate and checkpoint into an HBM arena (ertm/imaging.py:113-137).
@@ -685,11 +708,12 @@ class RTMImager:
             for blk in range(nblocks - 1, -1, -1):
                 s0, s1 = store0.bounds(blk, nt)
                 slot = store0.states[blk][1]
-                segs = [(a, min(a + slots, s1)) for a in range(s0, s1, slots)]
+                segs = replay_segments(s0, s1, slots)
                 if len(segs) > 1:
                     # sub-checkpoints: replay state just before w[a] of each later segment
                     fcur = fb.run(1, s0, segs[-1][0], fcur, wavelets=ws, ck_interval=slots,
-                                  ck_origin=s0 + slots - 1, save=subs, save_stride=astride,
+                                  ck_origin=sub_checkpoint_origin(segs), save=subs,
+                                  save_stride=astride,
                                   restore=arenas, restore_stride=astride, restore_slot=slot)
                     replayed += segs[-1][0] - s0
                 for k in range(len(segs) - 1, -1, -1):

@@ -35,7 +35,8 @@ import numpy as np
 from . import _lib as L
 from . import device as dev
 from .exceptions import ImagingError
-from .imaging import CheckpointStore, PreparedBatch, RTMImager, checkpoint_interval
+from .imaging import (CheckpointStore, PreparedBatch, RTMImager, checkpoint_interval,
+                      replay_segments, sub_checkpoint_origin)
 from .propagation import build_propagator
 from .types import ImageResult
 
@@ -150,11 +151,11 @@ class Lane:
         for blk in range(len(st.states) - 1, -1, -1):
             s0, s1 = st.bounds(blk, nt)
             slot = st.states[blk][1]
-            segs = [(a, min(a + slots, s1)) for a in range(s0, s1, slots)]
+            segs = replay_segments(s0, s1, slots)
             if len(segs) > 1:
                 # sub-checkpoints: the state just before w[a] of each later segment
-                prog.append((1, s0, segs[-1][0], -1, 0, slots, s0 + slots - 1, (self.subs, 0),
-                             (self.arena, slot)))
+                prog.append((1, s0, segs[-1][0], -1, 0, slots, sub_checkpoint_origin(segs),
+                             (self.subs, 0), (self.arena, slot)))
             for k in range(len(segs) - 1, -1, -1):
                 a, e = segs[k]
                 src = (self.arena, slot) if k == 0 else (self.subs, k - 1)

@@ -229,7 +229,7 @@ def test_c5_shape_segment_replay_equals_whole_block_replay():
     nt = 400
     _say("large single shot 2000 x 6000, segment replay, nt", nt)
     model, solver, shot = _c5_shot(nt, delay=0.03)
-    # two checkpoints, blocks of 200 steps; 97 replay slots: segments of 97, 97, 6
+    # two checkpoints, blocks of 200 steps; 97 replay slots: segments of 6, 97, 97
     imaging = ImagingConfig(mode="pp", condition="zero_lag", checkpoint_mb=500.0)
     whole = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False)).migrate(shot, model)
     seg = RTMImager(imaging, solver,

@@ -247,6 +247,10 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
 
 _SIDE_STREAMS: dict = {}
 _SIDE_LOCK = threading.Lock()
+# Share of the free HBM the replay slots of one batch may take: what the
+# migration allocates after them (accumulators, sub-checkpoints, FFT buffers
+# of the stabiliser, post-processing) is a few per cent of the slots.
+_REPLAY_FREE_SHARE = 0.75
 
 
 def side_stream(role: str, launch=None):
@@ -770,13 +774,18 @@ class RTMImager:
 
     def _replay_slots(self, interval: int, cells: int) -> int:
         """Replay slots per segment: a whole block unless over the replay
-        budget (``cells`` summed over the shots advanced together)."""
+        budget (``cells`` summed over the shots advanced together) or over
+        ``_REPLAY_FREE_SHARE`` of the free HBM.  Free HBM counts the blocks
+        torch's caching allocator holds unused (an earlier batch's slots),
+        which these tensors reuse."""
         import torch
 
         if self.gpu_config.replay_slots > 0:
             return max(1, min(interval, int(self.gpu_config.replay_slots)))
         free, _total = torch.cuda.mem_get_info()
-        budget = min(float(self.gpu_config.replay_budget_gb) * 1024 ** 3, 0.25 * free)
+        free += max(0, torch.cuda.memory_reserved() - torch.cuda.memory_allocated())
+        budget = min(float(self.gpu_config.replay_budget_gb) * 1024 ** 3,
+                     _REPLAY_FREE_SHARE * free)
         return max(1, min(interval, int(budget // (4 * cells))))
 
     # ------------------------------------------------------------ helpers

@@ -170,6 +170,10 @@ struct ertm_batch_s {
     size_t psi1_bytes = 0;
     std::vector<float*> psi1_ptr;
     float* scratch = nullptr;  // per propagator: peak, scale
+    // per receiver-ring slot and propagator: the stabiliser's factor for the
+    // ring entry (compensated viscoacoustic backward passes with the ring)
+    float* rscale = nullptr;
+    size_t rscale_bytes = 0;
     void* block = nullptr;     // pool block holding dprops, drecv, dimg, scratch
     size_t block_bytes = 0;
     std::vector<hipStream_t> streams;  // streams this batch has enqueued work on
@@ -192,6 +196,7 @@ struct ertm_batch_s {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         if (cap) (void)hipStreamDestroy(cap);
         pool_put(dclk, 64);
+        pool_put(rscale, rscale_bytes);
         pool_put(damp, damp_bytes);
         pool_put(dark, sizeof(float*) * props.size());
         pool_put(block, block_bytes);
@@ -729,7 +734,8 @@ static int copy_states(ertm_batch_t b, int cur, float* const* arenas, int64_t sl
 
 // t_hi, amp and t arrays of the do_* helpers are indexed by propagator.
 static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t s,
-                    const DevClock* dc = nullptr, const Members* m = nullptr) {
+                    const DevClock* dc = nullptr, const Members* m = nullptr,
+                    const float* rscale = nullptr) {
     const int n = count_of(b, m);
     for (int i0 = 0; i0 < n; i0 += ERTM_MAX_LAUNCH) {
         LaunchArgs la{};
@@ -739,7 +745,7 @@ static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t 
         use_clock(la, dc, -(count - 1));  // the group's first (latest) step
         Timed tm(s, dc != nullptr);
         if (tm.status() != ERTM_OK) return tm.status();
-        launch_image_flush(b->dprops, b->dimg, la, b->max_img, s);
+        launch_image_flush(b->dprops, b->dimg, la, b->max_img, rscale, (int)b->props.size(), s);
         LAUNCHCHK("image_flush");
         // cells: the physical cells of the flushed steps
         if (int rc = tm.done(ERTM_TK_FLUSH, count * cells_of(b, la))) return rc;
@@ -876,8 +882,12 @@ static int do_image(ertm_batch_t b, int cur, const int32_t* t, int op, hipStream
     return ERTM_OK;
 }
 
+// rscale, ring_slot: with the receiver-field ring, the slot's row of
+// per-propagator factors (k_stab_control writes it) and the slot index the
+// low-pass rewrites with the filtered pressure; nullptr and -1 without.
 static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int interval,
-                        double* ctl, hipStream_t s) {
+                        double* ctl, hipStream_t s, float* rscale = nullptr,
+                        int ring_slot = -1) {
     if (!b->all_visco) return fail(ERTM_EINVAL, "stabilize: batch is not viscoacoustic");
     if (!ctl) return fail(ERTM_EINVAL, "stabilize: ctl is NULL");
     const int n = (int)b->props.size();
@@ -893,7 +903,8 @@ static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int
     launch_absmax_batch(b->dprops, la, ERTM_VI_P, max_n, peaks, s);
     LAUNCHCHK("absmax");
     const bool lowpass_now = step_index >= 30 && interval > 1 && step_index % interval == 0;
-    launch_stab_control(b->dprops, la, peaks, ctl, step_index, cap, lowpass_now ? 1 : 0, scales, s);
+    launch_stab_control(b->dprops, la, peaks, ctl, step_index, cap, lowpass_now ? 1 : 0, scales,
+                        rscale, s);
     LAUNCHCHK("stab_control");
     if (step_index < 30) return ERTM_OK;
     launch_stab_scale(b->dprops, la, scales, b->max_tiles, s);
@@ -938,6 +949,7 @@ static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int
                 a.nzp = nzp;
                 a.nxp = nxp;
                 a.cur = cur;
+                a.ring_slot = ring_slot;
                 for (int i = 0; i < m; ++i) {
                     const int k = ks[g0 + i];
                     const double time = step_index * b->props[k].dt64;
@@ -953,7 +965,7 @@ static int do_stabilize(ertm_batch_t b, int cur, int step_index, double cap, int
                 LAUNCHCHK("lowpass_response");
                 if (hipfftExecZ2D(fc.inv, (hipfftDoubleComplex*)fc.cplx, fc.real) != HIPFFT_SUCCESS)
                     return fail(ERTM_EHIP, "hipfftExecZ2D failed");
-                launch_lowpass_finish(b->dprops, a, fc.real, s);
+                launch_lowpass_finish(b->dprops, b->dimg, a, fc.real, s);
                 LAUNCHCHK("lowpass_finish");
             }
         }
@@ -1232,10 +1244,25 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
                                            r->restore_stride, false, s)) != ERTM_OK)
         return rc;
     const int nsteps = r->t_end - r->t_begin;
-    // backward correlation through the receiver-field ring
-    const bool use_ring = r->phase == 2 && r->op_after == ERTM_IMG_CORRELATE && !r->stabilize &&
-                          ring_ok(b);
+    // backward correlation through the receiver-field ring.  With the
+    // viscoacoustic stabiliser the ring holds the step's pressure before the
+    // stabiliser; its factor per slot (the clip scale, or 1 when the low-pass
+    // has rewritten the slot) makes the flushed value the stabilised one.
+    const bool use_ring = r->phase == 2 && r->op_after == ERTM_IMG_CORRELATE && ring_ok(b);
     const int ring_len = use_ring ? b->img[0].ring_len : 0;
+    float* rscale = nullptr;
+    if (use_ring && r->stabilize) {
+        const size_t bytes = sizeof(float) * (size_t)ring_len * n;
+        if (b->rscale_bytes < bytes) {
+            pool_put(b->rscale, b->rscale_bytes);
+            b->rscale = nullptr;
+            b->rscale_bytes = 0;
+            if (!(b->rscale = (float*)pool_get(bytes)))
+                return fail(ERTM_EHIP, "run: ring scale allocation failed");
+            b->rscale_bytes = bytes;
+        }
+        rscale = b->rscale;
+    }
     const int t0 = r->phase == 2 ? r->t_end - 1 : r->t_begin, dir = r->phase == 2 ? -1 : 1;
     const int fop = use_ring ? ERTM_IMG_RING_STORE
                              : ((!r->stabilize && r->op_after >= 0) ? r->op_after : -1);
@@ -1263,17 +1290,21 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
             return rc2;
         if (use_ring && !imaged) return fail(ERTM_ESTATE, "run: ring imaging launch refused");
         cur ^= 1;
+        if (r->stabilize) {
+            const int rslot = use_ring ? t % ring_len : -1;
+            if ((rc2 = do_stabilize(b, cur, step_index, r->cap, r->lp_interval, r->ctl, s,
+                                    rscale ? rscale + (size_t)rslot * n : nullptr, rslot)) !=
+                ERTM_OK)
+                return rc2;
+        }
+        // the flush comes after the stabiliser, whose factors it reads
         if (use_ring) {
             if (pending == 0) t_hi.assign(tt.begin(), tt.end());
             if (++pending == ring_len || (!dc && q == nsteps - 1)) {
-                if ((rc2 = do_flush(b, t_hi.data(), pending, s, dc)) != ERTM_OK) return rc2;
+                if ((rc2 = do_flush(b, t_hi.data(), pending, s, dc, nullptr, rscale)) != ERTM_OK)
+                    return rc2;
                 pending = 0;
             }
-        }
-        if (r->stabilize) {
-            if ((rc2 = do_stabilize(b, cur, step_index, r->cap, r->lp_interval, r->ctl, s)) !=
-                ERTM_OK)
-                return rc2;
         }
         ++step_index;
         if (r->phase != 2) {

@@ -184,9 +184,13 @@ __global__ void k_image(const ErtmProp* __restrict__ props, const ErtmImage* __r
 // Fold `count` backward steps t = t_hi, t_hi-1, ... (la.t = t_hi per
 // propagator, la.op = count) from the receiver-field ring into the
 // accumulators: per cell the additions happen in the same order, with the
-// same float32 products, as `count` ERTM_IMG_CORRELATE passes.
+// same float32 products, as `count` ERTM_IMG_CORRELATE passes.  With rscale
+// (viscoacoustic stabiliser) ring entry t of table entry p is multiplied by
+// rscale[(t mod R) * rs_n + p] first, in float32 as the stabiliser scales the
+// state, which makes it the stabilised field the correlation reads.
 __global__ void k_image_flush(const ErtmProp* __restrict__ props,
-                              const ErtmImage* __restrict__ imgs, LaunchArgs la) {
+                              const ErtmImage* __restrict__ imgs, LaunchArgs la,
+                              const float* __restrict__ rscale, int rs_n) {
     const int slot = la_prop(la, blockIdx.y);
     const ErtmProp& P = props[slot];
     const ErtmImage& I = imgs[slot];
@@ -213,7 +217,9 @@ __global__ void k_image_flush(const ErtmProp* __restrict__ props,
             ill = ill + (double)ss;
         }
         const bool cross = t >= I.start_step;
-        const float rv0 = ring[qr];
+        const float rs = rscale ? rscale[(int64_t)(t % I.ring_len) * rs_n + slot] : 1.0f;
+        float rv0 = ring[qr];
+        if (rscale) rv0 = rv0 * rs;
         if (rec) {
             float rr = rv0 * rv0;
             r0 = r0 + (double)rr;
@@ -223,7 +229,8 @@ __global__ void k_image_flush(const ErtmProp* __restrict__ props,
             n0 = n0 + (double)sr;
         }
         if (nrw > 1) {
-            const float rv1 = ring[nring + qr];
+            float rv1 = ring[nring + qr];
+            if (rscale) rv1 = rv1 * rs;
             if (rec) {
                 float rr = rv1 * rv1;
                 r1 = r1 + (double)rr;
@@ -244,10 +251,10 @@ __global__ void k_image_flush(const ErtmProp* __restrict__ props,
 }
 
 void launch_image_flush(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la,
-                        int max_cells, hipStream_t s) {
+                        int max_cells, const float* rscale, int rs_n, hipStream_t s) {
     if (max_cells <= 0 || la.op <= 0) return;
     hipLaunchKernelGGL(k_image_flush, dim3((max_cells + 255) / 256, la.n), dim3(256), 0, s, dprops,
-                       dimg, la);
+                       dimg, la, rscale, rs_n);
 }
 
 void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,
@@ -400,9 +407,12 @@ void launch_derivative(const void* f, void* out, int n0, int n1, int axis, int p
 // ---------------------------------------------------------------------------
 // viscoacoustic stabiliser, ertm/propagation.py:726-749
 // ---------------------------------------------------------------------------
+// rscale (may be null): the factor the receiver-field ring entry of this step
+// needs to equal the stabilised pressure, for members 0..n-1: the clip scale,
+// or 1 at low-pass steps (the low-pass rewrites the entry after filtering).
 __global__ void k_stab_control(const float* __restrict__ peaks, double* __restrict__ ctl,
                                int n, int step_index, double cap, int lowpass_now,
-                               float* __restrict__ scales) {
+                               float* __restrict__ scales, float* __restrict__ rscale) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double peak = (double)peaks[k];
@@ -420,13 +430,14 @@ __global__ void k_stab_control(const float* __restrict__ peaks, double* __restri
         if (lowpass_now) c[2] += 1.0;
     }
     scales[k] = sc;
+    if (rscale) rscale[k] = (step_index >= 30 && lowpass_now) ? 1.0f : sc;
 }
 
 void launch_stab_control(const ErtmProp*, const LaunchArgs& la, const float* peaks, double* ctl,
                          int step_index, double cap, int lowpass_now, float* scales,
-                         hipStream_t s) {
+                         float* rscale, hipStream_t s) {
     hipLaunchKernelGGL(k_stab_control, dim3(1), dim3(64), 0, s, peaks, ctl, la.n, step_index, cap,
-                       lowpass_now, scales);
+                       lowpass_now, scales, rscale);
 }
 
 __global__ void k_stab_scale(const ErtmProp* __restrict__ props, LaunchArgs la,
@@ -509,7 +520,11 @@ __global__ void k_lp_response(const ErtmProp* __restrict__ props, LpArgs a,
     v->y *= w;
 }
 
-__global__ void k_lp_finish(const ErtmProp* __restrict__ props, LpArgs a,
+// With a.ring_slot >= 0 the filtered pressure of each physical cell is also
+// written to that receiver-field ring slot (rows on the state's pitch: cell
+// (i, j) of the physical grid at row i, column j + pad).
+__global__ void k_lp_finish(const ErtmProp* __restrict__ props,
+                            const ErtmImage* __restrict__ imgs, LpArgs a,
                             const double* __restrict__ r) {
     const int y = blockIdx.y;
     const int64_t cells = (int64_t)a.nzp * a.nxp;
@@ -517,8 +532,16 @@ __global__ void k_lp_finish(const ErtmProp* __restrict__ props, LpArgs a,
     if (q >= cells) return;
     const ErtmProp& P = lp_prop(props, a, y);
     const int i = (int)(q / a.nxp), j = (int)(q - (int64_t)i * a.nxp);
-    lp_field(P, a, y)[(int64_t)i * P.ld + j] =
-        (float)(r[(int64_t)y * cells + q] / ((double)a.nzp * (double)a.nxp));
+    const float v = (float)(r[(int64_t)y * cells + q] / ((double)a.nzp * (double)a.nxp));
+    lp_field(P, a, y)[(int64_t)i * P.ld + j] = v;
+    if (a.ring_slot >= 0 && (y & 1) == 0) {
+        const ErtmImage& I = imgs[a.prop[y >> 1]];
+        const int ii = i - P.pad, jj = j - P.pad;
+        if (ii >= 0 && ii < P.nz && jj >= 0 && jj < P.nx) {
+            const int64_t nring = (int64_t)P.nz * P.ld;
+            I.rring[(int64_t)a.ring_slot * I.nrw * nring + (int64_t)ii * P.ld + j] = v;
+        }
+    }
 }
 
 void launch_lowpass_prep(const ErtmProp* dprops, const LpArgs& a, double* real, hipStream_t s) {
@@ -533,11 +556,11 @@ void launch_lowpass_response(const ErtmProp* dprops, const LpArgs& a, void* cplx
                        dprops, a, (double2*)cplx);
 }
 
-void launch_lowpass_finish(const ErtmProp* dprops, const LpArgs& a, const double* real,
-                           hipStream_t s) {
+void launch_lowpass_finish(const ErtmProp* dprops, const ErtmImage* dimg, const LpArgs& a,
+                           const double* real, hipStream_t s) {
     const int64_t n = (int64_t)a.nzp * a.nxp;
     hipLaunchKernelGGL(k_lp_finish, dim3((unsigned)((n + 255) / 256), 2 * a.n), dim3(256), 0, s,
-                       dprops, a, real);
+                       dprops, dimg, a, real);
 }
 
 // ---------------------------------------------------------------------------

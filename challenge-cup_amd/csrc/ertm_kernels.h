@@ -111,8 +111,10 @@ void launch_prep_mute_taper(float* rec, int nt, int nrec, const double* t, const
 void launch_prep_rotate(float* vx, float* vz, int64_t n, double c, double s_, hipStream_t s);
 void launch_resample(const float* in, int n_in, int nrec, const double* w, const int32_t* off,
                      int n_out, int taps, float* out, int out_rows, hipStream_t s);
+// rscale: null, or the per-ring-slot factors of the viscoacoustic stabiliser
+// (R rows of rs_n floats, one per batch-table entry)
 void launch_image_flush(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la,
-                        int max_cells, hipStream_t s);
+                        int max_cells, const float* rscale, int rs_n, hipStream_t s);
 void launch_image(const ErtmProp* dprops, const ErtmImage* dimg, const LaunchArgs& la, int order,
                   int max_cells, hipStream_t s);
 void launch_absmax(const float* x, int64_t n, float* out, hipStream_t s);
@@ -141,7 +143,7 @@ void launch_derivative(const void* f, void* out, int n0, int n1, int axis, int p
 // Viscoacoustic stabiliser pieces (ertm/propagation.py:726-749).
 void launch_stab_control(const ErtmProp* dprops, const LaunchArgs& la, const float* peaks,
                          double* ctl, int step_index, double cap, int lowpass_now, float* scales,
-                         hipStream_t s);
+                         float* rscale, hipStream_t s);
 void launch_stab_scale(const ErtmProp* dprops, const LaunchArgs& la, const float* scales,
                        int max_tiles, hipStream_t s);
 // A group of the stabiliser's low-pass: members k < n are propagators prop[k]
@@ -152,12 +154,13 @@ void launch_stab_scale(const ErtmProp* dprops, const LaunchArgs& la, const float
 constexpr int ERTM_LP_GROUP = 16;
 struct LpArgs {
     int32_t n, nzp, nxp, cur;
+    int32_t ring_slot;  // >= 0: also write the filtered p into this ring slot
     int32_t prop[ERTM_LP_GROUP];
     double ratio[ERTM_LP_GROUP];
 };
 void launch_lowpass_prep(const ErtmProp* dprops, const LpArgs& a, double* real, hipStream_t s);
 void launch_lowpass_response(const ErtmProp* dprops, const LpArgs& a, void* cplx, hipStream_t s);
-void launch_lowpass_finish(const ErtmProp* dprops, const LpArgs& a, const double* real,
-                           hipStream_t s);
+void launch_lowpass_finish(const ErtmProp* dprops, const ErtmImage* dimg, const LpArgs& a,
+                           const double* real, hipStream_t s);
 
 }  // namespace ertm

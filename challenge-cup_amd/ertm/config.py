@@ -167,10 +167,11 @@ class GpuConfig:
     kernel_variant (variant, zc) of the step kernels (see
                    ertm_batch_set_variant); None = the tuner's choice.  Every
                    variant gives the same results.
-    image_ring     backward steps whose receiver fields a fused elastic or
-                   acoustic batch folds into the image accumulators in one
-                   pass (0 = fold every step inside the step kernel).  Same
-                   results; 32 is ~1 % faster than 16 on 64 elastic shots.
+    image_ring     backward steps whose receiver fields a fused batch folds
+                   into the image accumulators in one pass (0 = fold every
+                   step inside the step kernel, or in a pass of its own after
+                   the viscoacoustic stabiliser).  Same results; 32 is ~1 %
+                   faster than 16 on 64 elastic shots.
     device_postprocess  normalise, Laplacian-filter, low-pass and mask the
                    images on the device (float64; the FFT rounds differently
                    from NumPy's, so images can differ from the host path in

@@ -696,7 +696,7 @@ class RTMImager:
             slots = self._replay_slots(store0.interval, cells)
             rbufs = [torch.empty((slots, m.shape[0], m.shape[1]), dtype=torch.float32,
                                  device="cuda") for m in models]
-            ring = int(self.gpu_config.image_ring) if engine in ("elastic", "acoustic") else 0
+            ring = int(self.gpu_config.image_ring)
             ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, rwaves, st, illum,
                                     rbuf=rb, nslots=slots, ring=ring, ring_ld=bc.ld)
                    for m, st, rb, bc in zip(models, starts, rbufs, bcores)]

@@ -137,8 +137,8 @@ def request_bytes(req: ShotRequest, gpu: GpuConfig | None = None) -> int:
     sub = math.ceil(interval / slots) * state if slots < interval else 0
     nmodes = len(req.modes or (req.imaging.mode,))
     accum = (2 * nmodes + 1 + 3) * cells * 2                      # float64 -> 2 floats
-    # receiver-field ring of the backward pass (elastic and acoustic engines)
-    ring = int(gpu.image_ring) * nmodes * nz * ld if engine in ("elastic", "acoustic") else 0
+    # receiver-field ring of the backward pass
+    ring = int(gpu.image_ring) * nmodes * nz * ld
     nrec = int(req.shot.receivers.nrec)
     ncomp = max(1, len(req.shot.components))
     record = nt * nrec * ncomp + 16 * nrec * ncomp

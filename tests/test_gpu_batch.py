@@ -111,6 +111,36 @@ def test_batch_nested_replay_and_pp_ps():
         assert np.array_equal(out[1][m].image, alone[m].image)
 
 
+def test_compensated_visco_ring_equals_imaging_after_the_stabiliser():
+    """Compensated viscoacoustic backward passes store the pressure of each
+    step in the receiver-field ring before the stabiliser runs; the flush
+    multiplies each entry by its step's clip factor, and low-pass steps
+    rewrite their entry.  With a small amplitude cap (clipping at most
+    steps) and a low-pass every 7 steps, inside ring windows of 5 and 32
+    steps, the images equal the separate imaging pass after the stabiliser
+    bit for bit."""
+    _say("compensated visco ring")
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case("vi_norm")
+    shot, model, imaging, solver, q = migrate_objects(c)
+    q = dataclasses.replace(q, amplitude_cap=1.5, lowpass_interval=7)
+    shots = [shot, _moved(shot, 3 * model.grid.dx)]
+    outs = {}
+    for ring in (0, 5, 32):
+        outs[ring] = RTMImager(imaging, solver, q, GpuConfig(kernel_variant=(2, 16),
+                                                             image_ring=ring)).migrate_batch(
+            shots, model)
+    meta = outs[0][0]["pp"].meta["q"]
+    assert meta["clipped_steps"] > 0 and meta["lowpass_steps"] > 0, meta
+    for ring in (5, 32):
+        for a, b in zip(outs[0], outs[ring]):
+            assert np.array_equal(a["pp"].image, b["pp"].image), ring
+            assert a["pp"].meta["q"]["clipped_steps"] == b["pp"].meta["q"]["clipped_steps"]
+    assert np.abs(outs[0][0]["pp"].image).max() > 0
+
+
 def test_batch_max_batch_splits_groups():
     _say("batch max_batch")
     from ertm.config import GpuConfig

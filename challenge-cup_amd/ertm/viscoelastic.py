@@ -85,6 +85,15 @@ def single_mechanism(q: float, f_ref: float) -> tuple[float, float]:
     return float(u / (2.0 * np.pi * f_ref)), float(1.0 - u * u)
 
 
+# Least-squares fits by (q, f_ref, n_mechanisms, band_ratio).  The fit is a
+# deterministic function of these, so a cached result equals a new one; every
+# propagator of a batch (and the backward propagators) fits the same few Q
+# values of one model, at tens of milliseconds per fit.  Fallbacks to the
+# closed form are not cached, so their warnings are logged on every call.
+_FIT_CACHE: dict = {}
+_FIT_CACHE_MAX = 4096
+
+
 def fit_gsls(q: float, f_ref: float, *, n_mechanisms: int = 3, band_ratio: float = 1.5,
              min_q: float = 2.0) -> GSLSParameters:
     """Constant-Q fit over [f_ref/band_ratio, f_ref*band_ratio] (ertm/viscoelastic.py:132-205)."""
@@ -104,6 +113,10 @@ def fit_gsls(q: float, f_ref: float, *, n_mechanisms: int = 3, band_ratio: float
         tau, dl = single_mechanism(q, f_ref)
         return _params(q, f_ref, [tau], [dl], (lo, hi), 0.0)
     n = int(n_mechanisms)
+    key = (float(q), float(f_ref), n, float(band_ratio))
+    hit = _FIT_CACHE.get(key)
+    if hit is not None:
+        return hit
     weights = 1.0 + 0.5 * np.exp(-0.5 * (np.log(freqs / f_ref) / 0.25) ** 2)
     x0 = np.concatenate([np.log(1.0 / (2.0 * np.pi * np.geomspace(lo, hi, n))),
                          np.log(np.full(n, 2.0 / q))])
@@ -126,7 +139,11 @@ def fit_gsls(q: float, f_ref: float, *, n_mechanisms: int = 3, band_ratio: float
         return _params(q, f_ref, [tau], [dl], (lo, hi), 0.0)
     tau, dl = np.exp(sol.x[:n]), np.exp(sol.x[n:])
     err = float(np.max(np.abs(np.log(np.maximum(q_of(sol.x), 1e-6)) - np.log(q))))
-    return _params(q, f_ref, tau, dl, (lo, hi), err)
+    params = _params(q, f_ref, tau, dl, (lo, hi), err)
+    if len(_FIT_CACHE) >= _FIT_CACHE_MAX:
+        _FIT_CACHE.clear()
+    _FIT_CACHE[key] = params
+    return params
 
 
 def gsls_table(q_values: Iterable[float], f_ref: float, *, n_mechanisms: int = 3,

@@ -47,3 +47,16 @@ def test_single_mechanism_peaks_at_reference_frequency():
 def test_nearly_elastic_q_uses_one_mechanism():
     p = fit_gsls(2.0e4, 30.0, n_mechanisms=3)
     assert len(p.tau_sigma) == 1 and p.fit_error == 0.0
+
+
+def test_cached_fit_equals_a_fresh_fit():
+    """A repeated fit returns the cached parameters; they equal a fit made
+    after the cache is cleared, value for value."""
+    from ertm import viscoelastic as ve
+
+    a = ve.fit_gsls(57.0, 30.0, n_mechanisms=3, band_ratio=1.5)
+    assert ve.fit_gsls(57.0, 30.0, n_mechanisms=3, band_ratio=1.5) is a
+    ve._FIT_CACHE.clear()
+    b = ve.fit_gsls(57.0, 30.0, n_mechanisms=3, band_ratio=1.5)
+    assert b is not a
+    assert a == b

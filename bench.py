@@ -415,11 +415,15 @@ def c3_bench(nshots, nt, batches=2):
 
 
 def c5_sweep(budgets_mb, nt, kernel_variant=None, conditions=("zero_lag", "excitation"),
-             segment_check=True, two_step=None):
+             segment_check=True, two_step=None, replay_gb=128.0):
     """Excitation vs zero_lag imaging of the large shot at several checkpoint
     budgets: one JSON line per run with the checkpoint interval, replayed
     steps / nt (recompute overhead), time and RTM cell-update rate; plus a
-    check that segment replay from sub-checkpoints gives the same image."""
+    check that segment replay from sub-checkpoints gives the same image.
+    ``replay_gb`` is the HBM for replay slots (48 MB per step of this grid):
+    the default 128 GB holds the longest block (2500 steps at the 1 GB
+    checkpoint budget), so every block is replayed once, as the reference
+    replays it; smaller values replay long blocks in segments."""
     import torch
     from ertm.config import GpuConfig, ImagingConfig, SolverConfig
     from ertm.imaging import RTMImager
@@ -432,7 +436,8 @@ def c5_sweep(budgets_mb, nt, kernel_variant=None, conditions=("zero_lag", "excit
     src = SourceSpec(x=7500.0, z=5.0, f0=40.0, delay=0.03, mechanism="explosive")
     recv = ReceiverSpec(x=np.arange(nx) * 2.5, z=np.full(nx, 5.0))
     solver = SolverConfig(nt=nt, dt=dt, order=8, boundary="free", engine="elastic", npml=20)
-    gpu = GpuConfig(strict_cfl=False, kernel_variant=kernel_variant, two_step=two_step)
+    gpu = GpuConfig(strict_cfl=False, kernel_variant=kernel_variant, two_step=two_step,
+                    replay_budget_gb=float(replay_gb))
     t0 = time.perf_counter()
     shot = build_solver(solver, gpu_config=gpu).forward(model, src, recv, components=("vx", "vz"))
     print(f"# c5 record modelled in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -451,7 +456,8 @@ def c5_sweep(budgets_mb, nt, kernel_variant=None, conditions=("zero_lag", "excit
             line = {"workload": "elastic 2000x6000 single shot", "condition": cond,
                     "kernel_variant": list(kernel_variant) if kernel_variant else "tuner",
                     "two_step": list(two_step) if two_step else "tuner",
-                    "checkpoint_mb": mb, "interval": meta["checkpoints"]["interval"],
+                    "checkpoint_mb": mb, "replay_budget_gb": float(replay_gb),
+                    "interval": meta["checkpoints"]["interval"],
                     "n_checkpoints": meta["checkpoints"]["n_states"],
                     "arena_gib": round(meta["checkpoints"]["device_arena_mb"] / 1024, 2),
                     "replay_overhead": round(meta["replay_overhead"], 4),
@@ -756,6 +762,8 @@ def main(argv=None) -> int:
                     help="--c5-sweep: checkpoint budgets (MB)")
     ap.add_argument("--c5-conditions", default="zero_lag,excitation",
                     help="--c5-sweep: imaging conditions")
+    ap.add_argument("--c5-replay-gb", type=float, default=128.0,
+                    help="--c5-sweep: HBM for replay slots (GB)")
     ap.add_argument("--engine-sweep", action="store_true",
                     help="instead of the RTM bench: plain step throughput of each engine's "
                          "single-launch kernel on the 64-shot grid")
@@ -814,7 +822,7 @@ def main(argv=None) -> int:
         conds = tuple(args.c5_conditions.split(","))
         ts = tuple(int(v) for v in args.two_step.split(",")) if args.two_step else None
         c5_sweep(budgets, args.c5_nt, kv, conds, segment_check=not args.kernel_variant,
-                 two_step=ts)
+                 two_step=ts, replay_gb=args.c5_replay_gb)
         return 0
     if args.engine_sweep:
         engine_sweep(args.shots, args.sweep_steps)

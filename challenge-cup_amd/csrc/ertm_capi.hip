@@ -388,10 +388,10 @@ int ertm_batch_destroy(ertm_batch_t b) {
     bool wait = !b->graphs.empty() || !b->fft.empty() || b->cap;
     if (!wait) {
         void** blocks[] = {&b->block, (void**)&b->dclk, (void**)&b->damp, (void**)&b->dark,
-                           &b->psi1};
+                           &b->psi1, (void**)&b->rscale};
         const size_t sizes[] = {b->block_bytes, 64, b->damp_bytes, sizeof(float*) * b->props.size(),
-                                b->psi1_bytes};
-        for (int i = 0; i < 5 && !wait; ++i) {
+                                b->psi1_bytes, b->rscale_bytes};
+        for (int i = 0; i < 6 && !wait; ++i) {
             if (!*blocks[i]) continue;
             if (pool_put_after(*blocks[i], sizes[i], b->streams)) *blocks[i] = nullptr;
             else wait = true;
@@ -1254,6 +1254,9 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
     if (use_ring && r->stabilize) {
         const size_t bytes = sizeof(float) * (size_t)ring_len * n;
         if (b->rscale_bytes < bytes) {
+            // flushes already queued with this batch may still read the old rows
+            if (b->rscale)
+                for (hipStream_t t : b->streams) HIPCHK(hipStreamSynchronize(t));
             pool_put(b->rscale, b->rscale_bytes);
             b->rscale = nullptr;
             b->rscale_bytes = 0;

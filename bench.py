@@ -867,7 +867,7 @@ def main(argv=None) -> int:
     if args.image_ring is not None:
         gkw["image_ring"] = args.image_ring
     imager = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False,
-                                                             replay_budget_gb=64.0,
+                                                             replay_budget_gb=200.0,
                                                              kernel_variant=kv, **gkw))
     t0 = time.perf_counter()
     job = imager.prepare_batch(shots, model, modes=("pp", "ps"))
@@ -922,13 +922,15 @@ def main(argv=None) -> int:
         "gcell_updates_per_s": round(gcells, 2),
         "gcell_roofline_frac": round(gcells * BYTES_PER_CELL / HBM_PEAK_GBS, 4),
         # cell-updates the passes executed: the checkpointing forward pass
-        # ends at its last checkpoint, so a little below 3 nt nz nx per shot
+        # ends at its last checkpoint, or runs to nt keeping the source
+        # fields of its last steps, whose replay is then skipped
         "gcell_updates_executed_per_s": round(
             imager.last_timing["cell_updates"] / elapsed / 1e9, 2),
         "gcell_roofline_frac_executed": round(
             imager.last_timing["cell_updates"] / elapsed * BYTES_PER_CELL / 1e12
             / (HBM_PEAK_GBS / 1e3), 4),
         "phase_s": {k: round(v / args.steps, 3) for k, v in parts.items()},
+        "kept_tail_steps": round(imager.last_timing["kept_tail_steps"] / args.steps, 1),
         "upload_s": round(upload_s, 3),
         "roofline": dict(roof, traffic=traffic,
                          # (variant, rows per chunk): 4 = 192-lane, 2 = 128-lane blocks

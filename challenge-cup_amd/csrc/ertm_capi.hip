@@ -1234,6 +1234,17 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
                         (long long)r->save_stride, (long long)sf, k);
     }
     if (r->op_after > ERTM_IMG_EXC_APPLY) return fail(ERTM_EINVAL, "run: bad image op");
+    if ((r->op_after == ERTM_IMG_EXTRACT || r->op_after == ERTM_IMG_CORRELATE) &&
+        r->t_end > r->t_begin) {
+        // every imaged step t writes or reads replay slot t - t0
+        for (int k = 0; k < (int)b->img.size(); ++k) {
+            const ErtmImage& I = b->img[k];
+            if (I.rbuf && (r->t_begin < I.t0 || (int64_t)r->t_end - I.t0 > I.nslots))
+                return fail(ERTM_EINVAL,
+                            "run: steps [%d, %d) of prop %d outside its replay slots [%d, %d)",
+                            r->t_begin, r->t_end, k, I.t0, I.t0 + I.nslots);
+        }
+    }
     if (r->graph_steps < 0) return fail(ERTM_EINVAL, "run: graph_steps must be >= 0");
     std::vector<float> amp(n, 0.0f);
     std::vector<int32_t> tt(n, 0);

@@ -195,6 +195,13 @@ class GpuConfig:
                    they are float32 with two or more traces and no
                    preprocess callable is given; same results as the host
                    path (whose einsum sums the taps in the same order).
+    keep_tail      batched migrations keep the source fields of the last
+                   steps in the replay slots during the checkpointing pass
+                   instead of replaying them, when the source wavelet is
+                   bitwise constant from that block's checkpoint on (replay
+                   step t then does what forward step t + 1 did: same state,
+                   same amplitude) and the slots hold more than the last
+                   block.  Same results.
     """
 
     strict_cfl: bool = True
@@ -208,6 +215,7 @@ class GpuConfig:
     graph_steps: int = 0
     two_step: tuple | None = None
     device_resample: bool = True
+    keep_tail: bool = True
 
     def validate(self) -> None:
         if self.device_mem_gb <= 0:

@@ -166,6 +166,33 @@ def sub_checkpoint_origin(segs: list[tuple[int, int]]) -> int:
     return segs[1][0] - 1
 
 
+def keep_tail_start(wavelets, nt: int, interval: int, room: int) -> int | None:
+    """First step T of the tail whose source fields the checkpointing pass
+    keeps (``record_source_wavefields``' ``tail``), or None.
+
+    Replaying a block restarts from the state after step s0 and steps it
+    with wavelet[s0], wavelet[s0 + 1], ... (ertm/imaging.py:150-160), while
+    the forward pass stepped that state with wavelet[s0 + 1], ...  Where
+    every wavelet is bitwise constant from s0 to nt - 1 the two are the same
+    operations on the same state, so replay step t extracts what forward
+    step t + 1 does.  T is the latest of the first block start with that
+    property and nt - ``room`` (the replay slots there are); None when the
+    tail would hold no more than the last block, whose forward steps the
+    pass otherwise skips."""
+    nt, interval = int(nt), max(1, int(interval))
+    last = (nt - 1) // interval * interval
+    z = 0
+    for w in wavelets:
+        u = np.ascontiguousarray(np.asarray(w, dtype=np.float32)[:nt]).view(np.uint32)
+        if u.size < nt:
+            return None
+        diff = np.nonzero(u != u[nt - 1])[0]
+        z = max(z, int(diff[-1]) + 1 if diff.size else 0)
+    first = -(-z // interval) * interval
+    t = max(first, nt - max(int(room), 0))
+    return t if t < last else None
+
+
 def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
                             budget_mb: float) -> CheckpointStore:
     """Forward-propagate and checkpoint into an HBM arena (ertm/imaging.py:113-137).
@@ -180,7 +207,7 @@ def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
 def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarray], *,
                              budget_mb: float, variant=None, graph_steps: int = 0,
                              during=None, to_last_checkpoint: bool = False,
-                             two_step=(0, 0)) -> list[CheckpointStore]:
+                             two_step=(0, 0), tail=None) -> list[CheckpointStore]:
     """``record_source_wavefield`` for several shots advanced in one launch per step.
 
     All shots must share nt and the checkpoint interval (the interval follows
@@ -191,6 +218,12 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     checkpoint instead of at nt: a migration restarts every block from its
     checkpoint, so the steps after the last one produce nothing it reads
     (the checkpoints are the same either way).
+
+    ``tail = (T, imgs)`` (``keep_tail_start``): the pass saves the
+    checkpoints up to the state after step T, then runs steps T + 1 .. nt
+    with the source-field extraction into the replay slots of ``imgs``
+    (their t0 = T + 1), so slot t - T holds what replaying step t would
+    extract; step nt takes the wavelet's constant last value.
     """
     import torch
 
@@ -213,7 +246,8 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     curs = {c.cur for c in cores}
     if len(step0) != 1 or len(curs) != 1:
         raise ImagingError("batched propagators must be at the same step")
-    b = dev.Batch(cores, variant=variant, graph_steps=graph_steps, two_step=two_step)
+    b = dev.Batch(cores, imgs=tail[1] if tail is not None else None, variant=variant,
+                  graph_steps=graph_steps, two_step=two_step)
     if variant is None:
         from . import tuning
 
@@ -225,9 +259,22 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     ws = [np.asarray(w, dtype=np.float32) for w in wavelets]
     kw = batch_run_kwargs(props)
     # the checkpoint of start s holds the state after step s
-    t_end = starts[-1] + 1 if to_last_checkpoint else nt
-    cur = b.run(0, 0, t_end, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
-                save=arenas, save_stride=stride, step_index0=step0.pop(), **kw)
+    si0 = step0.pop()
+    if tail is None:
+        t_end = starts[-1] + 1 if to_last_checkpoint else nt
+        cur = b.run(0, 0, t_end, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
+                    save=arenas, save_stride=stride, step_index0=si0, **kw)
+    else:
+        t_keep = int(tail[0])
+        if not 0 <= t_keep < nt - 1:
+            raise ImagingError(f"kept tail start {t_keep} outside [0, {nt - 1})")
+        cur = b.run(0, 0, t_keep + 1, cores[0].cur, wavelets=ws, ck_interval=interval,
+                    ck_origin=0, save=arenas, save_stride=stride, step_index0=si0, **kw)
+        # no checkpoint is read from here on: every later block lies in the tail
+        wz = [np.append(w, w[-1:]) for w in ws]
+        t_end = nt + 1
+        cur = b.run(0, t_keep + 1, t_end, cur, wavelets=wz, op_after=L.IMG_EXTRACT,
+                    step_index0=si0 + t_keep + 1, **kw)
     try:
         if during is not None:
             during()
@@ -375,6 +422,8 @@ class RTMImager:
         self.gpu_config.validate()
         self._validate_engine_mode()
         self.last_timing: dict[str, float] = {}
+        # replay slots per batch shape: set by its first batch, never grown
+        self._rooms: dict[tuple, int] = {}
 
     def q_options(self, *, compensation: bool = False) -> QOptions | None:
         """GSLS options for the viscoacoustic engine (ertm/imaging.py:187-202)."""
@@ -495,7 +544,8 @@ class RTMImager:
         engine = self.solver_config.engine
         condition = str(self.config.condition).lower()
         timing = {"forward_s": 0.0, "replay_backward_s": 0.0, "postprocess_s": 0.0,
-                  "cell_updates": 0.0, "shots": 0.0, "launch_groups": 0.0}
+                  "cell_updates": 0.0, "shots": 0.0, "launch_groups": 0.0,
+                  "kept_tail_steps": 0.0}
         cap = int(self.gpu_config.max_batch)
         outs: list[list[dict[str, ImageResult] | None]] = [[None] * len(j.shots) for j in jobs]
         units = []
@@ -519,7 +569,8 @@ class RTMImager:
                         [job.shots[i] for i in idx], [job.records[i] for i in idx],
                         [job.waves[i] for i in idx], job.swave, job.rwaves, condition,
                         self.config.separation)
-                for key in ("forward_s", "replay_backward_s", "cell_updates", "shots"):
+                for key in ("forward_s", "replay_backward_s", "cell_updates", "shots",
+                            "kept_tail_steps"):
                     timing[key] += self.last_timing.get(key, 0.0)
                 timing["launch_groups"] += 1.0
                 # metadata now, so that no device buffer of this group outlives it
@@ -633,6 +684,40 @@ class RTMImager:
                     backs.append(back)
                     tabs.append(tab)
 
+        # Replay slots, allocated before the forward pass: it fills the
+        # first ones when it keeps the source fields of the last steps.
+        tail = None
+        t_keep = nt
+        if condition != "excitation":
+            cells = sum(m.shape[0] * m.shape[1] for m in models)
+            interval = _common({checkpoint_interval(nt, max(int(f.state_bytes()), 1),
+                                                    float(self.config.checkpoint_mb))
+                                for f in fwds}, "checkpoint interval")
+            arena_bytes = (4 * len(range(0, nt, interval))
+                           * max(f._core.state_floats for f in fwds) * n)
+            # Later batches of a shape take no more slots than the first: a
+            # larger request than the block the caching allocator holds from
+            # the previous batch makes it free its cache (a device
+            # synchronisation) and map the slots anew.
+            key = (nt, interval, tuple((m.shape[0], m.shape[1]) for m in models))
+            room = min(self._slot_room(cells, reserve=arena_bytes),
+                       self._rooms.get(key, 1 << 62))
+            self._rooms[key] = room
+            slots = max(1, min(interval, room))
+            kept = (keep_tail_start(waves, nt, interval, room)
+                    if self.gpu_config.keep_tail else None)
+            if kept is not None:
+                t_keep = kept
+                slots = max(slots, nt - t_keep)
+            rbufs = [torch.empty((slots, m.shape[0], m.shape[1]), dtype=torch.float32,
+                                 device="cuda") for m in models]
+            fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, 0,
+                                      rbuf=rb, nslots=slots) for m, rb in zip(models, rbufs)]
+            if kept is not None:
+                for fi in fimgs:
+                    fi.s.t0 = t_keep + 1
+                tail = (t_keep, fimgs)
+
         setup = None
         if side.cuda_stream != torch.cuda.current_stream().cuda_stream:
             # the set-up has a stream of its own: run it beside the whole
@@ -642,7 +727,7 @@ class RTMImager:
             stores = record_source_wavefields(
                 fwds, waves, budget_mb=float(self.config.checkpoint_mb), variant=kv,
                 graph_steps=gsteps, during=setup.join if setup else build_backs,
-                to_last_checkpoint=True, two_step=ts)
+                to_last_checkpoint=True, two_step=ts, tail=tail)
         except BaseException:
             if setup is not None:
                 setup.wait()
@@ -671,6 +756,7 @@ class RTMImager:
         replayed = 0
         torch.cuda.current_stream().synchronize()
         timing["forward_s"] = time.perf_counter() - t0
+        timing["kept_tail_steps"] = float(nt - t_keep)
         t1 = time.perf_counter()
         if condition == "excitation":
             ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, "P", rwaves[:1], st,
@@ -692,16 +778,10 @@ class RTMImager:
             bb.close()
             accs = [(ib.num, ib.peak_amp ** 2, ib.rill) for ib in ibs]
         else:
-            cells = sum(m.shape[0] * m.shape[1] for m in models)
-            slots = self._replay_slots(store0.interval, cells)
-            rbufs = [torch.empty((slots, m.shape[0], m.shape[1]), dtype=torch.float32,
-                                 device="cuda") for m in models]
             ring = int(self.gpu_config.image_ring)
             ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, rwaves, st, illum,
                                     rbuf=rb, nslots=slots, ring=ring, ring_ld=bc.ld)
                    for m, st, rb, bc in zip(models, starts, rbufs, bcores)]
-            fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, 0,
-                                      rbuf=rb, nslots=slots) for m, rb in zip(models, rbufs)]
             fb = dev.Batch(fcores, imgs=fimgs, variant=kv, graph_steps=gsteps, two_step=ts)
             bb = dev.Batch(bcores, recvs=tabs, imgs=ibs, variant=kv, graph_steps=gsteps)
             subs = None
@@ -709,8 +789,19 @@ class RTMImager:
                 nsub = int(math.ceil(store0.interval / slots))
                 subs = [torch.empty((nsub, astride), dtype=torch.float32, device="cuda")
                         for _c in fcores]
+            if t_keep < nt:
+                # the kept tail: its source fields are in the slots already
+                for ib in ibs:
+                    ib.s.t0 = t_keep
+                bb.update_images(ibs)
+                bcur = bb.run(2, t_keep, nt, bcur, op_after=L.IMG_CORRELATE,
+                              step_index0=back_steps, **bkw)
+                back_steps += nt - t_keep
             for blk in range(nblocks - 1, -1, -1):
                 s0, s1 = store0.bounds(blk, nt)
+                s1 = min(s1, t_keep)
+                if s1 <= s0:
+                    continue
                 slot = store0.states[blk][1]
                 segs = replay_segments(s0, s1, slots)
                 if len(segs) > 1:
@@ -772,21 +863,22 @@ class RTMImager:
         self.last_timing = timing
         return all_outs, infos
 
-    def _replay_slots(self, interval: int, cells: int) -> int:
-        """Replay slots per segment: a whole block unless over the replay
-        budget (``cells`` summed over the shots advanced together) or over
-        ``_REPLAY_FREE_SHARE`` of the free HBM.  Free HBM counts the blocks
-        torch's caching allocator holds unused (an earlier batch's slots),
-        which these tensors reuse."""
+    def _slot_room(self, cells: int, reserve: int = 0) -> int:
+        """Replay slots (steps of ``cells`` float32 fields, summed over the
+        shots advanced together) that fit the replay budget and
+        ``_REPLAY_FREE_SHARE`` of the free HBM left after ``reserve`` bytes
+        still to be allocated.  Free HBM counts the blocks torch's caching
+        allocator holds unused (an earlier batch's slots), which these
+        tensors reuse.  Over it a block is replayed in segments."""
         import torch
 
         if self.gpu_config.replay_slots > 0:
-            return max(1, min(interval, int(self.gpu_config.replay_slots)))
+            return max(1, int(self.gpu_config.replay_slots))
         free, _total = torch.cuda.mem_get_info()
         free += max(0, torch.cuda.memory_reserved() - torch.cuda.memory_allocated())
         budget = min(float(self.gpu_config.replay_budget_gb) * 1024 ** 3,
-                     _REPLAY_FREE_SHARE * free)
-        return max(1, min(interval, int(budget // (4 * cells))))
+                     _REPLAY_FREE_SHARE * max(0, free - int(reserve)))
+        return max(1, int(budget // (4 * cells)))
 
     # ------------------------------------------------------------ helpers
     def _validate_engine_mode(self) -> None:

@@ -193,6 +193,18 @@ def keep_tail_start(wavelets, nt: int, interval: int, room: int) -> int | None:
     return t if t < last else None
 
 
+def slot_plan(wavelets, nt: int, interval: int, room: int,
+              keep_tail: bool = True) -> tuple[int, int | None]:
+    """(replay slots, kept tail start or None) of a migration whose slots may
+    hold ``room`` steps: a whole block (fewer when ``room`` is smaller), or
+    the kept tail when it is longer."""
+    slots = max(1, min(int(interval), int(room)))
+    t_keep = keep_tail_start(wavelets, nt, interval, room) if keep_tail else None
+    if t_keep is not None:
+        slots = max(slots, int(nt) - t_keep)
+    return slots, t_keep
+
+
 def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
                             budget_mb: float) -> CheckpointStore:
     """Forward-propagate and checkpoint into an HBM arena (ertm/imaging.py:113-137).
@@ -703,12 +715,9 @@ class RTMImager:
             room = min(self._slot_room(cells, reserve=arena_bytes),
                        self._rooms.get(key, 1 << 62))
             self._rooms[key] = room
-            slots = max(1, min(interval, room))
-            kept = (keep_tail_start(waves, nt, interval, room)
-                    if self.gpu_config.keep_tail else None)
+            slots, kept = slot_plan(waves, nt, interval, room, self.gpu_config.keep_tail)
             if kept is not None:
                 t_keep = kept
-                slots = max(slots, nt - t_keep)
             rbufs = [torch.empty((slots, m.shape[0], m.shape[1]), dtype=torch.float32,
                                  device="cuda") for m in models]
             fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, 0,

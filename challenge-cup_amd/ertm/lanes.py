@@ -36,7 +36,7 @@ from . import _lib as L
 from . import device as dev
 from .exceptions import ImagingError
 from .imaging import (CheckpointStore, PreparedBatch, RTMImager, checkpoint_interval,
-                      replay_segments, sub_checkpoint_origin)
+                      replay_segments, slot_plan, sub_checkpoint_origin)
 from .propagation import build_propagator
 from .types import ImageResult
 
@@ -51,14 +51,26 @@ def lanes_supported(imager: RTMImager) -> bool:
             and str(imager.config.condition).lower() != "excitation")
 
 
-def lane_replay_slots(imager: RTMImager, interval: int, cells: int) -> int:
-    """Replay slots of one lane: a whole block unless over the replay budget
-    (``GpuConfig.replay_slots`` if set).  The count only sets how a block is
-    split into replay segments; the image does not depend on it."""
-    gpu = imager.gpu_config
+def lane_slot_room(gpu, cells: int) -> int:
+    """Replay slots one migration of ``cells`` cells may hold outside a
+    batch: ``GpuConfig.replay_slots`` if set, else what the replay budget
+    holds (the serving loop admits requests by these bytes)."""
     if gpu.replay_slots > 0:
-        return max(1, min(interval, int(gpu.replay_slots)))
-    return max(1, min(interval, int(float(gpu.replay_budget_gb) * _GIB // (4 * max(cells, 1)))))
+        return max(1, int(gpu.replay_slots))
+    return max(1, int(float(gpu.replay_budget_gb) * _GIB // (4 * max(cells, 1))))
+
+
+def lane_replay_slots(imager: RTMImager, interval: int, cells: int, wave=None,
+                      nt: int = 0) -> tuple[int, int | None]:
+    """(replay slots, kept tail start or None) of one lane (``slot_plan``
+    for ``lane_slot_room``).  They only set which steps are replayed and how
+    a block is split into replay segments; the image does not depend on
+    them."""
+    gpu = imager.gpu_config
+    room = lane_slot_room(gpu, cells)
+    if wave is None:
+        return max(1, min(interval, room)), None
+    return slot_plan([wave], nt, interval, room, gpu.keep_tail)
 
 
 class Lane:
@@ -97,9 +109,11 @@ class Lane:
                             [imager._routing(nm) for nm in names], back._k_relaxed_host())
         tab.set_records([record[nm] for nm in names])
         self.fwd, self.back, self.tab = fwd, back, tab
-        self.wave = np.ascontiguousarray(job.waves[0], dtype=np.float32)
-        nt = int(len(self.wave))
+        wave = np.ascontiguousarray(job.waves[0], dtype=np.float32)
+        nt = int(len(wave))
         self.nt = nt
+        # a kept tail's last forward step (nt) takes the constant last value
+        self.wave = np.ascontiguousarray(np.append(wave, wave[-1:]))
         # the interval from the fields-only state size before the first step
         # (ertm/imaging.py:121-124), as record_source_wavefield takes it
         self.per_state = max(int(fwd.state_bytes()), 1)
@@ -107,19 +121,20 @@ class Lane:
         starts = list(range(0, nt, interval))
         fc, bc = fwd._core, back._core
         self.stride = int(fc.state_floats)
+        nz, nx = model.shape
+        self.slots, kept = lane_replay_slots(imager, interval, nz * nx, wave, nt)
+        self.t_keep = nt if kept is None else kept
         self.arena = torch.empty((len(starts), self.stride), dtype=torch.float32, device="cuda")
         self.store = CheckpointStore(states=[(s, k) for k, s in enumerate(starts)],
                                      interval=interval, bytes_per_state=self.per_state,
                                      arena=self.arena, state_floats=self.stride,
-                                     forward_steps=starts[-1] + 1)
+                                     forward_steps=starts[-1] + 1 if kept is None else nt + 1)
         st = int(np.floor((float(shot.source.delay) + float(cfg.start_periods)
                            / max(float(shot.source.f0), 1e-6)) / spec.dt))
         self.start_step = int(np.clip(st, 0, nt))
         rec_illum = L.ILLUM_REC if cfg.receiver_normalization else 0
         illum = rec_illum | (L.ILLUM_SRC if self.condition == "normalized" and cfg.illumination
                              else 0)
-        nz, nx = model.shape
-        self.slots = lane_replay_slots(imager, interval, nz * nx)
         self.rbuf = torch.empty((self.slots, nz, nx), dtype=torch.float32, device="cuda")
         ring = int(imager.gpu_config.image_ring)
         self.ib = dev.ImageBuffers(nz, nx, cfg.separation, job.swave, job.rwaves, self.start_step,
@@ -146,10 +161,21 @@ class Lane:
         """(phase, t_begin, t_end, op, img_t0, ck_interval, ck_origin, save,
         restore) per pass, in order; save/restore are (tensor, slot) or None.
         The same passes ``RTMImager._run_shots`` makes for this shot."""
-        st, nt, slots = self.store, self.nt, self.slots
-        prog = [(0, 0, st.forward_steps, -1, 0, st.interval, 0, (self.arena, 0), None)]
+        st, nt, slots, tk = self.store, self.nt, self.slots, self.t_keep
+        if tk < nt:
+            # checkpoints up to the state after step tk, then the kept tail:
+            # forward steps tk + 1 .. nt extract what replay steps tk .. nt - 1
+            # would, and the backward pass over it reads them
+            prog = [(0, 0, tk + 1, -1, 0, st.interval, 0, (self.arena, 0), None),
+                    (0, tk + 1, nt + 1, L.IMG_EXTRACT, tk + 1, 0, 0, None, None),
+                    (2, tk, nt, L.IMG_CORRELATE, tk, 0, 0, None, None)]
+        else:
+            prog = [(0, 0, st.forward_steps, -1, 0, st.interval, 0, (self.arena, 0), None)]
         for blk in range(len(st.states) - 1, -1, -1):
             s0, s1 = st.bounds(blk, nt)
+            s1 = min(s1, tk)
+            if s1 <= s0:
+                continue
             slot = st.states[blk][1]
             segs = replay_segments(s0, s1, slots)
             if len(segs) > 1:

@@ -56,10 +56,10 @@ from . import numerics as num
 from ._lib import EngineUnavailable
 from .config import GpuConfig, ImagingConfig, QConfig, SolverConfig
 from .exceptions import ERTMError, ImagingError, PipelineError
-from .imaging import RTMImager, checkpoint_interval
-from .lanes import Lane, LaneRunner, lanes_supported
+from .imaging import RTMImager, checkpoint_interval, slot_plan
+from .lanes import Lane, LaneRunner, lane_slot_room, lanes_supported
 from .pipeline import StackAccumulator
-from .propagation import PropagationSpec
+from .propagation import PropagationSpec, source_wavelet
 from .types import ElasticModel, ImageResult, ShotRecord
 
 _GIB = 1024.0 ** 3
@@ -131,9 +131,9 @@ def request_bytes(req: ShotRequest, gpu: GpuConfig | None = None) -> int:
     interval = checkpoint_interval(nt, nf * nzp * nxp * 4, float(req.imaging.checkpoint_mb))
     arena = math.ceil(nt / interval) * state
     cells = nz * nx
-    slots = min(interval, max(1, int(float(gpu.replay_budget_gb) * _GIB // (4 * cells))))
-    if gpu.replay_slots > 0:
-        slots = min(interval, int(gpu.replay_slots))
+    # a whole block, or the kept tail of the checkpointing pass
+    wave = source_wavelet(req.shot.source, spec.dt, nt)
+    slots, _ = slot_plan([wave], nt, interval, lane_slot_room(gpu, cells), gpu.keep_tail)
     sub = math.ceil(interval / slots) * state if slots < interval else 0
     nmodes = len(req.modes or (req.imaging.mode,))
     accum = (2 * nmodes + 1 + 3) * cells * 2                      # float64 -> 2 floats

@@ -11,8 +11,9 @@ the compensated viscoacoustic tolerance) and the image of the same migration
 with the tail switched off, bit for bit, and the migration must have
 replayed fewer steps than that one (the tail was used).  Cases: whole-block slots,
 and slots fewer than a block (the tail starts mid-block and earlier blocks
-are replayed in segments); one shot and a batch of three; the 64-shot
-workload's grid and launch shape at nt = 600.
+are replayed in segments); one shot and a batch of three; lanes of
+different schedules; the 64-shot workload's grid and launch shape at
+nt = 600.
 """
 
 from __future__ import annotations
@@ -175,3 +176,36 @@ def test_kept_tail_on_the_64_shot_grid():
             for mode in ("pp", "ps"):
                 assert np.array_equal(outs[key][k][mode].image, base[k][mode].image), (key, k,
                                                                                        mode)
+
+
+def test_kept_tail_in_lanes():
+    """Lanes (ertm_run_lanes) of different schedules, each with a wavelet
+    that turns constant: the kept tail's forward steps run as extracting
+    lane steps.  Each image equals the same shot's migration without the
+    tail, bit for bit, with fewer steps replayed."""
+    _say("kept tail in lanes")
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+    from ertm.lanes import migrate_lanes
+
+    items, expected = [], []
+    for tag, nt, slots in (("el_pp_divcurl", None, 0), ("el_ps_stress", 253, 0),
+                           ("el_sp_zl", None, 23), ("el_pp_stress", None, 0),
+                           ("ac_zero_lag", None, 6), ("ac_raw_rn", 277, 0)):
+        c = migrate_case(tag)
+        shot, model, imaging, solver, q = migrate_objects(c)
+        if nt:
+            solver = dataclasses.replace(solver, nt=int(nt))
+            c = dict(c, nt=int(nt))
+        w = _wavelet(c)
+        im = RTMImager(imaging, solver, q, GpuConfig(replay_slots=slots))
+        items.append((im, im.prepare_batch([shot], model, wavelets=[w])))
+        plain = RTMImager(imaging, solver, q, GpuConfig(replay_slots=slots, keep_tail=False))
+        expected.append((tag, imaging.mode, plain.migrate(shot, model, wavelet=w)))
+    results = migrate_lanes(items)
+    for res, (tag, mode, alone) in zip(results, expected):
+        _say(tag, "replayed", res[mode].meta["replayed_steps"], "against",
+             alone.meta["replayed_steps"])
+        assert res[mode].meta["replayed_steps"] < alone.meta["replayed_steps"], tag
+        assert np.abs(alone.image).max() > 0, tag
+        assert np.array_equal(res[mode].image, alone.image), tag

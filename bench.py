@@ -415,15 +415,19 @@ def c3_bench(nshots, nt, batches=2):
 
 
 def c5_sweep(budgets_mb, nt, kernel_variant=None, conditions=("zero_lag", "excitation"),
-             segment_check=True, two_step=None, replay_gb=128.0):
+             segment_check=True, two_step=None, replay_gb=200.0):
     """Excitation vs zero_lag imaging of the large shot at several checkpoint
     budgets: one JSON line per run with the checkpoint interval, replayed
     steps / nt (recompute overhead), time and RTM cell-update rate; plus a
     check that segment replay from sub-checkpoints gives the same image.
-    ``replay_gb`` is the HBM for replay slots (48 MB per step of this grid):
-    the default 128 GB holds the longest block (2500 steps at the 1 GB
-    checkpoint budget), so every block is replayed once, as the reference
-    replays it; smaller values replay long blocks in segments."""
+    ``replay_gb`` is the HBM for replay slots (48 MB per step of this grid),
+    capped at three quarters of the HBM free after the checkpoints: the
+    default 200 GB holds the longest block (2500 steps at the 1 GB
+    checkpoint budget), so every block is replayed once, and zero_lag
+    migrations keep the source fields of the last ~4000 steps of their
+    checkpointing pass instead of replaying them (the wavelet is bitwise
+    constant from step ~550); smaller values replay long blocks in
+    segments."""
     import torch
     from ertm.config import GpuConfig, ImagingConfig, SolverConfig
     from ertm.imaging import RTMImager
@@ -461,6 +465,7 @@ def c5_sweep(budgets_mb, nt, kernel_variant=None, conditions=("zero_lag", "excit
                     "n_checkpoints": meta["checkpoints"]["n_states"],
                     "arena_gib": round(meta["checkpoints"]["device_arena_mb"] / 1024, 2),
                     "replay_overhead": round(meta["replay_overhead"], 4),
+                    "kept_tail_steps": int(imager.last_timing.get("kept_tail_steps", 0)),
                     "seconds": round(secs, 2),
                     "rtm_gcell_updates_per_s": round(3 * nt * nz * nx / secs / 1e9, 2),
                     "finite": bool(np.isfinite(res.image).all())}
@@ -762,7 +767,7 @@ def main(argv=None) -> int:
                     help="--c5-sweep: checkpoint budgets (MB)")
     ap.add_argument("--c5-conditions", default="zero_lag,excitation",
                     help="--c5-sweep: imaging conditions")
-    ap.add_argument("--c5-replay-gb", type=float, default=128.0,
+    ap.add_argument("--c5-replay-gb", type=float, default=200.0,
                     help="--c5-sweep: HBM for replay slots (GB)")
     ap.add_argument("--engine-sweep", action="store_true",
                     help="instead of the RTM bench: plain step throughput of each engine's "

@@ -201,7 +201,10 @@ class GpuConfig:
                    bitwise constant from that block's checkpoint on (replay
                    step t then does what forward step t + 1 did: same state,
                    same amplitude) and the slots hold more than the last
-                   block.  Same results.
+                   block.  Excitation imaging replays the blocks from the
+                   first such block start in one run carried on to nt, so its
+                   checkpointing pass ends at that block's checkpoint.  Same
+                   results.
     """
 
     strict_cfl: bool = True

@@ -166,21 +166,15 @@ def sub_checkpoint_origin(segs: list[tuple[int, int]]) -> int:
     return segs[1][0] - 1
 
 
-def keep_tail_start(wavelets, nt: int, interval: int, room: int) -> int | None:
-    """First step T of the tail whose source fields the checkpointing pass
-    keeps (``record_source_wavefields``' ``tail``), or None.
-
-    Replaying a block restarts from the state after step s0 and steps it
-    with wavelet[s0], wavelet[s0 + 1], ... (ertm/imaging.py:150-160), while
-    the forward pass stepped that state with wavelet[s0 + 1], ...  Where
-    every wavelet is bitwise constant from s0 to nt - 1 the two are the same
-    operations on the same state, so replay step t extracts what forward
-    step t + 1 does.  T is the latest of the first block start with that
-    property and nt - ``room`` (the replay slots there are); None when the
-    tail would hold no more than the last block, whose forward steps the
-    pass otherwise skips."""
+def constant_block_start(wavelets, nt: int, interval: int) -> int | None:
+    """The first checkpoint block start s0 from which every wavelet is
+    bitwise constant up to step nt - 1, or None when there is none before
+    nt.  From s0 on, replaying a block restarted from the state after step
+    s0 steps it with wavelet[s0], wavelet[s0 + 1], ... (ertm/imaging.py:
+    150-160) exactly as the forward pass stepped it with wavelet[s0 + 1],
+    ...: replay step t repeats forward step t + 1, and a replay run on past
+    a block's end produces the next block's replayed fields."""
     nt, interval = int(nt), max(1, int(interval))
-    last = (nt - 1) // interval * interval
     z = 0
     for w in wavelets:
         u = np.ascontiguousarray(np.asarray(w, dtype=np.float32)[:nt]).view(np.uint32)
@@ -189,6 +183,23 @@ def keep_tail_start(wavelets, nt: int, interval: int, room: int) -> int | None:
         diff = np.nonzero(u != u[nt - 1])[0]
         z = max(z, int(diff[-1]) + 1 if diff.size else 0)
     first = -(-z // interval) * interval
+    return first if first < nt else None
+
+
+def keep_tail_start(wavelets, nt: int, interval: int, room: int) -> int | None:
+    """First step T of the tail whose source fields the checkpointing pass
+    keeps (``record_source_wavefields``' ``tail``), or None.
+
+    Where every wavelet is bitwise constant from a block start on
+    (``constant_block_start``), replay step t extracts what forward step
+    t + 1 does.  T is the latest of that block start and nt - ``room`` (the
+    replay slots there are); None when the tail would hold no more than the
+    last block, whose forward steps the pass otherwise skips."""
+    nt, interval = int(nt), max(1, int(interval))
+    last = (nt - 1) // interval * interval
+    first = constant_block_start(wavelets, nt, interval)
+    if first is None:
+        return None
     t = max(first, nt - max(int(room), 0))
     return t if t < last else None
 
@@ -219,7 +230,8 @@ def record_source_wavefield(prop: Propagator2D, wavelet: np.ndarray, *,
 def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarray], *,
                              budget_mb: float, variant=None, graph_steps: int = 0,
                              during=None, to_last_checkpoint: bool = False,
-                             two_step=(0, 0), tail=None) -> list[CheckpointStore]:
+                             two_step=(0, 0), tail=None,
+                             end_after: int | None = None) -> list[CheckpointStore]:
     """``record_source_wavefield`` for several shots advanced in one launch per step.
 
     All shots must share nt and the checkpoint interval (the interval follows
@@ -229,7 +241,8 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     ``to_last_checkpoint`` ends the pass at the step that saves the last
     checkpoint instead of at nt: a migration restarts every block from its
     checkpoint, so the steps after the last one produce nothing it reads
-    (the checkpoints are the same either way).
+    (the checkpoints are the same either way).  ``end_after`` (a block
+    start) ends it after the step that saves that block's checkpoint.
 
     ``tail = (T, imgs)`` (``keep_tail_start``): the pass saves the
     checkpoints up to the state after step T, then runs steps T + 1 .. nt
@@ -274,6 +287,10 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     si0 = step0.pop()
     if tail is None:
         t_end = starts[-1] + 1 if to_last_checkpoint else nt
+        if end_after is not None:
+            if int(end_after) not in starts:
+                raise ImagingError(f"end_after {end_after} is not a checkpoint block start")
+            t_end = int(end_after) + 1
         cur = b.run(0, 0, t_end, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
                     save=arenas, save_stride=stride, step_index0=si0, **kw)
     else:
@@ -700,11 +717,20 @@ class RTMImager:
         # first ones when it keeps the source fields of the last steps.
         tail = None
         t_keep = nt
-        if condition != "excitation":
+        # excitation: from the first block start at which every wavelet is
+        # constant, one replay run carried on to nt replays all later blocks
+        # (and their checkpoints are not needed)
+        t_chain = nt
+        interval = _common({checkpoint_interval(nt, max(int(f.state_bytes()), 1),
+                                                float(self.config.checkpoint_mb))
+                            for f in fwds}, "checkpoint interval")
+        if condition == "excitation":
+            first = (constant_block_start(waves, nt, interval)
+                     if self.gpu_config.keep_tail else None)
+            if first is not None:
+                t_chain = first
+        else:
             cells = sum(m.shape[0] * m.shape[1] for m in models)
-            interval = _common({checkpoint_interval(nt, max(int(f.state_bytes()), 1),
-                                                    float(self.config.checkpoint_mb))
-                                for f in fwds}, "checkpoint interval")
             arena_bytes = (4 * len(range(0, nt, interval))
                            * max(f._core.state_floats for f in fwds) * n)
             # Later batches of a shape take no more slots than the first: a
@@ -736,7 +762,8 @@ class RTMImager:
             stores = record_source_wavefields(
                 fwds, waves, budget_mb=float(self.config.checkpoint_mb), variant=kv,
                 graph_steps=gsteps, during=setup.join if setup else build_backs,
-                to_last_checkpoint=True, two_step=ts, tail=tail)
+                to_last_checkpoint=True, two_step=ts, tail=tail,
+                end_after=t_chain if t_chain < nt else None)
         except BaseException:
             if setup is not None:
                 setup.wait()
@@ -773,6 +800,10 @@ class RTMImager:
             fb = dev.Batch(fcores, imgs=ibs, variant=kv, graph_steps=gsteps)
             for blk in range(nblocks):
                 s0, s1 = store0.bounds(blk, nt)
+                if s0 > t_chain:
+                    break
+                if s0 == t_chain:
+                    s1 = nt  # the chained replay of every later block
                 fcur = fb.run(1, s0, s1, fcur, wavelets=ws, op_after=L.IMG_EXC_PEAK,
                               restore=arenas, restore_stride=astride,
                               restore_slot=store0.states[blk][1])

@@ -14,7 +14,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from ertm.imaging import keep_tail_start
+from ertm.imaging import constant_block_start, keep_tail_start
 from oracle import rtm as ortm
 
 
@@ -79,6 +79,51 @@ def test_replayed_fields_are_the_next_forward_fields_once_the_wavelet_is_constan
         else:
             # w still varies inside or after the block start: not the same
             assert not shifted, f"block {b} at {s0}"
+
+
+@pytest.mark.parametrize("engine,sep,boundary", [("elastic", "divcurl", "free"),
+                                                 ("acoustic", "stress", "cpml")])
+def test_a_replay_carried_on_past_its_block_replays_the_next_blocks(engine, sep, boundary):
+    """The excitation maps replay the blocks in order; from the first block
+    at which the wavelet is constant, one replay run on to nt gives every
+    later block's fields, so their checkpoints are not needed."""
+    nz, nx, dt, nt, n_vary = 26, 30, 1.0e-3, 36, 11
+    m = _model(nz, nx, 7)
+    w = _wavelet(nt, n_vary, -0.0, 9)
+
+    def prop():
+        p = ortm.build(engine, m, dt, nt, order=8, npml=8, boundary=boundary)
+        p.set_source(140.0, 50.0)
+        return p
+
+    st = ortm.record(prop(), w, budget_mb=1.0)
+    starts = [s0 for s0, _ in st.states]
+    first = constant_block_start([w], nt, st.interval)
+    assert first == -(-n_vary // st.interval) * st.interval
+    replayer = prop()
+    per_block = {}
+    for b, s0 in enumerate(starts):
+        for i, f in enumerate(ortm.replay(replayer, w, st, b, sep)):
+            per_block[s0 + i] = f
+    for b0 in (starts.index(first), starts.index(first) - 1):
+        # one run from block b0's checkpoint to nt
+        replayer.load_state(st.states[b0][1])
+        chained = []
+        for t in range(starts[b0], nt):
+            replayer.step(float(w[t]))
+            chained.append({k: np.array(v, copy=True)
+                            for k, v in replayer.image_fields(sep).items()})
+        same = all(_same(f, per_block[starts[b0] + i]) for i, f in enumerate(chained))
+        assert same == (starts[b0] >= first), (b0, starts[b0], first)
+
+
+def test_constant_block_start():
+    nt, interval = 100, 10
+    assert constant_block_start([_wavelet(nt, 31, 0.0, 1)], nt, interval) == 40
+    assert constant_block_start([_wavelet(nt, 30, 0.0, 1)], nt, interval) == 30
+    assert constant_block_start([_wavelet(nt, 31, 0.0, 1), _wavelet(nt, 95, 1.5, 2)], nt,
+                                interval) is None
+    assert constant_block_start([np.zeros(nt, np.float32)], nt, interval) == 0
 
 
 def _ricker(nt, f0, delay, dt):

@@ -205,6 +205,11 @@ class GpuConfig:
                    first such block start in one run carried on to nt, so its
                    checkpointing pass ends at that block's checkpoint.  Same
                    results.
+    tail_budget_gb HBM the kept tail may fill: None = the replay budget (a
+                   batch migration's default); 0 = no more than the
+                   whole-block replay slots the migration holds anyway (the
+                   serving loop's default: its requests share the HBM, and a
+                   request's tail slots would cost the batching of others).
     """
 
     strict_cfl: bool = True
@@ -219,6 +224,7 @@ class GpuConfig:
     two_step: tuple | None = None
     device_resample: bool = True
     keep_tail: bool = True
+    tail_budget_gb: float | None = None
 
     def validate(self) -> None:
         if self.device_mem_gb <= 0:
@@ -227,6 +233,8 @@ class GpuConfig:
             raise ConfigError("gpu.replay_slots must not be negative")
         if self.replay_budget_gb <= 0:
             raise ConfigError("gpu.replay_budget_gb must be positive")
+        if self.tail_budget_gb is not None and self.tail_budget_gb < 0:
+            raise ConfigError("gpu.tail_budget_gb must be non-negative or null")
         if self.max_batch < 1:
             raise ConfigError("gpu.max_batch must be at least 1")
         if self.image_ring < 0:

@@ -204,13 +204,22 @@ def keep_tail_start(wavelets, nt: int, interval: int, room: int) -> int | None:
     return t if t < last else None
 
 
-def slot_plan(wavelets, nt: int, interval: int, room: int,
-              keep_tail: bool = True) -> tuple[int, int | None]:
-    """(replay slots, kept tail start or None) of a migration whose slots may
-    hold ``room`` steps: a whole block (fewer when ``room`` is smaller), or
-    the kept tail when it is longer."""
+def slot_plan(wavelets, nt: int, interval: int, room: int, gpu: GpuConfig,
+              cells: int) -> tuple[int, int | None]:
+    """(replay slots, kept tail start or None) of a migration of ``cells``
+    cells (summed over the shots advanced together) whose slots may hold
+    ``room`` steps: a whole block (fewer when ``room`` is smaller), or the
+    kept tail when it is longer; the tail fills at most
+    ``gpu.tail_budget_gb`` (None: all of ``room``; 0: the whole-block
+    slots)."""
     slots = max(1, min(int(interval), int(room)))
-    t_keep = keep_tail_start(wavelets, nt, interval, room) if keep_tail else None
+    t_keep = None
+    if gpu.keep_tail:
+        t_room = int(room)
+        if gpu.tail_budget_gb is not None:
+            extra = int(float(gpu.tail_budget_gb) * 1024 ** 3 // (4 * max(int(cells), 1)))
+            t_room = min(t_room, max(slots, extra))
+        t_keep = keep_tail_start(wavelets, nt, interval, t_room)
     if t_keep is not None:
         slots = max(slots, int(nt) - t_keep)
     return slots, t_keep
@@ -741,7 +750,7 @@ class RTMImager:
             room = min(self._slot_room(cells, reserve=arena_bytes),
                        self._rooms.get(key, 1 << 62))
             self._rooms[key] = room
-            slots, kept = slot_plan(waves, nt, interval, room, self.gpu_config.keep_tail)
+            slots, kept = slot_plan(waves, nt, interval, room, self.gpu_config, cells)
             if kept is not None:
                 t_keep = kept
             rbufs = [torch.empty((slots, m.shape[0], m.shape[1]), dtype=torch.float32,

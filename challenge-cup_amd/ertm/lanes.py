@@ -70,7 +70,7 @@ def lane_replay_slots(imager: RTMImager, interval: int, cells: int, wave=None,
     room = lane_slot_room(gpu, cells)
     if wave is None:
         return max(1, min(interval, room)), None
-    return slot_plan([wave], nt, interval, room, gpu.keep_tail)
+    return slot_plan([wave], nt, interval, room, gpu, cells)
 
 
 class Lane:

@@ -133,7 +133,7 @@ def request_bytes(req: ShotRequest, gpu: GpuConfig | None = None) -> int:
     cells = nz * nx
     # a whole block, or the kept tail of the checkpointing pass
     wave = source_wavelet(req.shot.source, spec.dt, nt)
-    slots, _ = slot_plan([wave], nt, interval, lane_slot_room(gpu, cells), gpu.keep_tail)
+    slots, _ = slot_plan([wave], nt, interval, lane_slot_room(gpu, cells), gpu, cells)
     sub = math.ceil(interval / slots) * state if slots < interval else 0
     nmodes = len(req.modes or (req.imaging.mode,))
     accum = (2 * nmodes + 1 + 3) * cells * 2                      # float64 -> 2 floats
@@ -144,6 +144,18 @@ def request_bytes(req: ShotRequest, gpu: GpuConfig | None = None) -> int:
     record = nt * nrec * ncomp + 16 * nrec * ncomp
     floats = props + arena + slots * cells + sub + accum + ring + record
     return int(floats * 4 * 1.05) + (8 << 20)
+
+
+def serving_gpu_config(gpu: GpuConfig | None) -> GpuConfig:
+    """The engine options a serving loop runs with: ``gpu`` (default
+    ``GpuConfig()``) with kept tails only inside the whole-block replay
+    slots a request holds anyway unless ``tail_budget_gb`` is set.  The
+    requests share the HBM, and slots one request's tail took would cost
+    the batching of others."""
+    gpu = gpu or GpuConfig()
+    if gpu.tail_budget_gb is None:
+        gpu = dataclasses.replace(gpu, tail_budget_gb=0.0)
+    return gpu
 
 
 def _key(req: ShotRequest, imager_key) -> tuple:
@@ -292,7 +304,7 @@ class ServingLoop:
         if streams < 1 or max_group < 1 or hbm_budget_gb <= 0 or lane_ticks < 1:
             raise ValueError("streams, max_group, lane_ticks and hbm_budget_gb must be positive")
         self.budget = int(float(hbm_budget_gb) * _GIB)
-        self.gpu_config = gpu_config or GpuConfig()
+        self.gpu_config = serving_gpu_config(gpu_config)
         self.lanes = bool(lanes)
         self.lane_ticks = int(lane_ticks)
         self._lock = threading.Lock()

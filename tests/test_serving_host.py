@@ -120,3 +120,28 @@ def test_one_engine_does_not_block_the_other():
     a0 = adm.take()
     assert (a0[0], a0[1], _names(a0)) == ("lane", 0, ["a0"]) and adm.kind == ["acoustic", "elastic"]
     assert not adm.draining and not adm.pending
+
+
+def test_serving_admission_counts_kept_tails_only_inside_the_block_slots():
+    """The default SourceSpec's Ricker wavelet is bitwise constant long
+    before nt = 4000, so a batch migration keeps a tail in up to the replay
+    budget; the serving loop keeps tails only inside the whole-block slots,
+    so a request's HBM price is the one without a tail."""
+    from ertm.serving import serving_gpu_config
+
+    r = _request(600, 1200, 4000)
+    plain = request_bytes(r, GpuConfig(keep_tail=False))
+    assert request_bytes(r, GpuConfig(tail_budget_gb=0.0)) == plain
+    assert request_bytes(r) > plain + 8 * 2 ** 30
+    # a tail budget between the two prices the tail it allows
+    assert plain < request_bytes(r, GpuConfig(tail_budget_gb=4.0)) < request_bytes(r)
+    assert serving_gpu_config(None).tail_budget_gb == 0.0
+    assert serving_gpu_config(GpuConfig(tail_budget_gb=2.0)).tail_budget_gb == 2.0
+    from ertm.exceptions import ConfigError
+
+    try:
+        GpuConfig(tail_budget_gb=-1.0).validate()
+    except ConfigError:
+        pass
+    else:
+        raise AssertionError("a negative tail budget was accepted")

@@ -33,11 +33,19 @@ struct ElState {
     float q_vz[2 * K + 1];                                      // rows k-K*D .. k+K*D
     float ir_c, mu_c;  // own column, carried from the previous row (see el_iter)
     float n_sxx, n_szz, n_sxz, n_vx, n_vz, n_ir, n_mu, n_lam;   // inputs of this row
+    unsigned i_ir, i_mu, i_lam;  // with a medium palette: their indices, looked up in el_iter
 };
 
 // Load the inputs of iteration k: stress row k+2K*D, velocity row m = k+K*D,
 // 1/rho row m+1 (down) / m (up), mu row k+1 (down) / k (up), lambda row k.
-template <int K, int QXL, int D>
+// Medium palette index of padded row offset `rowb` (bytes of a float row) at
+// the lane's column: one byte per cell, rows on the float rows' pitch / 4.
+// Rows outside the grid read 0, the zero medium.
+__device__ __forceinline__ unsigned mindex(const ElBlock& B, int rowb) {
+    return __builtin_amdgcn_raw_buffer_load_b8(B.rMI, B.c + (rowb >> 2), 0, 0);
+}
+
+template <int K, int QXL, int D, bool PAL>
 __device__ __forceinline__ void el_fetch(const ElBlock& B, ElState<K, QXL>& X, int k) {
     const int ld4 = B.ld4;
     const int rs = (k + 2 * K * D) * ld4, rv = (k + K * D) * ld4;
@@ -47,9 +55,15 @@ __device__ __forceinline__ void el_fetch(const ElBlock& B, ElState<K, QXL>& X, i
     X.n_sxz = bload(B.rS, B.c4 + (rs + B.fXZ));
     X.n_vx = bload(B.rS, B.c4 + (rv + B.fVX));
     X.n_vz = bload(B.rS, B.c4 + (rv + B.fVZ));
-    X.n_ir = bload(B.rIR, B.c4 + ri);
-    X.n_mu = bload(B.rMU, B.c4 + rm);
-    X.n_lam = bload(B.rLAM, B.c4 + k * ld4);
+    if constexpr (PAL) {
+        X.i_ir = mindex(B, ri);
+        X.i_mu = mindex(B, rm);
+        X.i_lam = mindex(B, k * ld4);
+    } else {
+        X.n_ir = bload(B.rIR, B.c4 + ri);
+        X.n_mu = bload(B.rMU, B.c4 + rm);
+        X.n_lam = bload(B.rLAM, B.c4 + k * ld4);
+    }
 }
 
 // vx queue length: Dz+vx at row k reads rows k-K+1 .. k+K, i.e. K rows ahead
@@ -71,13 +85,18 @@ constexpr int qx_len() { return (!img_may_s<IMG>() && D > 0) ? 2 * K : 2 * K + 1
 // m = k + D*K.  1/rho is needed at rows m and m+1 and mu at rows k and k+1:
 // the row reached first along the stream is carried, the other one loaded.
 // L = this iteration's LDS buffer (7 rows of NT floats).
-template <int K, int NT, bool STRIP, bool STRESS, int IMG, int D>
+template <int K, int NT, bool STRIP, bool STRESS, int IMG, int D, bool PAL>
 __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, qx_len<K, IMG, D>()>& X,
                                         int k, float* L) {
     constexpr int QS = 2 * K + 1, QX = qx_len<K, IMG, D>(), QZ = 2 * K + 1;
     constexpr int XO = QX - 2 * K;  // q_vx[i] holds row k + (i - (K - 1 + XO)) * D
     const int tid = B.tid, ld4 = B.ld4, nzp = B.nzp;
     const int m = k + D * K;
+    if constexpr (PAL) {  // this row's medium from the palette (the same float values)
+        X.n_ir = B.ptab[X.i_ir];
+        X.n_mu = B.ptab[256 + X.i_mu];
+        X.n_lam = B.ptab[512 + X.i_lam];
+    }
     const float vx_old = X.n_vx, vz_old = X.n_vz, lam_k = X.n_lam;
     const float ir_m = D > 0 ? X.ir_c : X.n_ir;   // 1/rho, row m
     const float ir_m1 = D > 0 ? X.n_ir : X.ir_c;  // row m+1
@@ -93,7 +112,7 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, qx_len<K, I
     X.q_szz[QS - 1] = X.n_szz;
     X.q_sxz[QS - 1] = X.n_sxz;
     // prefetch the next iteration's inputs; in flight during this row
-    el_fetch<K, QX, D>(B, X, k + D);
+    el_fetch<K, QX, D, PAL>(B, X, k + D);
 
     // ---- LDS rows of this iteration, one barrier: stress row m and 1/rho
     // row m (velocity x-derivatives); velocity row k, computed K iterations
@@ -326,7 +345,7 @@ __device__ __forceinline__ void el_iter(const ElBlock& B, ElState<K, qx_len<K, I
 
 // One chunk: 2K prologue rows (velocity only), then the chunk's rows (both
 // updates), streaming in direction D.
-template <int K, int NT, bool STRIP, int IMG, int D>
+template <int K, int NT, bool STRIP, int IMG, int D, bool PAL>
 __device__ __forceinline__ void el_loop(const ElBlock& B, float* lds) {
     constexpr int QX = qx_len<K, IMG, D>(), QZ = 2 * K + 1;
     const int ld4 = B.ld4;
@@ -344,21 +363,27 @@ __device__ __forceinline__ void el_loop(const ElBlock& B, float* lds) {
         X.q_szz[q + 1] = bload(B.rS, B.c4 + (r + B.fZZ));
         X.q_sxz[q + 1] = bload(B.rS, B.c4 + (r + B.fXZ));
     }
-    X.ir_c = bload(B.rIR, B.c4 + (kbeg + D * K + (D > 0 ? 0 : 1)) * ld4);
-    X.mu_c = bload(B.rMU, B.c4 + (kbeg + (D > 0 ? 0 : 1)) * ld4);
-    el_fetch<K, QX, D>(B, X, kbeg);
+    const int ri0 = (kbeg + D * K + (D > 0 ? 0 : 1)) * ld4, rm0 = (kbeg + (D > 0 ? 0 : 1)) * ld4;
+    if constexpr (PAL) {
+        X.ir_c = B.ptab[mindex(B, ri0)];
+        X.mu_c = B.ptab[256 + mindex(B, rm0)];
+    } else {
+        X.ir_c = bload(B.rIR, B.c4 + ri0);
+        X.mu_c = bload(B.rMU, B.c4 + rm0);
+    }
+    el_fetch<K, QX, D, PAL>(B, X, kbeg);
     int k = kbeg;
     int par = 0;
     for (int p = 0; p < 2 * K; ++p, k += D, par ^= 1)
-        el_iter<K, NT, STRIP, false, IMG, D>(B, X, k, lds + par * 7 * NT);
+        el_iter<K, NT, STRIP, false, IMG, D, PAL>(B, X, k, lds + par * 7 * NT);
     for (int p = B.z0; p < B.z1; ++p, k += D, par ^= 1)
-        el_iter<K, NT, STRIP, true, IMG, D>(B, X, k, lds + par * 7 * NT);
+        el_iter<K, NT, STRIP, true, IMG, D, PAL>(B, X, k, lds + par * 7 * NT);
 }
 
 // The plain step and the no-S imaging variants fit 96 VGPRs, i.e. 5 waves
 // per SIMD instead of 4; the variants with the S field would spill there and
 // keep 4.
-template <int K, int NT, int IMG>
+template <int K, int NT, int IMG, bool PAL>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(img_may_s<IMG>() ? 4 : 5)))
 k_elastic_fused(const ErtmProp* __restrict__ props,
                                                       const ErtmImage* __restrict__ imgs,
@@ -482,6 +507,15 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
     }
 
     __shared__ float lds[2 * 7 * NT];  // two buffers of 7 rows (see el_iter)
+    // the medium palette: 1/rho, mu, lambda of each index (entry 0: zeros)
+    __shared__ float ptab[PAL ? 768 : 1];
+    if constexpr (PAL) {
+        B.rMI = make_rsrc(P.midx, (int64_t)nzp * P.ld);
+        gcfloat* tab = gp(P.mtab);
+        for (int i = B.tid; i < 768; i += NT) ptab[i] = tab[i];
+        B.ptab = ptab;
+        __syncthreads();
+    }
     // Does the block (columns x0-2K .. x0+TX+2K, rows z0-2K .. z1+2K) reach a
     // CPML strip?  Uniform per block.
     const int cx0 = x0 - 2 * K, cx1 = x0 + TX + 2 * K;
@@ -494,11 +528,11 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
     // hits in L2.
     const bool up = (chunk & 1) != 0;
     if (strip) {
-        if (up) el_loop<K, NT, true, IMG, -1>(B, lds);
-        else el_loop<K, NT, true, IMG, 1>(B, lds);
+        if (up) el_loop<K, NT, true, IMG, -1, PAL>(B, lds);
+        else el_loop<K, NT, true, IMG, 1, PAL>(B, lds);
     } else {
-        if (up) el_loop<K, NT, false, IMG, -1>(B, lds);
-        else el_loop<K, NT, false, IMG, 1>(B, lds);
+        if (up) el_loop<K, NT, false, IMG, -1, PAL>(B, lds);
+        else el_loop<K, NT, false, IMG, 1, PAL>(B, lds);
     }
 }
 
@@ -922,9 +956,18 @@ template <int K, int NT, int IMG>
 static void launch_fused_k(const ErtmProp* dprops, const ErtmImage* dimg,
                            const ErtmProp* hprops, const LaunchArgs& la, int zc, hipStream_t s) {
     int nb = 0;
-    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la_prop(la, k)], zc));
-    hipLaunchKernelGGL((k_elastic_fused<K, NT, IMG>), dim3(nb, la.n), dim3(NT), 0, s, dprops, dimg,
-                       la, zc);
+    bool pal = true;  // every member has a medium palette
+    for (int k = 0; k < la.n; ++k) {
+        const ErtmProp& P = hprops[la_prop(la, k)];
+        nb = max(nb, blocks_for<K, NT>(P, zc));
+        pal = pal && P.npal > 0 && P.npal < 256 && P.midx && P.mtab;
+    }
+    if (pal)
+        hipLaunchKernelGGL((k_elastic_fused<K, NT, IMG, true>), dim3(nb, la.n), dim3(NT), 0, s,
+                           dprops, dimg, la, zc);
+    else
+        hipLaunchKernelGGL((k_elastic_fused<K, NT, IMG, false>), dim3(nb, la.n), dim3(NT), 0, s,
+                           dprops, dimg, la, zc);
 }
 
 template <int K, int NT>

@@ -84,6 +84,8 @@ constexpr bool img_may_s() {
 // Per-block constants of the fused elastic step.
 struct ElBlock {
     rsrc_t rS, rT, rIR, rLAM, rMU;
+    rsrc_t rMI;         // medium palette indices (one byte per cell; ErtmProp.midx)
+    const float* ptab;  // the palette in LDS: 1/rho, mu, lambda at 0, 256, 512
     gcfloat* AZ;
     gcfloat* BZ;
     int nzp, nxp, z0, z1, c, c4, c4own, tid, ld4;

@@ -16,7 +16,7 @@ from .exceptions import ERTMError, SolverError
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libertm_hip.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 MAX_LAUNCH = 64
 MAX_MECH = 8
 NOALLOC_MAX_CELLS = 1 << 21  # ERTM_NOALLOC_MAX_CELLS: image cells of ertm_postprocess_noalloc
@@ -52,6 +52,7 @@ class ErtmProp(C.Structure):
         ("state", C.c_void_p * 2),
         ("src_iz", C.c_int32), ("src_ix", C.c_int32), ("mechanism", C.c_int32),
         ("_pad0", C.c_int32),
+        ("midx", C.c_void_p), ("mtab", C.c_void_p), ("npal", C.c_int32), ("_pad1", C.c_int32),
     ]
 
 

@@ -581,7 +581,11 @@ class ElasticPropagator(Propagator2D):
             mu = (rho * vs * vs).astype(np.float32)
             lam = (rho * (vp * vp - 2.0 * vs * vs)).astype(np.float32)
             irho = (1.0 / rho).astype(np.float32)
-            return {"lam": lam, "mu": mu, "irho": irho}
+            out = {"lam": lam, "mu": mu, "irho": irho}
+            pal = dev.medium_palette([irho, mu, lam])
+            if pal is not None:
+                out["midx"], out["mtab"], out["npal"] = pal
+            return out
 
         key = dev.media_digest("elastic", self.pad, model.vp, model.vs, model.rho)
         self._make_core(dev.host_media(key, build), media_key=key)

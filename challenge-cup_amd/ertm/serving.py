@@ -126,6 +126,8 @@ def request_bytes(req: ShotRequest, gpu: GpuConfig | None = None) -> int:
     psi = nterms * (nzp * 2 * w + 2 * w * ld) + 4 * nterms * 64
     state = nf * _round_up(nzp * ld, 64) + psi                    # floats, one buffer
     media = (3 if engine == "elastic" else 3 + nmech) * nzp * ld
+    if engine == "elastic":
+        media += nzp * ld // 4 + 768                              # medium palette
     props = 2 * (2 * state + media)                               # forward + backward
     nt = int(spec.nt)
     interval = checkpoint_interval(nt, nf * nzp * nxp * 4, float(req.imaging.checkpoint_mb))

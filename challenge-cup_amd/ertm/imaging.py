@@ -280,6 +280,12 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     curs = {c.cur for c in cores}
     if len(step0) != 1 or len(curs) != 1:
         raise ImagingError("batched propagators must be at the same step")
+    if tail is not None and end_after is not None:
+        raise ImagingError("a kept tail runs the pass to nt: no end_after with it")
+    if tail is not None and not 0 <= int(tail[0]) < nt - 1:
+        raise ImagingError(f"kept tail start {tail[0]} outside [0, {nt - 1})")
+    if end_after is not None and int(end_after) not in starts:
+        raise ImagingError(f"end_after {end_after} is not a checkpoint block start")
     b = dev.Batch(cores, imgs=tail[1] if tail is not None else None, variant=variant,
                   graph_steps=graph_steps, two_step=two_step)
     if variant is None:
@@ -297,15 +303,11 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
     if tail is None:
         t_end = starts[-1] + 1 if to_last_checkpoint else nt
         if end_after is not None:
-            if int(end_after) not in starts:
-                raise ImagingError(f"end_after {end_after} is not a checkpoint block start")
             t_end = int(end_after) + 1
         cur = b.run(0, 0, t_end, cores[0].cur, wavelets=ws, ck_interval=interval, ck_origin=0,
                     save=arenas, save_stride=stride, step_index0=si0, **kw)
     else:
         t_keep = int(tail[0])
-        if not 0 <= t_keep < nt - 1:
-            raise ImagingError(f"kept tail start {t_keep} outside [0, {nt - 1})")
         cur = b.run(0, 0, t_keep + 1, cores[0].cur, wavelets=ws, ck_interval=interval,
                     ck_origin=0, save=arenas, save_stride=stride, step_index0=si0, **kw)
         # no checkpoint is read from here on: every later block lies in the tail

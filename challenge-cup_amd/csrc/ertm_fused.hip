@@ -41,7 +41,8 @@ struct ElState {
 // Medium palette index of padded row offset `rowb` (bytes of a float row) at
 // the lane's column: one byte per cell, rows on the float rows' pitch / 4.
 // Rows outside the grid read 0, the zero medium.
-__device__ __forceinline__ unsigned mindex(const ElBlock& B, int rowb) {
+template <class Blk>
+__device__ __forceinline__ unsigned mindex(const Blk& B, int rowb) {
     return __builtin_amdgcn_raw_buffer_load_b8(B.rMI, B.c + (rowb >> 2), 0, 0);
 }
 
@@ -553,6 +554,8 @@ k_elastic_fused(const ErtmProp* __restrict__ props,
 // ---------------------------------------------------------------------------
 struct AcBlock {
     rsrc_t rS, rT, rIR, rKA;
+    rsrc_t rMI;         // medium palette indices (acoustic; ErtmProp.midx)
+    const float* ptab;  // the palette in LDS: 1/rho at 0, kappa at 256
     gcfloat* AZ;
     gcfloat* BZ;
     int nzp, nxp, z0, z1, c, c4, c4own, tid, ld4;
@@ -586,11 +589,12 @@ struct AcState {
     float n_p, n_vx, n_vz, n_ir, n_ka;  // inputs of this row
     float n_eps, n_r[VL > 0 ? VL : 1];  // viscoacoustic inputs of row k
     int n_q;
+    unsigned i_ir, i_ka;  // with a medium palette: indices of n_ir, n_ka (looked up in ac_iter)
 };
 
 // Inputs of iteration k: pressure row k+2K*D, old velocities of row
 // m = k+K*D, 1/rho row m+1 (down) / m (up), kappa row k.
-template <int K, int VL, int D>
+template <int K, int VL, int D, bool PAL>
 __device__ __forceinline__ void ac_fetch(const AcBlock& B, AcState<K, VL>& X, int k) {
     const int ld4 = B.ld4;
     const int rp = (k + 2 * K * D) * ld4, rv = (k + K * D) * ld4;
@@ -598,8 +602,13 @@ __device__ __forceinline__ void ac_fetch(const AcBlock& B, AcState<K, VL>& X, in
     X.n_p = bload(B.rS, B.c4 + (rp + B.fP));
     X.n_vx = bload(B.rS, B.c4 + (rv + B.fVX));
     X.n_vz = bload(B.rS, B.c4 + (rv + B.fVZ));
-    X.n_ir = bload(B.rIR, B.c4 + ri);
-    X.n_ka = bload(B.rKA, B.c4 + k * ld4);
+    if constexpr (PAL) {
+        X.i_ir = mindex(B, ri);
+        X.i_ka = mindex(B, k * ld4);
+    } else {
+        X.n_ir = bload(B.rIR, B.c4 + ri);
+        X.n_ka = bload(B.rKA, B.c4 + k * ld4);
+    }
     if constexpr (VL > 0) {
         const int rk = k * ld4;
         X.n_eps = bload(B.rS, B.c4 + (rk + B.fEPS));
@@ -612,11 +621,15 @@ __device__ __forceinline__ void ac_fetch(const AcBlock& B, AcState<K, VL>& X, in
 // IMG: ERTM_FIMG_NONE, ERTM_FIMG_EXTRACT (the new pressure into the replay
 // slot) or ERTM_FIMG_RING (into ring slot t mod R, every receiver field):
 // the acoustic image field is the pressure (k_image's wave_value).
-template <int K, int NT, bool STRIP, bool PRES, int IMG, int VL, int D>
+template <int K, int NT, bool STRIP, bool PRES, int IMG, int VL, int D, bool PAL>
 __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K, VL>& X, int k, float* L) {
     constexpr int QP = 2 * K + 1;
     const int tid = B.tid, ld4 = B.ld4, nzp = B.nzp;
     const int m = k + D * K;
+    if constexpr (PAL) {  // this row's medium from the palette (the same float values)
+        X.n_ir = B.ptab[X.i_ir];
+        X.n_ka = B.ptab[256 + X.i_ka];
+    }
     const float vx_old = X.n_vx, vz_old = X.n_vz, ka_k = X.n_ka;
     float eps_k = 0.0f, r_k[VL > 0 ? VL : 1];
     int q_k = 0;
@@ -631,7 +644,7 @@ __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K, VL>& X, int
 #pragma unroll
     for (int q = 0; q < QP - 1; ++q) X.q_p[q] = X.q_p[q + 1];
     X.q_p[QP - 1] = X.n_p;
-    ac_fetch<K, VL, D>(B, X, k + D);
+    ac_fetch<K, VL, D, PAL>(B, X, k + D);
 
     float* l_p = L;
     float* l_ir = L + NT;
@@ -766,7 +779,7 @@ __device__ __forceinline__ void ac_iter(const AcBlock& B, AcState<K, VL>& X, int
     X.ir_c = D > 0 ? ir_m1 : ir_m;
 }
 
-template <int K, int NT, bool STRIP, int IMG, int VL, int D>
+template <int K, int NT, bool STRIP, int IMG, int VL, int D, bool PAL>
 __device__ __forceinline__ void ac_loop(const AcBlock& B, float* lds) {
     const int ld4 = B.ld4;
     AcState<K, VL> X;
@@ -778,17 +791,19 @@ __device__ __forceinline__ void ac_loop(const AcBlock& B, float* lds) {
 #pragma unroll
     for (int q = 0; q < 2 * K; ++q)  // pressure rows kbeg + q*D -> slots 1..2K
         X.q_p[q + 1] = bload(B.rS, B.c4 + ((kbeg + q * D) * ld4 + B.fP));
-    X.ir_c = bload(B.rIR, B.c4 + (kbeg + D * K + (D > 0 ? 0 : 1)) * ld4);
-    ac_fetch<K, VL, D>(B, X, kbeg);
+    const int ri0 = (kbeg + D * K + (D > 0 ? 0 : 1)) * ld4;
+    if constexpr (PAL) X.ir_c = B.ptab[mindex(B, ri0)];
+    else X.ir_c = bload(B.rIR, B.c4 + ri0);
+    ac_fetch<K, VL, D, PAL>(B, X, kbeg);
     int k = kbeg;
     int par = 0;
     for (int p = 0; p < 2 * K; ++p, k += D, par ^= 1)
-        ac_iter<K, NT, STRIP, false, IMG, VL, D>(B, X, k, lds + par * 3 * NT);
+        ac_iter<K, NT, STRIP, false, IMG, VL, D, PAL>(B, X, k, lds + par * 3 * NT);
     for (int p = B.z0; p < B.z1; ++p, k += D, par ^= 1)
-        ac_iter<K, NT, STRIP, true, IMG, VL, D>(B, X, k, lds + par * 3 * NT);
+        ac_iter<K, NT, STRIP, true, IMG, VL, D, PAL>(B, X, k, lds + par * 3 * NT);
 }
 
-template <int K, int NT, int IMG, int VL>
+template <int K, int NT, int IMG, int VL, bool PAL>
 __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restrict__ props,
                                                        const ErtmImage* __restrict__ imgs,
                                                        LaunchArgs la, int zc) {
@@ -887,17 +902,26 @@ __global__ void __launch_bounds__(NT) k_acoustic_fused(const ErtmProp* __restric
     }
 
     __shared__ float lds[2 * 3 * NT];
+    // the medium palette: 1/rho and kappa of each index (entry 0: zeros)
+    __shared__ float ptab[PAL ? 512 : 1];
+    if constexpr (PAL) {
+        B.rMI = make_rsrc(P.midx, (int64_t)nzp * P.ld);
+        gcfloat* tab = gp(P.mtab);
+        for (int i = B.tid; i < 512; i += NT) ptab[i] = tab[i];
+        B.ptab = ptab;
+        __syncthreads();
+    }
     const int cx0 = x0 - 2 * K, cx1 = x0 + TX + 2 * K;
     const int rz0 = B.z0 - 2 * K, rz1 = B.z1 + 2 * K;
     const bool strip = (B.wx > 0 && (cx0 < B.wx || cx1 > nxp - B.wx)) ||
                        (B.wzt > 0 && rz0 < B.wzt) || (B.wzb > 0 && rz1 > nzp - B.wzb);
     const bool up = (chunk & 1) != 0;
     if (strip) {
-        if (up) ac_loop<K, NT, true, IMG, VL, -1>(B, lds);
-        else ac_loop<K, NT, true, IMG, VL, 1>(B, lds);
+        if (up) ac_loop<K, NT, true, IMG, VL, -1, PAL>(B, lds);
+        else ac_loop<K, NT, true, IMG, VL, 1, PAL>(B, lds);
     } else {
-        if (up) ac_loop<K, NT, false, IMG, VL, -1>(B, lds);
-        else ac_loop<K, NT, false, IMG, VL, 1>(B, lds);
+        if (up) ac_loop<K, NT, false, IMG, VL, -1, PAL>(B, lds);
+        else ac_loop<K, NT, false, IMG, VL, 1, PAL>(B, lds);
     }
 }
 
@@ -919,19 +943,42 @@ static bool launch_acoustic_k(const ErtmProp* dprops, const ErtmImage* dimg,
                               const ErtmProp* hprops, const LaunchArgs& la, int zc, int img,
                               hipStream_t s) {
     int nb = 0;
-    for (int k = 0; k < la.n; ++k) nb = max(nb, blocks_for<K, NT>(hprops[la_prop(la, k)], zc));
+    bool pal = VL == 0;  // acoustic, and every member has a medium palette
+    for (int k = 0; k < la.n; ++k) {
+        const ErtmProp& P = hprops[la_prop(la, k)];
+        nb = max(nb, blocks_for<K, NT>(P, zc));
+        pal = pal && P.npal > 0 && P.npal < 256 && P.midx && P.mtab;
+    }
     const dim3 g(nb, la.n);
-    if (img == ERTM_FIMG_NONE)
-        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_NONE, VL>), g, dim3(NT), 0, s,
-                           dprops, dimg, la, zc);
-    else if (img == ERTM_FIMG_EXTRACT || img == ERTM_FIMG_EXTRACT_NOS)
-        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_EXTRACT, VL>), g, dim3(NT), 0, s,
-                           dprops, dimg, la, zc);
-    else if (img == ERTM_FIMG_RING || img == ERTM_FIMG_RING_NOS)
-        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_RING, VL>), g, dim3(NT), 0, s,
-                           dprops, dimg, la, zc);
-    else
+    if (img != ERTM_FIMG_NONE && img != ERTM_FIMG_EXTRACT && img != ERTM_FIMG_EXTRACT_NOS &&
+        img != ERTM_FIMG_RING && img != ERTM_FIMG_RING_NOS)
         return false;
+    const int kind = img == ERTM_FIMG_NONE ? ERTM_FIMG_NONE
+                   : (img == ERTM_FIMG_RING || img == ERTM_FIMG_RING_NOS) ? ERTM_FIMG_RING
+                                                                            : ERTM_FIMG_EXTRACT;
+    if constexpr (VL == 0) {
+        if (pal) {
+            if (kind == ERTM_FIMG_NONE)
+                hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_NONE, 0, true>), g, dim3(NT),
+                                   0, s, dprops, dimg, la, zc);
+            else if (kind == ERTM_FIMG_EXTRACT)
+                hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_EXTRACT, 0, true>), g,
+                                   dim3(NT), 0, s, dprops, dimg, la, zc);
+            else
+                hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_RING, 0, true>), g, dim3(NT),
+                                   0, s, dprops, dimg, la, zc);
+            return true;
+        }
+    }
+    if (kind == ERTM_FIMG_NONE)
+        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_NONE, VL, false>), g, dim3(NT), 0, s,
+                           dprops, dimg, la, zc);
+    else if (kind == ERTM_FIMG_EXTRACT)
+        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_EXTRACT, VL, false>), g, dim3(NT), 0,
+                           s, dprops, dimg, la, zc);
+    else
+        hipLaunchKernelGGL((k_acoustic_fused<K, NT, ERTM_FIMG_RING, VL, false>), g, dim3(NT), 0, s,
+                           dprops, dimg, la, zc);
     return true;
 }
 

@@ -460,7 +460,11 @@ class AcousticPropagator(Propagator2D):
             vp = self._staged(model.vp)
             kappa = (rho * vp * vp).astype(np.float32)
             irho = (1.0 / rho).astype(np.float32)
-            return {"kappa": kappa, "irho": irho}
+            out = {"kappa": kappa, "irho": irho}
+            pal = dev.medium_palette([irho, kappa])
+            if pal is not None:
+                out["midx"], out["mtab"], out["npal"] = pal
+            return out
 
         key = dev.media_digest("acoustic", self.pad, model.vp, model.rho)
         self._make_core(dev.host_media(key, build), media_key=key)

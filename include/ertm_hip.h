@@ -114,11 +114,12 @@ typedef struct ErtmProp {
     int32_t src_iz, src_ix; /* padded source node, -1 = none */
     int32_t mechanism;      /* ERTM_SRC_* (elastic) */
     int32_t _pad0;
-    /* optional medium palette (elastic, npal > 0): one byte per padded cell,
-     * row r at midx + r*ld, indexing the float32 table mtab[768] = 1/rho,
-     * mu, lambda at [0, 256), [256, 512), [512, 768); index 0 is the zero
+    /* optional medium palette (elastic and acoustic, npal > 0): one byte per
+     * padded cell, row r at midx + r*ld, indexing the float32 table mtab:
+     * elastic 1/rho, mu, lambda at [0, 256), [256, 512), [512, 768);
+     * acoustic 1/rho, kappa at [0, 256), [256, 512).  Index 0 is the zero
      * medium (padding columns, and what rows outside the grid read), 1..npal
-     * the distinct (1/rho, mu, lambda) of the model.  The single-launch step
+     * the distinct tuples of the model.  The single-launch step
      * kernels then read one byte of medium per cell instead of three floats;
      * the values, and so the results, are the same.  npal = 0: no palette. */
     const uint8_t* midx;

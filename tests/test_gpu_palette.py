@@ -1,6 +1,6 @@
-"""Medium palette on the GPU: the single-launch elastic step reading one byte
-of medium per cell (an index into the model's distinct (1/rho, mu, lambda)
-tuples) instead of three float32 arrays.  Bar: bit-exact.  On layered models
+"""Medium palette on the GPU: the single-launch elastic and acoustic steps
+reading one byte of medium per cell (an index into the model's distinct
+(1/rho, mu, lambda) or (1/rho, kappa) tuples) instead of float32 arrays.  Bar: bit-exact.  On layered models
 (where the palette applies) the whole state after 40 steps, fields and CPML
 memories, equals the same launch shape without the palette and the
 two-launch kernels (which always read the float32 arrays), for every launch
@@ -67,8 +67,9 @@ def _layered_model(nz, nx, rng, dx=5.0, dz=4.0):
                         qp=np.full_like(vp, 100.0), qs=np.full_like(vp, 50.0))
 
 
+@pytest.mark.parametrize("engine", ["elastic", "acoustic"])
 @pytest.mark.parametrize("seed", range(8))
-def test_palette_step_equals_float_media_on_every_launch_shape(seed):
+def test_palette_step_equals_float_media_on_every_launch_shape(seed, engine):
     import torch
     from ertm import device as dev
     from ertm.propagation import PropagationSpec, build_propagator
@@ -76,12 +77,13 @@ def test_palette_step_equals_float_media_on_every_launch_shape(seed):
 
     rng = np.random.default_rng(700 + seed)
     boundary = ("cpml", "free", "none")[seed % 3]
-    mech = ("explosive", "shear", "force_z")[(seed // 3 + seed) % 3]
+    mech = (("explosive", "shear", "force_z")[(seed // 3 + seed) % 3] if engine == "elastic"
+            else "explosive")
     if seed == 7:
         nz, nx = 600, 1200
     else:
         nz, nx = int(rng.integers(23, 150)), int(rng.integers(19, 330))
-    _say("palette", seed, boundary, mech, nz, nx)
+    _say("palette", engine, seed, boundary, mech, nz, nx)
     m = _layered_model(nz, nx, rng)
     npml = int(rng.integers(5, 21))
     spec = PropagationSpec(dt=3e-4, nt=40, order=8, npml=npml, boundary=boundary)
@@ -94,7 +96,7 @@ def test_palette_step_equals_float_media_on_every_launch_shape(seed):
     for on in (True, False):
         with _palette(on):
             for v in shapes:
-                p = build_propagator(m, spec, engine="elastic")
+                p = build_propagator(m, spec, engine=engine)
                 assert (p._core.media.get("midx") is not None) == on
                 p.set_source(src)
                 p._batches["step"] = dev.Batch([p._core], variant=v)
@@ -138,3 +140,27 @@ def test_palette_migration_equals_float_media(kv, ring):
         for mode in ("pp", "ps"):
             assert np.abs(b[mode].image).max() > 0
             assert np.array_equal(a[mode].image, b[mode].image), mode
+
+
+def test_acoustic_palette_migration_equals_float_media():
+    """Acoustic zero_lag migration of a shot on a layered model with the
+    replay extraction and the receiver ring inside the step."""
+    from ertm.config import GpuConfig, ImagingConfig, SolverConfig
+    from ertm.imaging import RTMImager
+    from ertm.solver import build_solver
+    from ertm.types import ReceiverSpec, SourceSpec
+
+    rng = np.random.default_rng(47)
+    m = _layered_model(110, 150, rng, dx=5.0, dz=5.0)
+    solver = SolverConfig(nt=400, dt=4e-4, order=8, boundary="cpml", engine="acoustic", npml=12)
+    recv = ReceiverSpec(x=np.arange(0, 150, 2) * 5.0, z=np.full(75, 10.0))
+    shot = build_solver(solver, gpu_config=GpuConfig(strict_cfl=False)).forward(
+        m, SourceSpec(x=370.0, z=10.0, f0=25.0, delay=0.02), recv)
+    imaging = ImagingConfig(mode="pp", condition="zero_lag", checkpoint_mb=1.0)
+    outs = {}
+    for on in (True, False):
+        with _palette(on):
+            gpu = GpuConfig(strict_cfl=False, kernel_variant=(2, 19), image_ring=16)
+            outs[on] = RTMImager(imaging, solver, gpu_config=gpu).migrate(shot, m)
+    assert np.abs(outs[False].image).max() > 0
+    assert np.array_equal(outs[True].image, outs[False].image)

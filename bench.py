@@ -194,14 +194,17 @@ def serving_leg(n, scale, load, gcells_per_s, seed=2024):
             "errors": sorted({r.error for r in res if not r.ok})[:3]}
 
 
-def _serve(reqs, arrivals, lanes, streams=4):
+def _serve(reqs, arrivals, lanes, streams=4, tail_gb=None):
     """Submit ``reqs`` at ``arrivals`` (seconds after the start; None = all at
-    once) to one ServingLoop; returns its stats plus the makespan."""
+    once) to one ServingLoop; returns its stats plus the makespan.
+    ``tail_gb``: HBM a request's kept tail may fill (None: the loop's
+    default, the whole-block replay slots only)."""
     from ertm.config import GpuConfig
     from ertm.serving import ServingLoop
 
     with ServingLoop(hbm_budget_gb=200.0, streams=streams,
-                     gpu_config=GpuConfig(replay_budget_gb=4.0), lanes=lanes) as loop:
+                     gpu_config=GpuConfig(replay_budget_gb=4.0, tail_budget_gb=tail_gb),
+                     lanes=lanes) as loop:
         t0 = time.perf_counter()
         futs = []
         for k, req in enumerate(reqs):
@@ -216,11 +219,12 @@ def _serve(reqs, arrivals, lanes, streams=4):
     return stats
 
 
-def serving_sweep(n, scale, loads, lanes=True, streams=4, seed=2024):
+def serving_sweep(n, scale, loads, lanes=True, streams=4, seed=2024, tail_gb=None):
     """The mixed request stream's capacity and latency under load: first a
     closed loop (all ``n`` requests submitted at once; capacity = n / the
     makespan), then open-loop Poisson arrivals at each ``load`` x that
-    capacity, ``n`` requests per point.  One JSON line per point."""
+    capacity, ``n`` requests per point.  One JSON line per point.
+    ``tail_gb``: the requests' kept-tail budget (None: the loop's default)."""
     reqs = [serving_request(seed, i, scale)[0] for i in range(n)]
     mean_cells = float(np.mean([3 * r.solver.nt * r.model.shape[0] * r.model.shape[1]
                                 for r in reqs]))
@@ -228,7 +232,7 @@ def serving_sweep(n, scale, loads, lanes=True, streams=4, seed=2024):
 
     def line(kind, load, rate, st):
         out = {"metric": "serving_sweep", "mode": mode, "streams": streams, "kind": kind,
-               "load": load,
+               "load": load, "tail_budget_gb": tail_gb,
                "requests": n, "scale": scale, "offered_per_s": None if rate is None else
                round(rate, 4), "mean_cell_updates": mean_cells,
                "completed": st["completed"], "failed": st["failed"],
@@ -242,13 +246,13 @@ def serving_sweep(n, scale, loads, lanes=True, streams=4, seed=2024):
                "peak_reserved_gib": st.get("peak_reserved_gib"), "errors": st["errors"]}
         print(json.dumps(out), flush=True)
 
-    st = _serve(reqs, None, lanes, streams)
+    st = _serve(reqs, None, lanes, streams, tail_gb)
     capacity = st["completed"] / st["makespan_s"]
     line("closed_loop", None, None, st)
     for load in loads:
         rate = load * capacity
         gaps = np.random.default_rng([seed, int(load * 1000)]).exponential(1.0 / rate, n)
-        line("open_loop", load, rate, _serve(reqs, np.cumsum(gaps), lanes, streams))
+        line("open_loop", load, rate, _serve(reqs, np.cumsum(gaps), lanes, streams, tail_gb))
 
 
 def c5_model(seed=5):
@@ -748,6 +752,9 @@ def main(argv=None) -> int:
                          "closed-loop capacity, then Poisson arrivals at --serve-loads x that "
                          "capacity, --serve requests per point, one JSON line per point")
     ap.add_argument("--serve-loads", default="0.25,0.5,1,2,4")
+    ap.add_argument("--serve-tail-gb", type=float, default=None,
+                    help="--serve-sweep: HBM a request's kept tail may fill (default: only "
+                         "the whole-block replay slots it holds anyway)")
     ap.add_argument("--serve-groups", action="store_true",
                     help="serving: launch groups of one launch key only (no lanes)")
     ap.add_argument("--serve-streams", type=int, default=4,
@@ -842,7 +849,8 @@ def main(argv=None) -> int:
     if args.serve_sweep:
         serving_sweep(args.serve, args.serve_scale,
                       [float(v) for v in args.serve_loads.split(",")],
-                      lanes=not args.serve_groups, streams=args.serve_streams)
+                      lanes=not args.serve_groups, streams=args.serve_streams,
+                      tail_gb=args.serve_tail_gb)
         return 0
 
     if args.copy_probe_mib > 0:

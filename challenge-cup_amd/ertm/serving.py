@@ -249,7 +249,9 @@ class Admission:
                     blocked.add(kind)
                     self._drain_one(lane)
                     continue
-                self.pending.remove(item)
+                # by identity: the items hold requests whose dataclass
+                # equality compares their NumPy arrays
+                self.pending = deque(x for x in self.pending if x is not item)
                 self._reserve(item[2])
                 self.kind[w] = lane
                 self.count[w] += 1

@@ -4,6 +4,7 @@ same-key batching, and the request price."""
 from __future__ import annotations
 
 import numpy as np
+import pytest
 
 from ertm.config import GpuConfig, ImagingConfig, SolverConfig
 from ertm.serving import Admission, ShotRequest, request_bytes
@@ -145,3 +146,24 @@ def test_serving_admission_counts_kept_tails_only_inside_the_block_slots():
         pass
     else:
         raise AssertionError("a negative tail budget was accepted")
+
+
+def test_taking_a_lane_behind_a_waiting_request_does_not_compare_requests():
+    """A lane taken from behind a waiting request leaves the queue by
+    identity: the queued items hold ShotRequest dataclasses, whose equality
+    compares their records' NumPy arrays (an error for arrays of different
+    shapes).  Same schedule as the draining test above, with real requests
+    of different grids."""
+    reqs = {name: _request(64 + 4 * k, 80 + 8 * k, 300 + k,
+                           "acoustic" if name.startswith("a") else "elastic")
+            for k, name in enumerate(["e0", "e1", "a0", "e2", "e3"])}
+    with pytest.raises(ValueError):
+        bool(reqs["a0"] == reqs["e2"])   # what removing e2 with list.remove evaluated
+    adm = Admission(budget=100, streams=2, max_group=8)
+    for name, req in reqs.items():
+        adm.add((req, None, 10, "k" + name, 0.0, None, req.solver.engine))
+    got = [adm.take() for _ in range(3)]
+    # e2 is taken from behind the waiting a0
+    assert [(g[1], g[2][0][0]) for g in got] == [(0, reqs["e0"]), (1, reqs["e1"]),
+                                                 (1, reqs["e2"])]
+    assert [x[0] for x in adm.pending] == [reqs["a0"], reqs["e3"]]

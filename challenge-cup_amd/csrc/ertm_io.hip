@@ -440,20 +440,50 @@ void launch_stab_control(const ErtmProp*, const LaunchArgs& la, const float* pea
                        lowpass_now, scales, rscale);
 }
 
+// Clip scaling of the viscoacoustic state (ertm/propagation.py:740-747):
+// every float of p, vx, vz, eps and r_0..r_{L-1} times the shot's factor.
+// Four cells per thread as float4 (rows and field slabs are multiples of 64
+// floats), and all fields' loads issued before the stores: the slabs do not
+// overlap, so this is the same arithmetic as field after field, with up to
+// 4 + L loads in flight per thread instead of one.
 __global__ void k_stab_scale(const ErtmProp* __restrict__ props, LaunchArgs la,
                              const float* __restrict__ scales) {
     const int slot = la_prop(la, blockIdx.y);
     const float sc = scales[blockIdx.y];
     if (sc == 1.0f) return;
     const ErtmProp& P = props[slot];
-    float* S = P.state[la_cur(la, blockIdx.y)];
-    const int64_t cells = (int64_t)P.nzp * P.ld;
+    float* S1 = P.state[la_cur(la, blockIdx.y)];
     const int nf = 4 + P.nmech;  // p, vx, vz, eps, r_0..r_{L-1}
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cells;
+    if (((uintptr_t)S1 & 15) != 0 || (P.ld & 3) != 0 || (P.fstride & 3) != 0) {
+        // a caller's state that is not float4-aligned: one float at a time
+        const int64_t cells = (int64_t)P.nzp * P.ld;
+        for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cells;
+             q += (int64_t)gridDim.x * blockDim.x)
+            for (int f = 0; f < nf; ++f) {
+                float* x = S1 + (int64_t)f * P.fstride + q;
+                *x = *x * sc;
+            }
+        return;
+    }
+    float4* S = (float4*)S1;
+    const int64_t cells4 = (int64_t)P.nzp * P.ld / 4;
+    const int64_t fs4 = P.fstride / 4;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cells4;
          q += (int64_t)gridDim.x * blockDim.x) {
-        for (int f = 0; f < nf; ++f) {
-            float* x = S + (int64_t)f * P.fstride + q;
-            *x = *x * sc;
+        float4 v[4 + ERTM_MAX_MECH];
+#pragma unroll
+        for (int f = 0; f < 4 + ERTM_MAX_MECH; ++f)
+            if (f < nf) v[f] = S[f * fs4 + q];
+#pragma unroll
+        for (int f = 0; f < 4 + ERTM_MAX_MECH; ++f) {
+            if (f < nf) {
+                float4 w = v[f];
+                w.x = w.x * sc;
+                w.y = w.y * sc;
+                w.z = w.z * sc;
+                w.w = w.w * sc;
+                S[f * fs4 + q] = w;
+            }
         }
     }
 }

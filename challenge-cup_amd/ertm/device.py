@@ -581,7 +581,7 @@ class ImageBuffers:
 
     def __init__(self, nz: int, nx: int, separation: str, swave: str, rwaves, start_step: int,
                  illum: int, excitation: bool = False, rbuf=None, nslots: int = 0,
-                 ring: int = 0, ring_ld: int = 0):
+                 ring: int = 0, ring_ld: int = 0, rring=None):
         torch = torch_mod()
         self.s = L.ErtmImage()
         s = self.s
@@ -613,8 +613,12 @@ class ImageBuffers:
             # at column j + pad
             if ring_ld < nx:
                 raise ValueError("ring_ld must be the propagator's row pitch")
-            self.rring = torch.empty((int(ring), nrw, nz, int(ring_ld)), dtype=torch.float32,
-                                     device="cuda")
+            shape = (int(ring), nrw, nz, int(ring_ld))
+            if rring is None:
+                rring = torch.empty(shape, dtype=torch.float32, device="cuda")
+            elif tuple(rring.shape) != shape or rring.dtype != torch.float32:
+                raise ValueError(f"ring buffer of shape {tuple(rring.shape)}, expected {shape}")
+            self.rring = rring
             s.rring = L.ptr(self.rring)
             s.ring_len = int(ring)
         if excitation:

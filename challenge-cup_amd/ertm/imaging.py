@@ -755,6 +755,12 @@ class RTMImager:
             slots, kept = slot_plan(waves, nt, interval, room, self.gpu_config, cells)
             if kept is not None:
                 t_keep = kept
+            # the receiver-field rings before the replay slots: the step
+            # writes a new ring slot every backward step
+            ring = int(self.gpu_config.image_ring)
+            rrings = [torch.empty((ring, len(rwaves), m.shape[0], f._core.ld),
+                                  dtype=torch.float32, device="cuda")
+                      if ring > 0 and rwaves else None for m, f in zip(models, fwds)]
             rbufs = [torch.empty((slots, m.shape[0], m.shape[1]), dtype=torch.float32,
                                  device="cuda") for m in models]
             fimgs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, (), 0, 0,
@@ -829,10 +835,9 @@ class RTMImager:
             bb.close()
             accs = [(ib.num, ib.peak_amp ** 2, ib.rill) for ib in ibs]
         else:
-            ring = int(self.gpu_config.image_ring)
             ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, rwaves, st, illum,
-                                    rbuf=rb, nslots=slots, ring=ring, ring_ld=bc.ld)
-                   for m, st, rb, bc in zip(models, starts, rbufs, bcores)]
+                                    rbuf=rb, nslots=slots, ring=ring, ring_ld=bc.ld, rring=rr)
+                   for m, st, rb, bc, rr in zip(models, starts, rbufs, bcores, rrings)]
             fb = dev.Batch(fcores, imgs=fimgs, variant=kv, graph_steps=gsteps, two_step=ts)
             bb = dev.Batch(bcores, recvs=tabs, imgs=ibs, variant=kv, graph_steps=gsteps)
             subs = None

@@ -791,6 +791,8 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain float32 copy kernel (out = in * 1) over this many "
                          "MiB, three times: the reference reading for HBM byte counters")
+    ap.add_argument("--overlap-flush", action="store_true",
+                    help="fold the receiver ring on a stream of its own beside the backward steps")
     ap.add_argument("--image-ring", type=int, default=None,
                     help="GpuConfig.image_ring for the migrations (default: the config's)")
     ap.add_argument("--graph-steps", type=int, default=None,
@@ -879,6 +881,8 @@ def main(argv=None) -> int:
         gkw["two_step"] = tuple(int(v) for v in args.two_step.split(","))
     if args.image_ring is not None:
         gkw["image_ring"] = args.image_ring
+    if args.overlap_flush:
+        gkw["overlap_flush"] = True
     imager = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False,
                                                              replay_budget_gb=200.0,
                                                              kernel_variant=kv, **gkw))

@@ -187,6 +187,12 @@ struct ertm_batch_s {
     float** dark = nullptr;
     std::map<std::vector<int64_t>, hipGraphExec_t> graphs;
     hipStream_t cap = nullptr;  // private stream the loops are captured on
+    // ring folds beside the backward steps (ertm_batch_set_ring_flush): every
+    // flush_period steps, on a private stream; fl_done[w % 2] marks fold w
+    int flush_period = 0;
+    hipStream_t fls = nullptr;
+    hipEvent_t fl_step = nullptr, fl_done[2] = {nullptr, nullptr};
+    int fl_count = 0;
     void note(hipStream_t s) {
         for (hipStream_t t : streams)
             if (t == s) return;
@@ -195,6 +201,10 @@ struct ertm_batch_s {
     ~ertm_batch_s() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         if (cap) (void)hipStreamDestroy(cap);
+        if (fl_step) (void)hipEventDestroy(fl_step);
+        for (hipEvent_t e : fl_done)
+            if (e) (void)hipEventDestroy(e);
+        if (fls) (void)hipStreamDestroy(fls);
         pool_put(dclk, 64);
         pool_put(rscale, rscale_bytes);
         pool_put(damp, damp_bytes);
@@ -414,6 +424,18 @@ int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc) {
     if (zc < 8 || zc > 4096) return fail(ERTM_EINVAL, "z chunk %d out of [8, 4096]", zc);
     b->variant = variant;
     b->zc = zc;
+    return ERTM_OK;
+}
+
+int ertm_batch_set_ring_flush(ertm_batch_t b, int32_t period) {
+    if (!b) return fail(ERTM_EINVAL, "set_ring_flush: NULL batch");
+    if (period < 0) return fail(ERTM_EINVAL, "set_ring_flush: period %d < 0", period);
+    if (period > 0 && !b->fls) {
+        HIPCHK(hipStreamCreateWithFlags(&b->fls, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&b->fl_step, hipEventDisableTiming));
+        for (hipEvent_t& e : b->fl_done) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    b->flush_period = period;
     return ERTM_OK;
 }
 
@@ -750,6 +772,23 @@ static int do_flush(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t 
         // cells: the physical cells of the flushed steps
         if (int rc = tm.done(ERTM_TK_FLUSH, count * cells_of(b, la))) return rc;
     }
+    return ERTM_OK;
+}
+
+// One ring fold on the batch's flush stream, beside the backward steps: it
+// starts after the steps queued on s so far (they wrote its slots), and s
+// waits for the previous fold before the steps that follow, which overwrite
+// that fold's slots.  Folds run in order on their stream, so the
+// accumulators see them in step order.
+static int flush_beside(ertm_batch_t b, const int32_t* t_hi, int count, hipStream_t s,
+                        const float* rscale) {
+    b->note(b->fls);
+    HIPCHK(hipEventRecord(b->fl_step, s));
+    HIPCHK(hipStreamWaitEvent(b->fls, b->fl_step, 0));
+    if (b->fl_count > 0) HIPCHK(hipStreamWaitEvent(s, b->fl_done[(b->fl_count - 1) & 1], 0));
+    if (int rc = do_flush(b, t_hi, count, b->fls, nullptr, nullptr, rscale)) return rc;
+    HIPCHK(hipEventRecord(b->fl_done[b->fl_count & 1], b->fls));
+    ++b->fl_count;
     return ERTM_OK;
 }
 
@@ -1261,6 +1300,12 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
     // has rewritten the slot) makes the flushed value the stabilised one.
     const bool use_ring = r->phase == 2 && r->op_after == ERTM_IMG_CORRELATE && ring_ok(b);
     const int ring_len = use_ring ? b->img[0].ring_len : 0;
+    // folds beside the steps (eager loops without the stabiliser, a ring of
+    // at least two fold periods), else every ring_len steps on s
+    const bool beside = use_ring && !r->stabilize && r->graph_steps == 0 && b->fls &&
+                        b->flush_period > 0 && ring_len >= 2 * b->flush_period;
+    const int fold = beside ? b->flush_period : ring_len;
+    b->fl_count = 0;
     float* rscale = nullptr;
     if (use_ring && r->stabilize) {
         const size_t bytes = sizeof(float) * (size_t)ring_len * n;
@@ -1314,9 +1359,10 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
         // the flush comes after the stabiliser, whose factors it reads
         if (use_ring) {
             if (pending == 0) t_hi.assign(tt.begin(), tt.end());
-            if (++pending == ring_len || (!dc && q == nsteps - 1)) {
-                if ((rc2 = do_flush(b, t_hi.data(), pending, s, dc, nullptr, rscale)) != ERTM_OK)
-                    return rc2;
+            if (++pending == fold || (!dc && q == nsteps - 1)) {
+                rc2 = beside ? flush_beside(b, t_hi.data(), pending, s, rscale)
+                             : do_flush(b, t_hi.data(), pending, s, dc, nullptr, rscale);
+                if (rc2 != ERTM_OK) return rc2;
                 pending = 0;
             }
         }
@@ -1481,6 +1527,9 @@ int ertm_run(ertm_batch_t b, const ErtmRun* r, int32_t* cur_out, void* stream) {
         }
         q += 2;
     }
+    // what s runs next (a replay overwriting the slots, reads of the
+    // accumulators) comes after the last fold
+    if (beside && b->fl_count > 0) HIPCHK(hipStreamWaitEvent(s, b->fl_done[(b->fl_count - 1) & 1], 0));
     if (cur_out) *cur_out = cur;
     return ERTM_OK;
 }

@@ -126,6 +126,7 @@ EXPORTS = [
     ("ertm_batch_destroy", C.c_int, [C.c_void_p]),
     ("ertm_batch_set_variant", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     ("ertm_batch_set_two_step", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("ertm_batch_set_ring_flush", C.c_int, [C.c_void_p, C.c_int32]),
     ("ertm_step", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.c_void_p]),
     ("ertm_stabilize", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                  C.c_void_p, C.c_void_p]),

@@ -205,6 +205,14 @@ class GpuConfig:
                    first such block start in one run carried on to nt, so its
                    checkpointing pass ends at that block's checkpoint.  Same
                    results.
+    overlap_flush  fold the receiver-field ring into the image accumulators
+                   on a stream of its own while the next backward steps run
+                   (the ring then holds 2 x image_ring steps; eager loops
+                   only, not the compensated viscoacoustic pass, whose folds
+                   follow its stabiliser).  Same results; off by default:
+                   on 64 elastic shots the backward steps ran about as much
+                   slower beside the folds as the folds had taken between
+                   them.
     tail_budget_gb HBM the kept tail may fill: None = the replay budget (a
                    batch migration's default); 0 = no more than the
                    whole-block replay slots the migration holds anyway (the
@@ -224,6 +232,7 @@ class GpuConfig:
     two_step: tuple | None = None
     device_resample: bool = True
     keep_tail: bool = True
+    overlap_flush: bool = False
     tail_budget_gb: float | None = None
 
     def validate(self) -> None:

@@ -680,6 +680,14 @@ class Batch:
                 "ertm_batch_set_two_step")
         self.two_step = (int(variant), int(zc))
 
+    def set_ring_flush(self, period: int) -> None:
+        """Fold the receiver-field ring every ``period`` backward steps on a
+        stream of the library's own, beside the next steps
+        (ertm_batch_set_ring_flush); 0 = fold on the launching stream when
+        the ring is full."""
+        L.check(L.lib().ertm_batch_set_ring_flush(self.h, int(period)),
+                "ertm_batch_set_ring_flush")
+
     def set_variant(self, variant: int, zc: int = 64) -> None:
         L.check(L.lib().ertm_batch_set_variant(self.h, int(variant), int(zc)),
                 "ertm_batch_set_variant")

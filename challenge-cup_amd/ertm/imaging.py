@@ -758,7 +758,12 @@ class RTMImager:
             # the receiver-field rings before the replay slots: the step
             # writes a new ring slot every backward step
             ring = int(self.gpu_config.image_ring)
-            rrings = [torch.empty((ring, len(rwaves), m.shape[0], f._core.ld),
+            # folds beside the backward steps: a ring of two fold periods
+            compensated = engine == "viscoacoustic" and bool(self.q_config.enabled)
+            fold_beside = (self.gpu_config.overlap_flush and ring > 0 and bool(rwaves)
+                           and gsteps == 0 and not compensated)
+            ring_len = 2 * ring if fold_beside else ring
+            rrings = [torch.empty((ring_len, len(rwaves), m.shape[0], f._core.ld),
                                   dtype=torch.float32, device="cuda")
                       if ring > 0 and rwaves else None for m, f in zip(models, fwds)]
             rbufs = [torch.empty((slots, m.shape[0], m.shape[1]), dtype=torch.float32,
@@ -836,10 +841,12 @@ class RTMImager:
             accs = [(ib.num, ib.peak_amp ** 2, ib.rill) for ib in ibs]
         else:
             ibs = [dev.ImageBuffers(m.shape[0], m.shape[1], separation, swave, rwaves, st, illum,
-                                    rbuf=rb, nslots=slots, ring=ring, ring_ld=bc.ld, rring=rr)
+                                    rbuf=rb, nslots=slots, ring=ring_len, ring_ld=bc.ld, rring=rr)
                    for m, st, rb, bc, rr in zip(models, starts, rbufs, bcores, rrings)]
             fb = dev.Batch(fcores, imgs=fimgs, variant=kv, graph_steps=gsteps, two_step=ts)
             bb = dev.Batch(bcores, recvs=tabs, imgs=ibs, variant=kv, graph_steps=gsteps)
+            if fold_beside:
+                bb.set_ring_flush(ring)
             subs = None
             if slots < store0.interval:
                 nsub = int(math.ceil(store0.interval / slots))

@@ -265,6 +265,17 @@ int ertm_batch_set_variant(ertm_batch_t b, int32_t variant, int32_t zc);
  * steps (the CPML part of a state buffer per propagator) on first use.
  * Results are bit-identical to one step per launch. */
 int ertm_batch_set_two_step(ertm_batch_t b, int32_t variant, int32_t zc);
+/* Folds of the receiver-field ring beside the backward steps: no counterpart
+ * in the reference (its migrate correlates every step, ertm/imaging.py:
+ * 279-324).  With period > 0, ertm_run's eager backward passes with the ring
+ * and without the stabiliser fold every `period` steps on a stream of the
+ * library's own, while the next steps run on the caller's stream, when every
+ * image's ring_len holds at least 2 * period steps (else they fold every
+ * ring_len steps on the caller's stream, as with period = 0).  Events order
+ * the slots' reuse and the folds themselves, and the caller's stream waits
+ * for the last fold before ertm_run returns: the accumulators get the same
+ * additions in the same order.  Creates the stream on first use. */
+int ertm_batch_set_ring_flush(ertm_batch_t b, int32_t period);
 
 /*
  * Per-launch time table: propagator i of the batch is at time index t[i] and

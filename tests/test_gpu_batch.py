@@ -245,3 +245,36 @@ def test_pipelined_batches_on_a_caller_stream_equal_default_stream():
             for m in ("pp", "ps"):
                 assert np.array_equal(g[m].image, r[m].image)
                 assert g[m].meta == r[m].meta
+
+
+@pytest.mark.parametrize("ring", [1, 7, 16])
+@pytest.mark.parametrize("tag", ["el_pp_divcurl", "el_ps_stress", "ac_normalized", "ac_raw_rn"])
+def test_ring_folds_beside_the_steps_equal_folds_between_them(tag, ring):
+    """GpuConfig.overlap_flush: the receiver ring (2 x ring steps) is folded
+    every ``ring`` steps on a stream of the library's own while the next
+    backward steps run.  Each image equals the golden image and the one the
+    folds on the launching stream give, bit for bit, alone and for a batch
+    of three shots imaging PP and PS."""
+    _say("folds beside the steps", tag, ring)
+    from ertm.config import GpuConfig
+    from ertm.imaging import RTMImager
+
+    c = migrate_case(tag)
+    shot, model, imaging, solver, q = migrate_objects(c)
+    outs = {}
+    for beside in (True, False):
+        gpu = GpuConfig(kernel_variant=(2, 19), image_ring=ring, overlap_flush=beside)
+        outs[beside] = RTMImager(imaging, solver, q, gpu_config=gpu).migrate(shot, model)
+    assert np.array_equal(outs[True].image, c["image"])
+    assert np.array_equal(outs[True].image, outs[False].image)
+    if c["engine"] == "elastic":
+        shots = [shot, _moved(shot, 2 * model.grid.dx), _moved(shot, -3 * model.grid.dx)]
+        got = {}
+        for beside in (True, False):
+            gpu = GpuConfig(kernel_variant=(2, 19), image_ring=ring, overlap_flush=beside)
+            got[beside] = RTMImager(imaging, solver, q, gpu_config=gpu).migrate_batch(
+                shots, model, modes=("pp", "ps"))
+        for a, b in zip(got[True], got[False]):
+            for mode in ("pp", "ps"):
+                assert np.abs(b[mode].image).max() > 0
+                assert np.array_equal(a[mode].image, b[mode].image), mode

@@ -425,7 +425,8 @@ def c5_sweep(budgets_mb, nt, kernel_variant=None, conditions=("zero_lag", "excit
     steps / nt (recompute overhead), time and RTM cell-update rate; plus a
     check that segment replay from sub-checkpoints gives the same image.
     ``replay_gb`` is the HBM for replay slots (48 MB per step of this grid),
-    capped at three quarters of the HBM free after the checkpoints: the
+    capped at nine tenths of the HBM free after the checkpoints, rings, states
+    and accumulators: the
     default 200 GB holds the longest block (2500 steps at the 1 GB
     checkpoint budget), so every block is replayed once, and zero_lag
     migrations keep the source fields of the last ~4000 steps of their
@@ -884,7 +885,7 @@ def main(argv=None) -> int:
     if args.overlap_flush:
         gkw["overlap_flush"] = True
     imager = RTMImager(imaging, solver, gpu_config=GpuConfig(strict_cfl=False,
-                                                             replay_budget_gb=200.0,
+                                                             replay_budget_gb=240.0,
                                                              kernel_variant=kv, **gkw))
     t0 = time.perf_counter()
     job = imager.prepare_batch(shots, model, modes=("pp", "ps"))

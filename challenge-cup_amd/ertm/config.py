@@ -160,7 +160,9 @@ class GpuConfig:
     replay_slots   maximum image-field slots held per replay segment
                    (0 = one whole checkpoint block).
     replay_budget_gb  HBM for the replay slots of the shots migrated together
-                   (capped at three quarters of the free HBM, counting what
+                   (capped at nine tenths of the HBM left free by the
+                   checkpoints, receiver rings, backward states and
+                   accumulators of the batch, counting what
                    torch's allocator holds unused); over it a block is
                    replayed in segments from sub-checkpoints.
     max_batch      most shots advanced together by ``migrate_batch``.

@@ -334,10 +334,11 @@ def record_source_wavefields(props: list[Propagator2D], wavelets: list[np.ndarra
 
 _SIDE_STREAMS: dict = {}
 _SIDE_LOCK = threading.Lock()
-# Share of the free HBM the replay slots of one batch may take: what the
-# migration allocates after them (accumulators, sub-checkpoints, FFT buffers
-# of the stabiliser, post-processing) is a few per cent of the slots.
-_REPLAY_FREE_SHARE = 0.75
+# Share of the free HBM the replay slots of one batch may take once the
+# checkpoints, rings, backward states and accumulators are set aside (the
+# rest covers sub-checkpoints, the stabiliser's FFT buffers, receiver tables,
+# post-processing and the caching allocator's rounding).
+_REPLAY_FREE_SHARE = 0.9
 
 
 def side_stream(role: str, launch=None):
@@ -744,12 +745,27 @@ class RTMImager:
             cells = sum(m.shape[0] * m.shape[1] for m in models)
             arena_bytes = (4 * len(range(0, nt, interval))
                            * max(f._core.state_floats for f in fwds) * n)
+            ring = int(self.gpu_config.image_ring)
+            # folds beside the backward steps: a ring of two fold periods
+            compensated = engine == "viscoacoustic" and bool(self.q_config.enabled)
+            fold_beside = (self.gpu_config.overlap_flush and ring > 0 and bool(rwaves)
+                           and gsteps == 0 and not compensated)
+            ring_len = 2 * ring if fold_beside else ring
+            # what the migration allocates beside the slots: its checkpoints,
+            # the receiver rings, the backward states, and the float64
+            # accumulators twice (the previous batch's are copied to the
+            # host while this one runs)
+            beside = arena_bytes + sum(
+                4 * ring_len * len(rwaves) * m.shape[0] * f._core.ld
+                + 8 * f._core.state_floats
+                + 2 * 8 * (2 * len(rwaves) + 1) * m.shape[0] * m.shape[1]
+                for m, f in zip(models, fwds))
             # Later batches of a shape take no more slots than the first: a
             # larger request than the block the caching allocator holds from
             # the previous batch makes it free its cache (a device
             # synchronisation) and map the slots anew.
             key = (nt, interval, tuple((m.shape[0], m.shape[1]) for m in models))
-            room = min(self._slot_room(cells, reserve=arena_bytes),
+            room = min(self._slot_room(cells, reserve=beside),
                        self._rooms.get(key, 1 << 62))
             self._rooms[key] = room
             slots, kept = slot_plan(waves, nt, interval, room, self.gpu_config, cells)
@@ -757,12 +773,6 @@ class RTMImager:
                 t_keep = kept
             # the receiver-field rings before the replay slots: the step
             # writes a new ring slot every backward step
-            ring = int(self.gpu_config.image_ring)
-            # folds beside the backward steps: a ring of two fold periods
-            compensated = engine == "viscoacoustic" and bool(self.q_config.enabled)
-            fold_beside = (self.gpu_config.overlap_flush and ring > 0 and bool(rwaves)
-                           and gsteps == 0 and not compensated)
-            ring_len = 2 * ring if fold_beside else ring
             rrings = [torch.empty((ring_len, len(rwaves), m.shape[0], f._core.ld),
                                   dtype=torch.float32, device="cuda")
                       if ring > 0 and rwaves else None for m, f in zip(models, fwds)]
